@@ -1,0 +1,199 @@
+"""Problem description for residual/tangent assembly of C1 quadratic B-spline
+gradient elasticity on a structured (tensor-product) patch.
+
+Mirrors what the reference's structs carry for this path:
+  Nurbs (include/nurbs.h:4-24): porder 2, nelem[3], nbasis[3], nknot[3],
+      knotVector[3], nboun/boun
+  Phys  (include/phys.h:11-26): ndof 3, torder 0 (ge) / 2 (getime), par_mat[9]
+  BC    (include/bc.h:4-11): bc_type[36], par_periodic[9], par_dirichlet[9],
+      par_neumann
+plus what the device path adds: nquad (the reference hard-codes 4,
+src/snesiga_Residual.c:99) and optional per-element material coefficients.
+
+Material coefficients: W is linear in coef = (A36, A37, A38, e0, e345, le^2);
+the reference computes the first three from par_mat r, curv
+(phys/ge/residual.c:45-47).
+"""
+from dataclasses import dataclass, field
+from typing import Callable, Optional
+
+import numpy as np
+
+PORDER = 2
+NDOF = 3
+GE = 0
+GETIME = 1
+
+# r, curv, e0, e345, le, rho, cii, dt, t  (phys/ge/init.c:18, phys/getime/init.c:18)
+PAR_MAT_REFERENCE = np.array([0.25, 0.50, 500., 250., 0.025, 1.0, 1.0, 1.e-3 * (1. / 2.), 0.0])
+
+
+def uniform_knots(nelem):
+    """Open uniform knots on [0,1] with p = 2 (nurbs_genuniform.c:34-38), any nelem."""
+    k = np.empty(nelem + 1 + 2 * PORDER)
+    k[:PORDER + 1] = 0.0
+    for i in range(PORDER + 1, nelem + PORDER):
+        k[i] = (i - PORDER) * (1.0 / nelem)
+    k[nelem + PORDER:] = 1.0
+    return k
+
+
+def coef_from_par(par):
+    """(A36, A37, A38, e0, e345, le^2) from the reference par_mat."""
+    r, c = par[0], par[1]
+    return np.array([4.0 * (1.0 - c) / (r * r * r * (1.0 + 2.0 * c)),
+                     -6.0 * c / (r * r * (1.0 + 2.0 * c)),
+                     3.0 / (r * r * r * r * (1.0 + 2.0 * c)),
+                     par[2], par[3], par[4] * par[4]])
+
+
+def coef_from_lame(lam, mu, ell):
+    """St-Venant-Kirchhoff strain-gradient variant: W = (3 lam/2 + mu) e1^2 +
+    mu (e2^2 + e3^2) + 2 mu (E01^2 + E02^2 + E12^2) + l^2 |grad dev|^2.
+    Vectorised over arrays of lam, mu, ell.  (Not reference physics: the
+    reference only has the par_mat double-well.)"""
+    lam, mu, ell = np.broadcast_arrays(np.asarray(lam, float), np.asarray(mu, float), np.asarray(ell, float))
+    z = np.zeros_like(lam)
+    return np.stack([z, mu, z, 1.5 * lam + mu, 2.0 * mu, ell * ell], axis=-1)
+
+
+def splitmix64(seed, n):
+    """n uint64 draws of splitmix64 starting from state `seed`."""
+    M = (1 << 64) - 1
+    out = np.empty(n, np.uint64)
+    with np.errstate(over="ignore"):
+        idx = np.arange(1, n + 1, dtype=np.uint64)
+        z = (np.uint64(seed & M) + idx * np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        out[:] = z ^ (z >> np.uint64(31))
+    return out
+
+
+def uniform01(seed, n):
+    """Uniform doubles in [0,1) from the top 53 bits of splitmix64."""
+    return (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+@dataclass
+class Problem:
+    nelem: tuple
+    knots: list
+    physics: int = GE
+    nquad: int = 4
+    par_mat: np.ndarray = field(default_factory=lambda: PAR_MAT_REFERENCE.copy())
+    elem_coef: Optional[np.ndarray] = None
+    boun: tuple = (0, 1, 2, 3, 4, 5)
+    bc_type: np.ndarray = field(default_factory=lambda: np.ones(36, np.int32))
+    par_dirichlet: np.ndarray = field(default_factory=lambda: np.zeros(9))
+    par_periodic: np.ndarray = field(default_factory=lambda: np.zeros(9))
+    par_neumann: Optional[np.ndarray] = None
+    neumann: Optional[Callable] = None
+
+    @property
+    def torder(self):
+        return 2 if self.physics == GETIME else 0
+
+    @property
+    def nhist(self):
+        return self.torder + 1
+
+    @property
+    def nbasis(self):
+        return tuple(n + PORDER for n in self.nelem)
+
+    @property
+    def n_elem(self):
+        return int(self.nelem[0]) * int(self.nelem[1]) * int(self.nelem[2])
+
+    @property
+    def ndof(self):
+        nb = self.nbasis
+        return NDOF * nb[0] * nb[1] * nb[2]
+
+    @property
+    def nnz(self):
+        def s(nb):
+            return sum(min(a + PORDER, nb - 1) - max(a - PORDER, 0) + 1 for a in range(nb))
+        nb = self.nbasis
+        return NDOF * NDOF * s(nb[0]) * s(nb[1]) * s(nb[2])
+
+    def coef(self):
+        """Per-element coefficients (n_elem, 6); the reference's global ones if unset."""
+        if self.elem_coef is not None:
+            return np.ascontiguousarray(self.elem_coef, dtype=np.float64)
+        return np.tile(coef_from_par(self.par_mat), (self.n_elem, 1))
+
+    def greville(self):
+        """Greville abscissae per direction (control-point positions of x)."""
+        return [0.5 * (k[1:-2] + k[2:-1]) for k in self.knots]
+
+
+def bc_ex1():
+    """example/ex1/main.c:73-95: all faces u = 0 (standard Dirichlet),
+    high-order Neumann with zero traction."""
+    bc = np.zeros(36, np.int32)
+    for f in range(6):
+        bc[6 * f + 3:6 * f + 6] = 1
+    return bc
+
+
+def bc_clamp_stretch():
+    """Face 0 clamped (standard and high-order Dirichlet), face 1 u_x
+    prescribed with free tangential motion and high-order Neumann, faces 2-5
+    Neumann (zero traction)."""
+    bc = np.ones(36, np.int32)
+    bc[0:6] = 0             # face 0: ux, uy, uz and their normal gradients
+    bc[6 + 0] = 0           # face 1: ux Dirichlet; uy, uz and high order Neumann
+    return bc
+
+
+def bc_all_clamped():
+    return np.zeros(36, np.int32)
+
+
+def reference_example(n, physics=GE, nquad=4):
+    """The reference's example (ex1): unit cube, n^3 elements, all faces
+    standard-Dirichlet zero, high-order Neumann zero, reference par_mat."""
+    return Problem(nelem=(n, n, n), knots=[uniform_knots(n) for _ in range(3)], physics=physics,
+                   nquad=nquad, bc_type=bc_ex1())
+
+
+def stretch_problem(n, a=0.01, lam=1.0, mu=0.5, ell=0.1, noise=0.0, seed=0x5EED0002, nquad=4,
+                    material="lame"):
+    """Clamped at x=0, u_x = a X imposed at x=1 (par_dirichlet[0] = a), the
+    rest traction free.  material='lame' maps (lam, mu, ell) per element with
+    optional +-noise relative perturbation; material='reference' uses par_mat."""
+    n3 = (n, n, n) if np.isscalar(n) else tuple(n)
+    pb = Problem(nelem=n3, knots=[uniform_knots(m) for m in n3], nquad=nquad, bc_type=bc_clamp_stretch())
+    pb.par_dirichlet = np.zeros(9)
+    pb.par_dirichlet[0] = a
+    if material == "lame":
+        ne = pb.n_elem
+        if noise:
+            r = uniform01(seed, 2 * ne).reshape(2, ne)
+            lam_e = lam * (1.0 + noise * (2.0 * r[0] - 1.0))
+            mu_e = mu * (1.0 + noise * (2.0 * r[1] - 1.0))
+        else:
+            lam_e = np.full(ne, lam)
+            mu_e = np.full(ne, mu)
+        pb.elem_coef = coef_from_lame(lam_e, mu_e, np.full(ne, ell))
+    return pb
+
+
+def smooth_state(pb, amp=1e-3, noise=1e-6, seed=0x5EED0012, nhist=None):
+    """Non-equilibrium displacement at the control points:
+    U_i = amp sin(pi X) sin(pi Y) sin(pi Z) (1, .5, .25)_i + noise U(-1,1),
+    in reference layout (dof fastest, then z, y, x).  History slices (getime)
+    are perturbed copies."""
+    g = pb.greville()
+    sx, sy, sz = (np.sin(np.pi * gi) for gi in g)
+    base = amp * (sx[:, None, None] * sy[None, :, None] * sz[None, None, :])
+    U = base[..., None] * np.array([1.0, 0.5, 0.25])
+    U = U.reshape(-1)
+    nh = pb.nhist if nhist is None else nhist
+    out = np.empty(nh * U.size)
+    r = uniform01(seed, nh * U.size)
+    for h in range(nh):
+        out[h * U.size:(h + 1) * U.size] = U * (1.0 - 0.1 * h) + noise * (2.0 * r[h * U.size:(h + 1) * U.size] - 1.0)
+    return out

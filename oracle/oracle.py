@@ -1,0 +1,355 @@
+"""ctypes front end of the CPU oracle (liboracle.so) and of the reference's own
+libm-only sources built into oracle/_ref (libigap4ref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg; the product never imports this module.
+
+A "problem" is any object with the attributes
+    nelem (3 ints), knots (3 float64 arrays of nelem[d]+5), nquad, physics
+    (0 ge / 1 getime), par_mat (9), elem_coef (None or (n_elem, 6)), boun,
+    bc_type (36), par_dirichlet (9), par_periodic (9), par_neumann,
+    neumann (None or a Python callable t(face, order, idof, xq) -> float)
+which is what igap4standard_amd.problem.Problem provides.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libigap4ref.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+NEUMANN_FN = ctypes.CFUNCTYPE(None, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp)
+
+
+class OrProblem(ctypes.Structure):
+    _fields_ = [
+        ("nelem", ctypes.c_int * 3),
+        ("knot", _dp * 3),
+        ("nquad", ctypes.c_int),
+        ("physics", ctypes.c_int),
+        ("par_mat", _dp),
+        ("elem_coef", _dp),
+        ("nboun", ctypes.c_int),
+        ("boun", _ip),
+        ("bc_type", _ip),
+        ("par_dirichlet", _dp),
+        ("par_periodic", _dp),
+        ("par_neumann", _dp),
+        ("neumann", NEUMANN_FN),
+    ]
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.or_evalN.restype = ctypes.c_double
+        L.or_ge_energy.restype = ctypes.c_double
+        L.or_ndof.restype = ctypes.c_int64
+        L.or_nnz.restype = ctypes.c_int64
+        L.or_energy.restype = ctypes.c_double
+        L.or_tangent_elements.argtypes = [ctypes.POINTER(OrProblem), _dp, _i64p, _i32p, _dp,
+                                          ctypes.c_int64, ctypes.c_int64]
+        L.or_residual_elements.argtypes = [ctypes.POINTER(OrProblem), _dp, _dp, ctypes.c_int64,
+                                           ctypes.c_int64]
+        L.or_evalN.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_double]
+        _lib = L
+    return _lib
+
+
+class Bound:
+    """A problem bound to the C struct; keeps every referenced array alive."""
+
+    def __init__(self, pb):
+        self.pb = pb
+        self.knots = [np.ascontiguousarray(k, dtype=np.float64) for k in pb.knots]
+        self.par_mat = np.ascontiguousarray(pb.par_mat, dtype=np.float64)
+        self.elem_coef = None if pb.elem_coef is None else np.ascontiguousarray(pb.elem_coef, dtype=np.float64)
+        self.boun = np.ascontiguousarray(pb.boun, dtype=np.int32)
+        self.bc_type = np.ascontiguousarray(pb.bc_type, dtype=np.int32)
+        self.par_dirichlet = np.ascontiguousarray(pb.par_dirichlet, dtype=np.float64)
+        self.par_periodic = np.ascontiguousarray(pb.par_periodic, dtype=np.float64)
+        self.par_neumann = np.ascontiguousarray(
+            pb.par_neumann if pb.par_neumann is not None else np.zeros(1), dtype=np.float64)
+        s = OrProblem()
+        for d in range(3):
+            s.nelem[d] = int(pb.nelem[d])
+            s.knot[d] = _d(self.knots[d])
+        s.nquad = int(pb.nquad)
+        s.physics = int(pb.physics)
+        s.par_mat = _d(self.par_mat)
+        s.elem_coef = _d(self.elem_coef) if self.elem_coef is not None else _dp()
+        s.nboun = len(self.boun)
+        s.boun = self.boun.ctypes.data_as(_ip)
+        s.bc_type = self.bc_type.ctypes.data_as(_ip)
+        s.par_dirichlet = _d(self.par_dirichlet)
+        s.par_periodic = _d(self.par_periodic)
+        s.par_neumann = _d(self.par_neumann)
+        if pb.neumann is not None:
+            fn = pb.neumann
+
+            def cb(t, par, face, order, idof, xq):
+                t[0] = float(fn(face, order, idof, (xq[0], xq[1], xq[2])))
+
+            self._cb = NEUMANN_FN(cb)
+            s.neumann = self._cb
+        else:
+            s.neumann = NEUMANN_FN()
+        self.s = s
+
+    @property
+    def ptr(self):
+        return ctypes.byref(self.s)
+
+
+def ndof(pb):
+    return int(lib().or_ndof(Bound(pb).ptr))
+
+
+def nnz(pb):
+    return int(lib().or_nnz(Bound(pb).ptr))
+
+
+def pattern(pb):
+    b = Bound(pb)
+    n, z = lib().or_ndof(b.ptr), lib().or_nnz(b.ptr)
+    rowptr = np.zeros(n + 1, np.int64)
+    col = np.zeros(z, np.int32)
+    lib().or_pattern(b.ptr, rowptr.ctypes.data_as(_i64p), col.ctypes.data_as(_i32p))
+    return rowptr, col
+
+
+def dirichlet_rows(pb):
+    b = Bound(pb)
+    n = lib().or_dirichlet_count(b.ptr)
+    rows = np.zeros(max(n, 1), np.int32)
+    m = lib().or_dirichlet_rows(b.ptr, rows.ctypes.data_as(_i32p))
+    assert m == n
+    return rows[:n]
+
+
+def residual(pb, U_hist):
+    b = Bound(pb)
+    U = np.ascontiguousarray(U_hist, dtype=np.float64)
+    R = np.zeros(lib().or_ndof(b.ptr))
+    rc = lib().or_residual(b.ptr, _d(U), _d(R))
+    if rc:
+        raise RuntimeError("or_residual failed: %d" % rc)
+    return R
+
+
+def tangent(pb, U_hist, rowptr=None, col=None):
+    b = Bound(pb)
+    if rowptr is None:
+        rowptr, col = pattern(pb)
+    U = np.ascontiguousarray(U_hist, dtype=np.float64)
+    vals = np.zeros(len(col))
+    rc = lib().or_tangent(b.ptr, _d(U), rowptr.ctypes.data_as(_i64p), col.ctypes.data_as(_i32p), _d(vals))
+    if rc:
+        raise RuntimeError("or_tangent failed: %d" % rc)
+    return rowptr, col, vals
+
+
+def energy(pb, U_hist):
+    b = Bound(pb)
+    U = np.ascontiguousarray(U_hist, dtype=np.float64)
+    return float(lib().or_energy(b.ptr, _d(U)))
+
+
+def gauss(n):
+    x, w = np.zeros(n), np.zeros(n)
+    if lib().or_gauss(n, _d(x), _d(w)):
+        raise ValueError(n)
+    return x, w
+
+
+def eval_bspline(windows, xq):
+    """windows: three 6-knot arrays; xq: 3 coords -> (27, 10)."""
+    ws = [np.ascontiguousarray(w, dtype=np.float64) for w in windows]
+    arr = (_dp * 3)(*[_d(w) for w in ws])
+    N = np.zeros(270)
+    x = np.ascontiguousarray(xq, dtype=np.float64)
+    lib().or_eval_bspline(_d(N), arr, _d(x))
+    return N.reshape(27, 10)
+
+
+def evalN(kv, k, i, p, xi):
+    kv = np.ascontiguousarray(kv, dtype=np.float64)
+    return float(lib().or_evalN(_d(kv), len(kv), k, i, p, xi))
+
+
+def coef_from_par(par):
+    c = np.zeros(6)
+    lib().or_coef_from_par(_d(np.ascontiguousarray(par, dtype=np.float64)), _d(c))
+    return c
+
+
+def ge_energy(u, coef):
+    return float(lib().or_ge_energy(_d(np.ascontiguousarray(u, dtype=np.float64)),
+                                    _d(np.ascontiguousarray(coef, dtype=np.float64))))
+
+
+def ge_residual(u, coef):
+    r = np.zeros(30)
+    lib().or_ge_residual(_d(r), _d(np.ascontiguousarray(u, dtype=np.float64)),
+                         _d(np.ascontiguousarray(coef, dtype=np.float64)))
+    return r
+
+
+def ge_tangent(u, coef):
+    D = np.zeros(900)
+    lib().or_ge_tangent(_d(D), _d(np.ascontiguousarray(u, dtype=np.float64)),
+                        _d(np.ascontiguousarray(coef, dtype=np.float64)))
+    return D.reshape(30, 30)
+
+
+def ge_energy_coef_grad(u):
+    g = np.zeros(6)
+    lib().or_ge_energy_coef_grad(_d(g), _d(np.ascontiguousarray(u, dtype=np.float64)))
+    return g
+
+
+def getime_residual(u90, coef, tp):
+    r = np.zeros(30)
+    lib().or_getime_residual(_d(r), _d(np.ascontiguousarray(u90, dtype=np.float64)),
+                             _d(np.ascontiguousarray(coef, dtype=np.float64)),
+                             _d(np.ascontiguousarray(tp, dtype=np.float64)))
+    return r
+
+
+def getime_tangent(u90, coef, tp):
+    D = np.zeros(900)
+    lib().or_getime_tangent(_d(D), _d(np.ascontiguousarray(u90, dtype=np.float64)),
+                            _d(np.ascontiguousarray(coef, dtype=np.float64)),
+                            _d(np.ascontiguousarray(tp, dtype=np.float64)))
+    return D.reshape(30, 30)
+
+
+def iga_dirichlet(par, face_id, bc_order, idof, kv3, nb3):
+    kvs = [np.ascontiguousarray(k, dtype=np.float64) for k in kv3]
+    arr = (_dp * 3)(*[_d(k) for k in kvs])
+    nk = (ctypes.c_int * 3)(*[len(k) for k in kvs])
+    nb = (ctypes.c_int * 3)(*nb3)
+    out = np.zeros(nb3[0] * nb3[1])
+    lib().or_iga_dirichlet(_d(out), _d(np.ascontiguousarray(par, dtype=np.float64)), face_id, bc_order,
+                           idof, arr, nk, nb)
+    return out
+
+
+def tangent_elements_timed(pb, U_hist, rowptr, col, vals, e_begin, e_end):
+    """CPU baseline: the reference's per-element tangent loop over a range."""
+    b = Bound(pb)
+    U = np.ascontiguousarray(U_hist, dtype=np.float64)
+    rc = lib().or_tangent_elements(b.ptr, _d(U), rowptr.ctypes.data_as(_i64p), col.ctypes.data_as(_i32p),
+                                   _d(vals), e_begin, e_end)
+    if rc:
+        raise RuntimeError("or_tangent_elements failed: %d" % rc)
+
+
+def residual_elements_timed(pb, U_hist, R, e_begin, e_end):
+    b = Bound(pb)
+    U = np.ascontiguousarray(U_hist, dtype=np.float64)
+    rc = lib().or_residual_elements(b.ptr, _d(U), _d(R), e_begin, e_end)
+    if rc:
+        raise RuntimeError("or_residual_elements failed: %d" % rc)
+
+
+# --------------------------------------------------------------------------
+# The reference's own libm-only sources (oracle/_ref), for pinning the oracle.
+# --------------------------------------------------------------------------
+_ref = None
+
+
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        L = ctypes.CDLL(REF_PATH)
+        L.evalN.restype = ctypes.c_double
+        L.evalN.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        _ref = L
+    return _ref
+
+
+def ref_ge_residual(u, par):
+    r = np.zeros(30)
+    ref().ge_residual(_d(r), _d(np.ascontiguousarray(u, dtype=np.float64)),
+                      _d(np.ascontiguousarray(par, dtype=np.float64)))
+    return r
+
+
+def ref_ge_tangent(u, par):
+    D = np.zeros(900)
+    ref().ge_tangent(_d(D), _d(np.ascontiguousarray(u, dtype=np.float64)),
+                     _d(np.ascontiguousarray(par, dtype=np.float64)))
+    return D.reshape(30, 30)
+
+
+def ref_ge_energy(u, par):
+    h = np.zeros(13)
+    ref().ge_density(_d(h), _d(np.ascontiguousarray(u, dtype=np.float64)),
+                     _d(np.ascontiguousarray(par, dtype=np.float64)))
+    return h[0]
+
+
+def ref_getime_residual(u90, par):
+    r = np.zeros(30)
+    ref().getime_residual(_d(r), _d(np.ascontiguousarray(u90, dtype=np.float64)),
+                          _d(np.ascontiguousarray(par, dtype=np.float64)))
+    return r
+
+
+def ref_getime_tangent(u90, par):
+    D = np.zeros(900)
+    ref().getime_tangent(_d(D), _d(np.ascontiguousarray(u90, dtype=np.float64)),
+                         _d(np.ascontiguousarray(par, dtype=np.float64)))
+    return D.reshape(30, 30)
+
+
+def ref_eval_bspline(windows, xq):
+    ws = [np.ascontiguousarray(w, dtype=np.float64) for w in windows]
+    arr = (_dp * 3)(*[_d(w) for w in ws])
+    N = np.zeros(270)
+    ref().eval_Bspline(_d(N), arr, 2, _d(np.ascontiguousarray(xq, dtype=np.float64)))
+    return N.reshape(27, 10)
+
+
+def ref_evalN(kv, k, i, p, xi):
+    kv = np.ascontiguousarray(kv, dtype=np.float64)
+    return float(ref().evalN(_d(kv), len(kv), k, i, p, xi))
+
+
+def ref_gauss(n):
+    x, w = np.zeros(n), np.zeros(n)
+    ref().legendre_handle(_d(x), _d(w), ctypes.c_int(n), ctypes.c_double(0.0), ctypes.c_double(1.0))
+    return x, w
+
+
+def ref_iga_dirichlet(par, face_id, bc_order, idof, kv3, nb3):
+    kvs = [np.ascontiguousarray(k, dtype=np.float64) for k in kv3]
+    arr = (_dp * 3)(*[_d(k) for k in kvs])
+    nk = (ctypes.c_int * 3)(*[len(k) for k in kvs])
+    nb = (ctypes.c_int * 3)(*nb3)
+    out = np.zeros(nb3[0] * nb3[1])
+    ref().iga_dirichlet(_d(out), _d(np.ascontiguousarray(par, dtype=np.float64)), face_id, bc_order, idof,
+                        arr, nk, nb, 3, 2)
+    return out
