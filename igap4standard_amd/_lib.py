@@ -1,0 +1,108 @@
+"""ctypes binding of libigap4h.so (include/igap4h.h) and the reference struct
+layouts it takes (include/phys.h:11-26, include/nurbs.h:4-24,
+include/bc.h:4-11, include/iga.h:8-25).
+
+The HIP library is the product path: importing this module does not fall back
+to anything if the library is missing; `lib()` raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "csrc", "libigap4h.so")
+
+c_int_p = ctypes.POINTER(ctypes.c_int)
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_i32_p = ctypes.POINTER(ctypes.c_int32)
+c_i64_p = ctypes.POINTER(ctypes.c_int64)
+_fnp = ctypes.c_void_p  # host function pointers (never called by the library)
+
+
+class Phys(ctypes.Structure):
+    _fields_ = [("ndim", ctypes.c_int), ("ndof", ctypes.c_int), ("torder", ctypes.c_int),
+                ("sorder", ctypes.c_int), ("par_mat", c_double_p), ("ndensity", ctypes.c_int),
+                ("nfield", ctypes.c_int), ("residual", _fnp), ("tangent", _fnp), ("field", _fnp),
+                ("density", _fnp), ("densityb", _fnp), ("assert_par_mat", _fnp)]
+
+
+class Nurbs(ctypes.Structure):
+    _fields_ = [("porder", ctypes.c_int), ("nelem", c_int_p), ("nbasis", c_int_p), ("nknot", c_int_p),
+                ("knotVector", ctypes.POINTER(c_double_p)), ("nboun", ctypes.c_int), ("boun", c_int_p),
+                ("nsendPart", ctypes.c_int), ("sendPart", c_int_p), ("nrecvPart", ctypes.c_int),
+                ("recvPart", c_int_p), ("sendBlock", c_int_p), ("recvBlock", c_int_p),
+                ("universalIdx_temp", c_int_p), ("bc_periodic", c_int_p), ("nelem_global", c_int_p),
+                ("knotVector_global", ctypes.POINTER(c_double_p))]
+
+
+class BC(ctypes.Structure):
+    _fields_ = [("bc_type", c_int_p), ("par_periodic", c_double_p), ("par_dirichlet", c_double_p),
+                ("par_neumann", c_double_p)]
+
+
+class IGA(ctypes.Structure):
+    _fields_ = [("phys", ctypes.POINTER(Phys)), ("nurbs", ctypes.POINTER(Nurbs)), ("bc", ctypes.POINTER(BC)),
+                ("U_hist", c_double_p), ("nselfIdx", ctypes.c_int), ("selfIdx", c_int_p),
+                ("nsendIdx", ctypes.c_int), ("sendIdx", c_int_p), ("sendPtr", c_int_p),
+                ("nrecvIdx", ctypes.c_int), ("recvIdx", c_int_p), ("recvPtr", c_int_p),
+                ("globalIdx", c_int_p)]
+
+
+NEUMANN_FN = ctypes.CFUNCTYPE(None, c_double_p, c_double_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_double_p)
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("physics", ctypes.c_int), ("nquad", ctypes.c_int), ("elem_coef", c_double_p),
+                ("neumann", NEUMANN_FN)]
+
+
+IGAP4H_FP64, IGAP4H_FP32 = 0, 1
+IGAP4H_VALS_F64, IGAP4H_VALS_F32 = 0, 1
+IGAP4H_PHYS_GE, IGAP4H_PHYS_GETIME, IGAP4H_PHYS_FROM_TORDER = 0, 1, -1
+
+# every entry point declared in include/igap4h.h
+SYMBOLS = ("igap4h_create", "igap4h_destroy", "igap4h_last_error", "igap4h_sizes", "igap4h_pattern",
+           "igap4h_pattern_dev", "igap4h_dirichlet_rows", "igap4h_residual", "igap4h_tangent",
+           "igap4h_residual_dev", "igap4h_tangent_dev", "igap4h_set_coef", "igap4h_energy_dev")
+
+_lib = None
+
+
+def lib():
+    """Load libigap4h.so (build it with __graft_entry__.build() or
+    `make -C igap4standard_amd/csrc`).  Raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libigap4h.so is not built: %s (run __graft_entry__.build())" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.igap4h_create.argtypes = [ctypes.POINTER(IGA), ctypes.POINTER(Opts), ctypes.POINTER(vp)]
+        L.igap4h_destroy.argtypes = [vp]
+        L.igap4h_destroy.restype = None
+        L.igap4h_last_error.restype = ctypes.c_char_p
+        L.igap4h_sizes.argtypes = [vp, c_i64_p, c_i64_p, c_i64_p, c_i64_p, c_int_p]
+        L.igap4h_pattern.argtypes = [vp, c_i64_p, c_i32_p]
+        L.igap4h_pattern_dev.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]
+        L.igap4h_dirichlet_rows.argtypes = [vp, c_i32_p]
+        L.igap4h_residual.argtypes = [vp, c_double_p, c_double_p, c_double_p]
+        L.igap4h_tangent.argtypes = [vp, c_double_p, c_double_p, ctypes.c_int, ctypes.c_int, vp]
+        L.igap4h_residual_dev.argtypes = [vp, vp, vp, vp]
+        L.igap4h_tangent_dev.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp, vp]
+        L.igap4h_set_coef.argtypes = [vp, c_double_p]
+        L.igap4h_energy_dev.argtypes = [vp, vp, c_double_p, vp]
+        for name in SYMBOLS:
+            if name not in ("igap4h_destroy", "igap4h_last_error"):
+                getattr(L, name).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+class IgapError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().igap4h_last_error()
+        raise IgapError("igap4h error %d: %s" % (rc, msg.decode() if msg else ""))
+    return rc

@@ -1,0 +1,996 @@
+// igap4h: residual and tangent assembly for C1 quadratic B-spline gradient
+// elasticity on one MI355X (gfx950).  C-ABI in include/igap4h.h.
+//
+// Data layout in HBM (reference layout, one rank):
+//   U_hist  (torder+1) x ndof doubles, dof = 3*((nb1*nb2)*a0 + nb2*a1 + a2) + i
+//   R       ndof doubles
+//   CSR     rowptr int64 [ndof+1], col int32 [nnz] (ascending), vals f64|f32
+//   coef    n_elem x 6 doubles, element e = (ex*ne1 + ey)*ne2 + ez
+//   Re      n_elem x 81 doubles (per-element residual, gathered per dof)
+// Connectivity is arithmetic (structured patch); no index arrays per element.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/igap4h.h"
+#include "igap4h_physics.hpp"
+#include "igap4h_setup.hpp"
+
+using namespace igap4h;
+
+// ============================================================== errors
+static thread_local std::string g_err;
+static int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+#define HIPCHK(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) return fail(IGAP4H_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" const char *igap4h_last_error(void) { return g_err.c_str(); }
+
+// ============================================================== kernel args
+struct MeshArgs {
+    int ne[3];
+    int nb[3];
+    const double *knot[3];
+    int nq;
+    double cq[4], wq[4];
+    double pp[9];     // par_periodic (F0)
+    long long ndof;
+    double rho, cdamp, dt; // getime: par_mat[5..7]
+};
+
+template <int PK>
+struct PhysT;
+template <>
+struct PhysT<0> { // phys/ge: torder 0, gradient slots only in the contraction
+    using T = double;
+    using TR = double;
+    static constexpr int NS = 9, S0 = 1, NH = 1;
+};
+template <>
+struct PhysT<1> { // phys/getime: torder 2, value slot carries the mass term
+    using T = Jet<3>;
+    using TR = Jet<4>;
+    static constexpr int NS = 10, S0 = 0, NH = 3;
+};
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+template <class A>
+struct V4;
+template <>
+struct V4<double> { using t = d4; };
+template <>
+struct V4<float> { using t = f4; };
+
+__device__ __forceinline__ d4 mfma16(double a, double b, d4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4 mfma16(float a, float b, f4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// 1D basis tables of one element: phi[(d*4 + q)*9 + a*3 + k], k = value,
+// first, second derivative of function a at Gauss point q in direction d.
+__device__ __forceinline__ void fill_phi(const MeshArgs &m, const int e[3], double *sphi, int tid, int nthr)
+{
+    for (int idx = tid; idx < 3 * m.nq; idx += nthr) {
+        int d = idx / m.nq, q = idx % m.nq;
+        const double *k = m.knot[d] + e[d];
+        double X0 = k[2], X1 = k[3];
+        double xi = m.cq[q];
+        double x = (-xi + 1.0) * X0 + xi * X1;
+        double v[3], d1[3], d2[3];
+        bspline1d(k, x, v, d1, d2);
+        double *o = sphi + (d * 4 + q) * 9;
+        for (int a = 0; a < 3; a++) {
+            o[a * 3 + 0] = v[a];
+            o[a * 3 + 1] = d1[a];
+            o[a * 3 + 2] = d2[a];
+        }
+    }
+}
+
+// slot values of the basis function A = 9 ax + 3 ay + az at point (qx,qy,qz)
+__device__ __forceinline__ void basis_slots(const double *sphi, int qx, int qy, int qz, int A, double n[10])
+{
+    const double *px = sphi + (0 * 4 + qx) * 9 + (A / 9) * 3;
+    const double *py = sphi + (1 * 4 + qy) * 9 + ((A / 3) % 3) * 3;
+    const double *pz = sphi + (2 * 4 + qz) * 9 + (A % 3) * 3;
+    n[0] = px[0] * py[0] * pz[0];
+    n[1] = px[1] * py[0] * pz[0];
+    n[2] = px[0] * py[1] * pz[0];
+    n[3] = px[0] * py[0] * pz[1];
+    n[4] = px[2] * py[0] * pz[0];
+    n[5] = px[1] * py[1] * pz[0];
+    n[6] = px[1] * py[0] * pz[1];
+    n[7] = px[0] * py[2] * pz[0];
+    n[8] = px[0] * py[1] * pz[1];
+    n[9] = px[0] * py[0] * pz[2];
+}
+
+__device__ __forceinline__ long long node_of(const MeshArgs &m, int a0, int a1, int a2)
+{
+    return ((long long)a0 * m.nb[1] + a1) * m.nb[2] + a2;
+}
+
+// Interpolated slots u[h][30] at one Gauss point (snesiga_Residual.c:125-144),
+// F0 of par_periodic added to value and gradient slots of every slice.
+template <int NH>
+__device__ __forceinline__ void interp_slots(const MeshArgs &m, const double *sU, const double *sphi, const int e[3],
+                                             int qx, int qy, int qz, double u[NH][30], double *w)
+{
+    for (int h = 0; h < NH; h++)
+        for (int s = 0; s < 30; s++) u[h][s] = 0.0;
+    for (int A = 0; A < 27; A++) {
+        double n[10];
+        basis_slots(sphi, qx, qy, qz, A, n);
+        for (int h = 0; h < NH; h++)
+            for (int i = 0; i < 3; i++) {
+                double Uv = sU[h * 81 + 3 * A + i];
+#pragma unroll
+                for (int s = 0; s < 10; s++) u[h][10 * i + s] += n[s] * Uv;
+            }
+    }
+    double xq[3], hh[3];
+    int q[3] = {qx, qy, qz};
+    for (int d = 0; d < 3; d++) {
+        const double *k = m.knot[d] + e[d];
+        double xi = m.cq[q[d]];
+        xq[d] = (-xi + 1.0) * k[2] + xi * k[3];
+        hh[d] = k[3] - k[2];
+    }
+    for (int h = 0; h < NH; h++)
+        for (int i = 0; i < 3; i++) {
+            u[h][10 * i] += m.pp[3 * i] * xq[0] + m.pp[3 * i + 1] * xq[1] + m.pp[3 * i + 2] * xq[2];
+            for (int d = 0; d < 3; d++) u[h][10 * i + 1 + d] += m.pp[3 * i + d];
+        }
+    *w = m.wq[qx] * m.wq[qy] * m.wq[qz] * hh[0] * hh[1] * hh[2];
+}
+
+__device__ __forceinline__ Coef load_coef(const double *coef, long long e)
+{
+    const double *c = coef + 6 * e;
+    Coef r;
+    r.A36 = c[0]; r.A37 = c[1]; r.A38 = c[2]; r.e0 = c[3]; r.e345 = c[4]; r.lsq = c[5];
+    return r;
+}
+
+// ============================================================== residual
+// One wave per element, lane = Gauss point.  Writes the element vector
+// R_e[81] (snesiga_Residual.c:104-160) to Re; k_gather sums it per dof.
+template <int PK>
+__global__ __launch_bounds__(64) void k_residual_elem(MeshArgs m, const double *__restrict__ Uh,
+                                                      const double *__restrict__ coef, double *__restrict__ Re)
+{
+    constexpr int NH = PhysT<PK>::NH;
+    __shared__ double sU[NH * 81];
+    __shared__ double sphi[3 * 4 * 9];
+    __shared__ double sred[64 * 81];
+    const int tid = threadIdx.x;
+    const long long e = blockIdx.x;
+    int ev[3];
+    ev[2] = (int)(e % m.ne[2]);
+    ev[1] = (int)((e / m.ne[2]) % m.ne[1]);
+    ev[0] = (int)(e / ((long long)m.ne[2] * m.ne[1]));
+    for (int idx = tid; idx < NH * 81; idx += 64) {
+        int h = idx / 81, r = idx % 81, A = r / 3, i = r % 3;
+        long long nd = node_of(m, ev[0] + A / 9, ev[1] + (A / 3) % 3, ev[2] + A % 3);
+        sU[idx] = Uh[h * m.ndof + 3 * nd + i];
+    }
+    fill_phi(m, ev, sphi, tid, 64);
+    __syncthreads();
+    const int nqp = m.nq * m.nq * m.nq;
+    const Coef c = load_coef(coef, e);
+    double r[30];
+    for (int s = 0; s < 30; s++) r[s] = 0.0;
+    int qx = 0, qy = 0, qz = 0;
+    double w = 0.0;
+    if (tid < nqp) {
+        qx = tid / (m.nq * m.nq); qy = (tid / m.nq) % m.nq; qz = tid % m.nq;
+        double u[NH][30];
+        interp_slots<NH>(m, sU, sphi, ev, qx, qy, qz, u, &w);
+        if (PK == 0) {
+            residual_grad<double>(u[0], c, r);
+        } else {
+            using TR = typename PhysT<PK>::TR;
+            TR uj[30], rj[30];
+            // u = t_{n+1} (slice 0), u1 = t_n (slice 1), u0 = t_{n-1} (slice 2)
+            for (int s = 0; s < 30; s++) uj[s] = Num<TR>::lin((u[NH - 1][s] + u[1 % NH][s]) / 2.0, (u[0][s] - u[NH - 1][s]) / 2.0);
+            residual_grad<TR>(uj, c, rj);
+            for (int s = 0; s < 30; s++) r[s] = avg_residual(rj[s]);
+            for (int i = 0; i < 3; i++) {
+                int s = 10 * i;
+                r[s] = (m.cdamp * (u[0][s] - u[NH - 1][s])) / (2. * m.dt) +
+                       (m.rho * (u[0][s] + u[NH - 1][s] - 2 * u[1 % NH][s])) / (m.dt * m.dt);
+            }
+        }
+    }
+    // per-lane projection w sum_s N_{A,s} r_{i,s}, then a column sum over lanes
+    for (int A = 0; A < 27; A++) {
+        double n[10];
+        if (tid < nqp) basis_slots(sphi, qx, qy, qz, A, n);
+        for (int i = 0; i < 3; i++) {
+            double t = 0.0;
+            if (tid < nqp) {
+#pragma unroll
+                for (int s = 0; s < 10; s++) t += n[s] * r[10 * i + s];
+            }
+            sred[tid * 81 + 3 * A + i] = w * t;
+        }
+    }
+    __syncthreads();
+    for (int col = tid; col < 81; col += 64) {
+        double t = 0.0;
+        for (int l = 0; l < nqp; l++) t += sred[l * 81 + col];
+        Re[e * 81 + col] = t;
+    }
+}
+
+// Sum of element vectors per dof in element order (the reference's ADD order),
+// Neumann load, then Dirichlet rows R = U - g (INSERT, snesiga_Residual.c:382).
+__global__ __launch_bounds__(256) void k_gather(MeshArgs m, const double *__restrict__ Re,
+                                                const double *__restrict__ fneu, const unsigned char *__restrict__ dmask,
+                                                const double *__restrict__ gval, const double *__restrict__ U0,
+                                                double *__restrict__ R)
+{
+    long long dof = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (dof >= m.ndof) return;
+    long long node = dof / 3;
+    int i = (int)(dof % 3);
+    int a2 = (int)(node % m.nb[2]);
+    int a1 = (int)((node / m.nb[2]) % m.nb[1]);
+    int a0 = (int)(node / ((long long)m.nb[2] * m.nb[1]));
+    double s = 0.0;
+    for (int ex = max(0, a0 - 2); ex <= min(a0, m.ne[0] - 1); ex++)
+        for (int ey = max(0, a1 - 2); ey <= min(a1, m.ne[1] - 1); ey++)
+            for (int ez = max(0, a2 - 2); ez <= min(a2, m.ne[2] - 1); ez++) {
+                long long e = ((long long)ex * m.ne[1] + ey) * m.ne[2] + ez;
+                int A = (a0 - ex) * 9 + (a1 - ey) * 3 + (a2 - ez);
+                s += Re[e * 81 + 3 * A + i];
+            }
+    if (fneu) s += fneu[dof];
+    if (dmask[dof]) s = U0[dof] - gval[dof];
+    R[dof] = s;
+}
+
+// ============================================================== tangent
+constexpr int TNT = 256;  // threads per element
+constexpr int QC = 4;     // Gauss points per contraction chunk
+constexpr int TPW = 6;    // output tiles per wave (24 tiles of 16x16 over 4 waves)
+
+// Column tiles: row-block i of K holds columns (j >= i, B) padded to 32 per
+// j-block: widths 96 | 64 | 32 at offsets 0 | 96 | 160 of a 192-wide strip.
+__device__ __forceinline__ int ctile_block(int ct) { return ct < 6 ? 0 : (ct < 10 ? 1 : 2); }
+__device__ __forceinline__ int block_off(int i) { return i == 0 ? 0 : (i == 1 ? 96 : 160); }
+
+// One workgroup per element of one colour (ex = cx + 3 bx, ...): elements of
+// a colour share no basis function, so the read-modify-write scatter into the
+// CSR values needs no atomics and the sum order is fixed.
+template <int PK, class ACC>
+__global__ __launch_bounds__(TNT) void k_tangent(MeshArgs m, const double *__restrict__ Uh,
+                                                 const double *__restrict__ coef, double mass,
+                                                 const long long *__restrict__ rowptr,
+                                                 const unsigned char *__restrict__ dmask, void *__restrict__ vals,
+                                                 int vals_f32, int cx, int cy, int cz, int my, int mz)
+{
+    using T = typename PhysT<PK>::T;
+    constexpr int NS = PhysT<PK>::NS, S0 = PhysT<PK>::S0, NH = PhysT<PK>::NH;
+    constexpr int KC = QC * NS;
+    constexpr int NKA = KC * 32;          // A operand: N^T, [k][A(32)]
+    constexpr int NBUF = KC * (32 + 192); // A operand + T strip
+    static_assert(NBUF >= 81 * 81, "element matrix staging must fit");
+    static_assert(KC % 4 == 0, "k chunk must be a multiple of 4");
+    using V = typename V4<ACC>::t;
+
+    __shared__ double sU[NH * 81];
+    __shared__ double sphi[3 * 4 * 9];
+    __shared__ double sw[64];
+    __shared__ Compact<T> sCmp[64];
+    __shared__ double sNd[QC * 27 * 10];
+    __shared__ __attribute__((aligned(16))) ACC sBuf[NBUF];
+    ACC *sNk = sBuf;
+    ACC *sT = sBuf + NKA;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int b = blockIdx.x;
+    int ev[3];
+    ev[2] = cz + 3 * (b % mz);
+    ev[1] = cy + 3 * ((b / mz) % my);
+    ev[0] = cx + 3 * (b / (mz * my));
+    const long long e = ((long long)ev[0] * m.ne[1] + ev[1]) * m.ne[2] + ev[2];
+    const int nq = m.nq, nqp = nq * nq * nq;
+
+    for (int idx = tid; idx < NH * 81; idx += TNT) {
+        int h = idx / 81, r = idx % 81, A = r / 3, i = r % 3;
+        long long nd = node_of(m, ev[0] + A / 9, ev[1] + (A / 3) % 3, ev[2] + A % 3);
+        sU[idx] = Uh[h * m.ndof + 3 * nd + i];
+    }
+    fill_phi(m, ev, sphi, tid, TNT);
+    __syncthreads();
+    const Coef c = load_coef(coef, e);
+
+    // per-point compact data
+    if (tid < nqp) {
+        int qx = tid / (nq * nq), qy = (tid / nq) % nq, qz = tid % nq;
+        double u[NH][30], w;
+        interp_slots<NH>(m, sU, sphi, ev, qx, qy, qz, u, &w);
+        T uj[30];
+        if (PK == 0) {
+            for (int s = 0; s < 30; s++) uj[s] = Num<T>::cst(u[0][s]);
+        } else {
+            for (int s = 0; s < 30; s++) uj[s] = Num<T>::lin((u[NH - 1][s] + u[1 % NH][s]) / 2.0, (u[0][s] - u[NH - 1][s]) / 2.0);
+        }
+        Compact<T> cp;
+        compact_of<T>(uj, c, cp);
+        sCmp[tid] = cp;
+        sw[tid] = w;
+    }
+    __syncthreads();
+
+    V acc[TPW];
+    for (int t = 0; t < TPW; t++)
+        for (int r = 0; r < 4; r++) acc[t][r] = (ACC)0;
+
+    const int nchunk = (nqp + QC - 1) / QC;
+    for (int ch = 0; ch < nchunk; ch++) {
+        // (a) basis slot values for the chunk; A operand N^T[k][A]
+        for (int idx = tid; idx < QC * 270; idx += TNT) {
+            int ql = idx / 270, rem = idx % 270, A = rem / 10, s = rem % 10;
+            int qp = ch * QC + ql;
+            double v = 0.0;
+            if (qp < nqp) {
+                int qx = qp / (nq * nq), qy = (qp / nq) % nq, qz = qp % nq;
+                const double *px = sphi + (0 * 4 + qx) * 9 + (A / 9) * 3;
+                const double *py = sphi + (1 * 4 + qy) * 9 + ((A / 3) % 3) * 3;
+                const double *pz = sphi + (2 * 4 + qz) * 9 + (A % 3) * 3;
+                v = px[kSlotD[s][0]] * py[kSlotD[s][1]] * pz[kSlotD[s][2]];
+            }
+            sNd[idx] = v;
+            if (s >= S0) sNk[(ql * NS + s - S0) * 32 + A] = (ACC)v;
+        }
+        __syncthreads();
+        // (b) B operand T_ij[k][B] = w sum_t D[(i,s),(j,t)] N_{B,t}, i <= j
+        for (int task = tid; task < QC * 81; task += TNT) {
+            int ql = task / 81, rem = task % 81, j = rem / 27, B = rem % 27;
+            int qp = ch * QC + ql;
+            double out[3][10];
+            if (qp < nqp) {
+                double n[10];
+                for (int s = 0; s < 10; s++) n[s] = sNd[(ql * 27 + B) * 10 + s];
+                tangent_apply<T>(sCmp[qp], c, j, n, mass, sw[qp], out);
+            } else {
+                for (int i = 0; i < 3; i++)
+                    for (int s = 0; s < 10; s++) out[i][s] = 0.0;
+            }
+            for (int i = 0; i <= j; i++) {
+                ACC *col = sT + block_off(i) + (j - i) * 27 + B;
+                for (int s = S0; s < 10; s++) col[(ql * NS + s - S0) * 192] = (ACC)out[i][s];
+            }
+        }
+        __syncthreads();
+        // (c) K_i,(j>=i) += N^T T on MFMA
+#pragma unroll
+        for (int ks = 0; ks < KC / 4; ks++) {
+            int k = 4 * ks + (lane >> 4);
+#pragma unroll
+            for (int t = 0; t < TPW; t++) {
+                int tg = wave * TPW + t, ct = tg >> 1, at = tg & 1;
+                ACC a = sNk[k * 32 + at * 16 + (lane & 15)];
+                ACC bb = sT[k * 192 + ct * 16 + (lane & 15)];
+                acc[t] = mfma16(a, bb, acc[t]);
+            }
+        }
+        __syncthreads();
+    }
+
+    // stage the full 81x81 element matrix (mirror the i < j blocks)
+    ACC *sKe = sBuf;
+    for (int t = 0; t < TPW; t++) {
+        int tg = wave * TPW + t, ct = tg >> 1, at = tg & 1;
+        int i = ctile_block(ct);
+        int cl = ct * 16 + (lane & 15) - block_off(i);
+        if (cl >= (3 - i) * 27) continue;
+        int j = i + cl / 27, B = cl % 27;
+        for (int r = 0; r < 4; r++) {
+            int A = at * 16 + 4 * (lane >> 4) + r;
+            if (A >= 27) continue;
+            sKe[(3 * A + i) * 81 + 3 * B + j] = acc[t][r];
+            if (i != j) sKe[(3 * B + j) * 81 + 3 * A + i] = acc[t][r];
+        }
+    }
+    __syncthreads();
+    // scatter-add into the CSR values; Dirichlet rows/columns skipped
+    for (int idx = tid; idx < 81 * 81; idx += TNT) {
+        int r = idx / 81, cc = idx % 81;
+        int A = r / 3, i = r % 3, B = cc / 3, j = cc % 3;
+        int a0 = ev[0] + A / 9, a1 = ev[1] + (A / 3) % 3, a2 = ev[2] + A % 3;
+        int b0 = ev[0] + B / 9, b1 = ev[1] + (B / 3) % 3, b2 = ev[2] + B % 3;
+        long long grow = 3 * node_of(m, a0, a1, a2) + i;
+        long long gcol = 3 * node_of(m, b0, b1, b2) + j;
+        if (dmask[grow] | dmask[gcol]) continue;
+        int lo0 = max(0, a0 - 2), lo1 = max(0, a1 - 2), lo2 = max(0, a2 - 2);
+        int w1 = min(m.nb[1] - 1, a1 + 2) - lo1 + 1, w2 = min(m.nb[2] - 1, a2 + 2) - lo2 + 1;
+        long long pos = rowptr[grow] + 3LL * (((b0 - lo0) * w1 + (b1 - lo1)) * w2 + (b2 - lo2)) + j;
+        if (vals_f32)
+            ((float *)vals)[pos] += (float)sKe[idx];
+        else
+            ((double *)vals)[pos] += (double)sKe[idx];
+    }
+}
+
+// diagonal of Dirichlet rows = 1 (MatZeroRowsColumns(..., 1.0, ...))
+__global__ void k_dirichlet_diag(MeshArgs m, const int *__restrict__ rows, int n, const long long *__restrict__ rowptr,
+                                 void *vals, int vals_f32)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    long long r = rows[t];
+    long long node = r / 3;
+    int i = (int)(r % 3);
+    int a2 = (int)(node % m.nb[2]);
+    int a1 = (int)((node / m.nb[2]) % m.nb[1]);
+    int a0 = (int)(node / ((long long)m.nb[2] * m.nb[1]));
+    int lo0 = max(0, a0 - 2), lo1 = max(0, a1 - 2), lo2 = max(0, a2 - 2);
+    int w1 = min(m.nb[1] - 1, a1 + 2) - lo1 + 1, w2 = min(m.nb[2] - 1, a2 + 2) - lo2 + 1;
+    long long pos = rowptr[r] + 3LL * (((a0 - lo0) * w1 + (a1 - lo1)) * w2 + (a2 - lo2)) + i;
+    if (vals_f32)
+        ((float *)vals)[pos] = 1.0f;
+    else
+        ((double *)vals)[pos] = 1.0;
+}
+
+// zero fill of n32 32-bit words (a kernel, not a memset node, so that a
+// captured graph replays it faithfully)
+__global__ void k_fill_zero(unsigned int *p, long long n32)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n32; i += stride) p[i] = 0u;
+}
+
+// ============================================================== pattern
+__device__ __forceinline__ int stencil_w(int a, int nb) { return min(nb - 1, a + 2) - max(0, a - 2) + 1; }
+__device__ __forceinline__ long long stencil_prefix(int a, int nb)
+{
+    long long s = 0;
+    for (int t = 0; t < a; t++) s += stencil_w(t, nb);
+    return s;
+}
+
+// rowptr in closed form: rows ordered (a0, a1, a2, i)
+__global__ void k_rowptr(MeshArgs m, long long *rowptr)
+{
+    long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > m.ndof) return;
+    if (r == m.ndof) {
+        rowptr[r] = 9LL * stencil_prefix(m.nb[0], m.nb[0]) * stencil_prefix(m.nb[1], m.nb[1]) * stencil_prefix(m.nb[2], m.nb[2]);
+        return;
+    }
+    long long node = r / 3;
+    int i = (int)(r % 3);
+    int a2 = (int)(node % m.nb[2]);
+    int a1 = (int)((node / m.nb[2]) % m.nb[1]);
+    int a0 = (int)(node / ((long long)m.nb[2] * m.nb[1]));
+    long long T1 = stencil_prefix(m.nb[1], m.nb[1]), T2 = stencil_prefix(m.nb[2], m.nb[2]);
+    long long w0 = stencil_w(a0, m.nb[0]), w1 = stencil_w(a1, m.nb[1]), w2 = stencil_w(a2, m.nb[2]);
+    rowptr[r] = 9LL * (stencil_prefix(a0, m.nb[0]) * T1 * T2 + w0 * stencil_prefix(a1, m.nb[1]) * T2 +
+                       w0 * w1 * stencil_prefix(a2, m.nb[2])) +
+                3LL * i * w0 * w1 * w2;
+}
+
+// columns: one thread per (row, b0, b1) run of 3 * w2 ascending columns
+__global__ void k_cols(MeshArgs m, const long long *__restrict__ rowptr, int *__restrict__ col)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m.ndof * 25) return;
+    long long r = t / 25;
+    int run = (int)(t % 25), db0 = run / 5 - 2, db1 = run % 5 - 2;
+    long long node = r / 3;
+    int a2 = (int)(node % m.nb[2]);
+    int a1 = (int)((node / m.nb[2]) % m.nb[1]);
+    int a0 = (int)(node / ((long long)m.nb[2] * m.nb[1]));
+    int b0 = a0 + db0, b1 = a1 + db1;
+    if (b0 < 0 || b0 >= m.nb[0] || b1 < 0 || b1 >= m.nb[1]) return;
+    int lo0 = max(0, a0 - 2), lo1 = max(0, a1 - 2), lo2 = max(0, a2 - 2);
+    int w1 = min(m.nb[1] - 1, a1 + 2) - lo1 + 1, w2 = min(m.nb[2] - 1, a2 + 2) - lo2 + 1;
+    long long pos = rowptr[r] + 3LL * ((b0 - lo0) * w1 + (b1 - lo1)) * w2;
+    for (int b2 = lo2; b2 < lo2 + w2; b2++)
+        for (int j = 0; j < 3; j++) col[pos++] = (int)(3 * node_of(m, b0, b1, b2) + j);
+}
+
+// ============================================================== energy (ge)
+__global__ __launch_bounds__(64) void k_energy_elem(MeshArgs m, const double *__restrict__ U,
+                                                    const double *__restrict__ coef, double *__restrict__ Pe)
+{
+    __shared__ double sU[81];
+    __shared__ double sphi[3 * 4 * 9];
+    __shared__ double sred[64];
+    const int tid = threadIdx.x;
+    const long long e = blockIdx.x;
+    int ev[3];
+    ev[2] = (int)(e % m.ne[2]);
+    ev[1] = (int)((e / m.ne[2]) % m.ne[1]);
+    ev[0] = (int)(e / ((long long)m.ne[2] * m.ne[1]));
+    for (int idx = tid; idx < 81; idx += 64) {
+        int A = idx / 3, i = idx % 3;
+        sU[idx] = U[3 * node_of(m, ev[0] + A / 9, ev[1] + (A / 3) % 3, ev[2] + A % 3) + i];
+    }
+    fill_phi(m, ev, sphi, tid, 64);
+    __syncthreads();
+    const int nqp = m.nq * m.nq * m.nq;
+    double val = 0.0;
+    if (tid < nqp) {
+        int qx = tid / (m.nq * m.nq), qy = (tid / m.nq) % m.nq, qz = tid % m.nq;
+        double u[1][30], w;
+        interp_slots<1>(m, sU, sphi, ev, qx, qy, qz, u, &w);
+        val = w * energy_of<double>(u[0], load_coef(coef, e));
+    }
+    sred[tid] = val;
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        for (int l = 0; l < nqp; l++) s += sred[l];
+        Pe[e] = s;
+    }
+}
+
+// ============================================================== context
+struct igap4h_ctx {
+    Mesh mesh;
+    int physics, nquad, nhist;
+    int device;
+    double par_mat[9];
+    double cq[4], wq[4];
+    int64_t nnz;
+    std::vector<int32_t> drows;
+    // device
+    double *d_knot[3] = {nullptr, nullptr, nullptr};
+    double *d_coef = nullptr;
+    long long *d_rowptr = nullptr;
+    int *d_col = nullptr;
+    unsigned char *d_mask = nullptr;
+    double *d_g = nullptr;
+    double *d_fneu = nullptr;
+    int *d_drows = nullptr;
+    double *d_Re = nullptr;      // n_elem x 81
+    double *d_Uh = nullptr;      // staging for the host API
+    double *d_R = nullptr;
+    void *d_vals = nullptr;      // staging for the host API
+    size_t d_vals_bytes = 0;
+    double *d_Pe = nullptr;
+    double *U_hist_app = nullptr;
+};
+
+static MeshArgs mesh_args(const igap4h_ctx *c)
+{
+    MeshArgs m;
+    for (int d = 0; d < 3; d++) {
+        m.ne[d] = c->mesh.ne[d];
+        m.nb[d] = c->mesh.nb[d];
+        m.knot[d] = c->d_knot[d];
+    }
+    m.nq = c->nquad;
+    for (int q = 0; q < 4; q++) {
+        m.cq[q] = q < c->nquad ? c->cq[q] : 0.0;
+        m.wq[q] = q < c->nquad ? c->wq[q] : 0.0;
+    }
+    for (int k = 0; k < 9; k++) m.pp[k] = c->mesh.par_periodic[k];
+    m.ndof = c->mesh.ndof();
+    m.rho = c->par_mat[5];
+    m.cdamp = c->par_mat[6];
+    m.dt = c->par_mat[7];
+    return m;
+}
+
+template <class X>
+static int dev_alloc(X **p, size_t n)
+{
+    if (n == 0) n = 1;
+    if (hipMalloc((void **)p, n * sizeof(X)) != hipSuccess) {
+        *p = nullptr;
+        return fail(IGAP4H_ENOMEM, "hipMalloc of %zu bytes failed", n * sizeof(X));
+    }
+    return 0;
+}
+
+static void ctx_free(igap4h_ctx *c)
+{
+    if (!c) return;
+    int cur = -1;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(c->device);
+    for (int d = 0; d < 3; d++) (void)hipFree(c->d_knot[d]);
+    (void)hipFree(c->d_coef);
+    (void)hipFree(c->d_rowptr);
+    (void)hipFree(c->d_col);
+    (void)hipFree(c->d_mask);
+    (void)hipFree(c->d_g);
+    (void)hipFree(c->d_fneu);
+    (void)hipFree(c->d_drows);
+    (void)hipFree(c->d_Re);
+    (void)hipFree(c->d_Uh);
+    (void)hipFree(c->d_R);
+    (void)hipFree(c->d_vals);
+    (void)hipFree(c->d_Pe);
+    if (cur >= 0) (void)hipSetDevice(cur);
+    delete c;
+}
+
+extern "C" void igap4h_destroy(igap4h_ctx *ctx) { ctx_free(ctx); }
+
+static int check_device(int *dev)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(IGAP4H_ENODEV, "no HIP device visible");
+    HIPCHK(hipGetDevice(dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, *dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(IGAP4H_ENODEV, "device %d is %s; this library is built for gfx950", *dev, prop.gcnArchName);
+    return 0;
+}
+
+static int upload_coef(igap4h_ctx *c, const double *elem_coef)
+{
+    int64_t ne = c->mesh.nelem();
+    std::vector<double> h;
+    const double *src = elem_coef;
+    if (!src) {
+        double cf[6];
+        coef_from_par(c->par_mat, cf);
+        h.resize((size_t)ne * 6);
+        for (int64_t e = 0; e < ne; e++)
+            for (int k = 0; k < 6; k++) h[(size_t)e * 6 + k] = cf[k];
+        src = h.data();
+    }
+    HIPCHK(hipMemcpy(c->d_coef, src, (size_t)ne * 6 * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int igap4h_create(const IGA *app, const igap4h_opts *opts, igap4h_ctx **out)
+{
+    if (!out) return fail(IGAP4H_EINVAL, "ctx output pointer is NULL");
+    *out = nullptr;
+    if (!app || !app->phys || !app->nurbs || !app->bc) return fail(IGAP4H_EINVAL, "IGA, Phys, Nurbs and BC are required");
+    const Phys *ph = app->phys;
+    const Nurbs *nu = app->nurbs;
+    const BC *bc = app->bc;
+    if (ph->ndim != 3 || ph->ndof != 3) return fail(IGAP4H_EINVAL, "ndim = ndof = 3 required (got %d, %d)", ph->ndim, ph->ndof);
+    if (nu->porder != 2) return fail(IGAP4H_EINVAL, "porder 2 required (got %d)", nu->porder);
+    if (!nu->nelem || !nu->nbasis || !nu->nknot || !nu->knotVector || !ph->par_mat || !bc->bc_type ||
+        !bc->par_dirichlet || !bc->par_periodic)
+        return fail(IGAP4H_EINVAL, "incomplete Nurbs/Phys/BC");
+    igap4h_opts o = {IGAP4H_PHYS_FROM_TORDER, 0, nullptr, nullptr};
+    if (opts) o = *opts;
+    int physics = o.physics;
+    if (physics == IGAP4H_PHYS_FROM_TORDER) physics = ph->torder == 2 ? IGAP4H_PHYS_GETIME : IGAP4H_PHYS_GE;
+    if (physics != IGAP4H_PHYS_GE && physics != IGAP4H_PHYS_GETIME) return fail(IGAP4H_EINVAL, "unknown physics %d", physics);
+    int need_torder = physics == IGAP4H_PHYS_GE ? 0 : 2;
+    if (ph->torder != need_torder) return fail(IGAP4H_EINVAL, "physics %d needs torder %d (got %d)", physics, need_torder, ph->torder);
+    int nquad = o.nquad ? o.nquad : 4;
+    if (nquad < 1 || nquad > 4) return fail(IGAP4H_EINVAL, "nquad must be 1..4 (got %d)", nquad);
+
+    igap4h_ctx *c = new igap4h_ctx();
+    c->physics = physics;
+    c->nquad = nquad;
+    c->nhist = need_torder + 1;
+    gauss01(nquad, c->cq, c->wq);
+    for (int k = 0; k < 9; k++) c->par_mat[k] = ph->par_mat[k];
+    Mesh &m = c->mesh;
+    for (int d = 0; d < 3; d++) {
+        m.ne[d] = nu->nelem[d];
+        m.nb[d] = nu->nbasis[d];
+        if (m.ne[d] < 1 || m.nb[d] != m.ne[d] + 2 || nu->nknot[d] != m.ne[d] + 5 || !nu->knotVector[d]) {
+            delete c;
+            return fail(IGAP4H_EINVAL, "direction %d: need nbasis = nelem + 2, nknot = nelem + 5 (one rank, no periodicity)", d);
+        }
+        m.knot[d].assign(nu->knotVector[d], nu->knotVector[d] + nu->nknot[d]);
+    }
+    if (nu->nboun < 0 || nu->nboun > 6 || (nu->nboun > 0 && !nu->boun)) {
+        delete c;
+        return fail(IGAP4H_EINVAL, "bad boundary face list");
+    }
+    m.nboun = nu->nboun;
+    for (int i = 0; i < m.nboun; i++) {
+        if (nu->boun[i] < 0 || nu->boun[i] > 5) {
+            delete c;
+            return fail(IGAP4H_EINVAL, "boundary face id %d out of range", nu->boun[i]);
+        }
+        m.boun[i] = nu->boun[i];
+    }
+    for (int i = 0; i < 36; i++) m.bc_type[i] = bc->bc_type[i];
+    for (int i = 0; i < 9; i++) {
+        m.par_dirichlet[i] = bc->par_dirichlet[i];
+        m.par_periodic[i] = bc->par_periodic[i];
+    }
+    m.par_neumann = bc->par_neumann;
+    const int64_t ndof = m.ndof(), ne = m.nelem();
+    if (ndof > 2147483647LL) {
+        delete c;
+        return fail(IGAP4H_EINVAL, "ndof %lld exceeds int32 column indices", (long long)ndof);
+    }
+    if (app->nselfIdx != 0 && app->nselfIdx != ndof) {
+        delete c;
+        return fail(IGAP4H_EINVAL, "nselfIdx %d != ndof %lld (multi-rank partitions are not supported)", app->nselfIdx, (long long)ndof);
+    }
+    if (app->globalIdx) {
+        for (int64_t i = 0; i < ndof; i++)
+            if (app->globalIdx[i] != i) {
+                delete c;
+                return fail(IGAP4H_EINVAL, "globalIdx is not the identity at %lld (one rank only)", (long long)i);
+            }
+    }
+    c->U_hist_app = app->U_hist;
+    int rc = check_device(&c->device);
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    c->nnz = pattern_nnz(m.nb);
+    c->drows = dirichlet_rows(m);
+    std::vector<unsigned char> mask;
+    std::vector<double> g;
+    dirichlet_values(m, mask, g);
+    std::vector<double> fneu = neumann_load(m, nquad, o.neumann);
+
+#define ALLOC(p, n)                   \
+    do {                              \
+        int r_ = dev_alloc(&(p), (n)); \
+        if (r_) {                     \
+            ctx_free(c);              \
+            return r_;                \
+        }                             \
+    } while (0)
+#define CPY(dst, src, bytes)                                                          \
+    do {                                                                              \
+        if (hipMemcpy((dst), (src), (bytes), hipMemcpyHostToDevice) != hipSuccess) {  \
+            ctx_free(c);                                                              \
+            return fail(IGAP4H_EHIP, "upload failed");                                \
+        }                                                                             \
+    } while (0)
+    for (int d = 0; d < 3; d++) {
+        ALLOC(c->d_knot[d], m.knot[d].size());
+        CPY(c->d_knot[d], m.knot[d].data(), m.knot[d].size() * sizeof(double));
+    }
+    ALLOC(c->d_coef, (size_t)ne * 6);
+    rc = upload_coef(c, o.elem_coef);
+    if (rc) {
+        ctx_free(c);
+        return rc;
+    }
+    ALLOC(c->d_rowptr, (size_t)ndof + 1);
+    ALLOC(c->d_col, (size_t)c->nnz);
+    ALLOC(c->d_mask, (size_t)ndof);
+    CPY(c->d_mask, mask.data(), (size_t)ndof);
+    ALLOC(c->d_g, (size_t)ndof);
+    CPY(c->d_g, g.data(), (size_t)ndof * sizeof(double));
+    if (!fneu.empty()) {
+        ALLOC(c->d_fneu, (size_t)ndof);
+        CPY(c->d_fneu, fneu.data(), (size_t)ndof * sizeof(double));
+    }
+    ALLOC(c->d_drows, c->drows.size());
+    if (!c->drows.empty()) CPY(c->d_drows, c->drows.data(), c->drows.size() * sizeof(int32_t));
+    ALLOC(c->d_Re, (size_t)ne * 81);
+    // CSR pattern on the device
+    MeshArgs ma = mesh_args(c);
+    {
+        long long nthr = ndof + 1;
+        hipLaunchKernelGGL(k_rowptr, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, 0, ma, c->d_rowptr);
+        long long nt2 = ndof * 25;
+        hipLaunchKernelGGL(k_cols, dim3((unsigned)((nt2 + 255) / 256)), dim3(256), 0, 0, ma, (const long long *)c->d_rowptr, c->d_col);
+        if (hipDeviceSynchronize() != hipSuccess || hipGetLastError() != hipSuccess) {
+            ctx_free(c);
+            return fail(IGAP4H_EHIP, "pattern kernels failed");
+        }
+        long long last = 0;
+        (void)hipMemcpy(&last, c->d_rowptr + ndof, sizeof(long long), hipMemcpyDeviceToHost);
+        if (last != c->nnz) {
+            ctx_free(c);
+            return fail(IGAP4H_EHIP, "pattern size mismatch %lld vs %lld", last, (long long)c->nnz);
+        }
+    }
+    *out = c;
+    return 0;
+#undef ALLOC
+#undef CPY
+}
+
+extern "C" int igap4h_sizes(const igap4h_ctx *c, int64_t *ndof, int64_t *nnz, int64_t *ndir, int64_t *nelem, int *nhist)
+{
+    if (!c) return fail(IGAP4H_EINVAL, "NULL context");
+    if (ndof) *ndof = c->mesh.ndof();
+    if (nnz) *nnz = c->nnz;
+    if (ndir) *ndir = (int64_t)c->drows.size();
+    if (nelem) *nelem = c->mesh.nelem();
+    if (nhist) *nhist = c->nhist;
+    return 0;
+}
+
+extern "C" int igap4h_pattern_dev(igap4h_ctx *c, const int64_t **rowptr, const int32_t **col)
+{
+    if (!c) return fail(IGAP4H_EINVAL, "NULL context");
+    if (rowptr) *rowptr = (const int64_t *)c->d_rowptr;
+    if (col) *col = (const int32_t *)c->d_col;
+    return 0;
+}
+
+extern "C" int igap4h_pattern(igap4h_ctx *c, int64_t *rowptr, int32_t *col)
+{
+    if (!c) return fail(IGAP4H_EINVAL, "NULL context");
+    HIPCHK(hipSetDevice(c->device));
+    if (rowptr) HIPCHK(hipMemcpy(rowptr, c->d_rowptr, ((size_t)c->mesh.ndof() + 1) * sizeof(long long), hipMemcpyDeviceToHost));
+    if (col) HIPCHK(hipMemcpy(col, c->d_col, (size_t)c->nnz * sizeof(int), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int igap4h_dirichlet_rows(igap4h_ctx *c, int32_t *rows)
+{
+    if (!c || !rows) return fail(IGAP4H_EINVAL, "NULL argument");
+    if (!c->drows.empty()) std::memcpy(rows, c->drows.data(), c->drows.size() * sizeof(int32_t));
+    return 0;
+}
+
+extern "C" int igap4h_set_coef(igap4h_ctx *c, const double *elem_coef)
+{
+    if (!c) return fail(IGAP4H_EINVAL, "NULL context");
+    HIPCHK(hipSetDevice(c->device));
+    return upload_coef(c, elem_coef);
+}
+
+extern "C" int igap4h_residual_dev(igap4h_ctx *c, const double *d_Uh, double *d_R, void *stream)
+{
+    if (!c || !d_Uh || !d_R) return fail(IGAP4H_EINVAL, "NULL argument");
+    hipStream_t st = (hipStream_t)stream;
+    MeshArgs m = mesh_args(c);
+    long long ne = c->mesh.nelem();
+    if (c->physics == IGAP4H_PHYS_GE)
+        hipLaunchKernelGGL(k_residual_elem<0>, dim3((unsigned)ne), dim3(64), 0, st, m, d_Uh, (const double *)c->d_coef, c->d_Re);
+    else
+        hipLaunchKernelGGL(k_residual_elem<1>, dim3((unsigned)ne), dim3(64), 0, st, m, d_Uh, (const double *)c->d_coef, c->d_Re);
+    HIPCHK(hipGetLastError());
+    long long nd = m.ndof;
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, st, m, (const double *)c->d_Re,
+                       (const double *)c->d_fneu, (const unsigned char *)c->d_mask, (const double *)c->d_g, d_Uh, d_R);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+template <int PK, class ACC>
+static int launch_tangent(igap4h_ctx *c, const MeshArgs &m, const double *d_Uh, double mass, void *d_vals, int f32, hipStream_t st)
+{
+    for (int cx = 0; cx < 3; cx++)
+        for (int cy = 0; cy < 3; cy++)
+            for (int cz = 0; cz < 3; cz++) {
+                int mx = (m.ne[0] - cx + 2) / 3, my = (m.ne[1] - cy + 2) / 3, mz = (m.ne[2] - cz + 2) / 3;
+                long long nb = (long long)mx * my * mz;
+                if (nb <= 0) continue;
+                hipLaunchKernelGGL((k_tangent<PK, ACC>), dim3((unsigned)nb), dim3(TNT), 0, st, m, d_Uh,
+                                   (const double *)c->d_coef, mass, (const long long *)c->d_rowptr,
+                                   (const unsigned char *)c->d_mask, d_vals, f32, cx, cy, cz, my, mz);
+                HIPCHK(hipGetLastError());
+            }
+    return 0;
+}
+
+extern "C" int igap4h_tangent_dev(igap4h_ctx *c, const double *d_Uh, int prec, int vals_type, void *d_vals, void *stream)
+{
+    if (!c || !d_Uh || !d_vals) return fail(IGAP4H_EINVAL, "NULL argument");
+    if (prec != IGAP4H_FP64 && prec != IGAP4H_FP32) return fail(IGAP4H_EINVAL, "bad precision %d", prec);
+    if (vals_type != IGAP4H_VALS_F64 && vals_type != IGAP4H_VALS_F32) return fail(IGAP4H_EINVAL, "bad vals type %d", vals_type);
+    hipStream_t st = (hipStream_t)stream;
+    MeshArgs m = mesh_args(c);
+    int f32 = vals_type == IGAP4H_VALS_F32;
+    long long n32 = f32 ? c->nnz : 2 * c->nnz;
+    hipLaunchKernelGGL(k_fill_zero, dim3(2048), dim3(256), 0, st, (unsigned int *)d_vals, n32);
+    HIPCHK(hipGetLastError());
+    double mass = c->physics == IGAP4H_PHYS_GETIME ? c->par_mat[5] / (c->par_mat[7] * c->par_mat[7]) + c->par_mat[6] / (2. * c->par_mat[7]) : 0.0;
+    int rc;
+    if (c->physics == IGAP4H_PHYS_GE)
+        rc = prec == IGAP4H_FP64 ? launch_tangent<0, double>(c, m, d_Uh, mass, d_vals, f32, st)
+                                 : launch_tangent<0, float>(c, m, d_Uh, mass, d_vals, f32, st);
+    else
+        rc = prec == IGAP4H_FP64 ? launch_tangent<1, double>(c, m, d_Uh, mass, d_vals, f32, st)
+                                 : launch_tangent<1, float>(c, m, d_Uh, mass, d_vals, f32, st);
+    if (rc) return rc;
+    int nd = (int)c->drows.size();
+    if (nd > 0) {
+        hipLaunchKernelGGL(k_dirichlet_diag, dim3((nd + 255) / 256), dim3(256), 0, st, m, (const int *)c->d_drows, nd,
+                           (const long long *)c->d_rowptr, d_vals, f32);
+        HIPCHK(hipGetLastError());
+    }
+    return 0;
+}
+
+static int stage_U(igap4h_ctx *c, const double *U, double *U_hist)
+{
+    int64_t ndof = c->mesh.ndof();
+    if (!c->d_Uh) {
+        int r = dev_alloc(&c->d_Uh, (size_t)ndof * c->nhist);
+        if (r) return r;
+    }
+    // side effect of the reference callbacks: U_hist[0:nself] = U (:60)
+    if (U_hist && U && U_hist != U) std::memcpy(U_hist, U, (size_t)ndof * sizeof(double));
+    const double *src0 = U ? U : U_hist;
+    if (!src0) return fail(IGAP4H_EINVAL, "need U or U_hist");
+    HIPCHK(hipMemcpy(c->d_Uh, src0, (size_t)ndof * sizeof(double), hipMemcpyHostToDevice));
+    if (c->nhist > 1) {
+        if (!U_hist) return fail(IGAP4H_EINVAL, "getime needs U_hist history slices");
+        HIPCHK(hipMemcpy(c->d_Uh + ndof, U_hist + ndof, (size_t)ndof * (c->nhist - 1) * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+extern "C" int igap4h_residual(igap4h_ctx *c, const double *U, double *U_hist, double *R)
+{
+    if (!c || !R) return fail(IGAP4H_EINVAL, "NULL argument");
+    HIPCHK(hipSetDevice(c->device));
+    int rc = stage_U(c, U, U_hist);
+    if (rc) return rc;
+    int64_t ndof = c->mesh.ndof();
+    if (!c->d_R && (rc = dev_alloc(&c->d_R, (size_t)ndof))) return rc;
+    rc = igap4h_residual_dev(c, c->d_Uh, c->d_R, nullptr);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(R, c->d_R, (size_t)ndof * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int igap4h_tangent(igap4h_ctx *c, const double *U, double *U_hist, int prec, int vals_type, void *vals)
+{
+    if (!c || !vals) return fail(IGAP4H_EINVAL, "NULL argument");
+    HIPCHK(hipSetDevice(c->device));
+    int rc = stage_U(c, U, U_hist);
+    if (rc) return rc;
+    size_t bytes = (size_t)c->nnz * (vals_type == IGAP4H_VALS_F32 ? sizeof(float) : sizeof(double));
+    if (c->d_vals_bytes < bytes) {
+        (void)hipFree(c->d_vals);
+        c->d_vals = nullptr;
+        c->d_vals_bytes = 0;
+        if (hipMalloc(&c->d_vals, bytes) != hipSuccess) return fail(IGAP4H_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
+        c->d_vals_bytes = bytes;
+    }
+    rc = igap4h_tangent_dev(c, c->d_Uh, prec, vals_type, c->d_vals, nullptr);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(vals, c->d_vals, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int igap4h_energy_dev(igap4h_ctx *c, const double *d_U, double *energy, void *stream)
+{
+    if (!c || !d_U || !energy) return fail(IGAP4H_EINVAL, "NULL argument");
+    if (c->physics != IGAP4H_PHYS_GE) return fail(IGAP4H_EINVAL, "energy is implemented for phys/ge only");
+    long long ne = c->mesh.nelem();
+    if (!c->d_Pe) {
+        int r = dev_alloc(&c->d_Pe, (size_t)ne);
+        if (r) return r;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    MeshArgs m = mesh_args(c);
+    hipLaunchKernelGGL(k_energy_elem, dim3((unsigned)ne), dim3(64), 0, st, m, d_U, (const double *)c->d_coef, c->d_Pe);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h((size_t)ne);
+    HIPCHK(hipMemcpyAsync(h.data(), c->d_Pe, (size_t)ne * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    double s = 0.0;
+    for (double v : h) s += v;
+    *energy = s;
+    return 0;
+}

@@ -1,0 +1,50 @@
+// Host-side set-up of the assembly context: everything that does not depend
+// on the iterate U and is therefore computed once at igap4h_create instead of
+// inside every callback as the reference does.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "../../include/igap4h.h"
+
+namespace igap4h {
+
+struct Mesh {
+    int ne[3];                    // elements per direction
+    int nb[3];                    // basis functions per direction (ne + 2)
+    std::vector<double> knot[3];  // ne + 5 knots each
+    int nboun;
+    int boun[6];
+    int bc_type[36];
+    double par_dirichlet[9];
+    double par_periodic[9];
+    double *par_neumann;          // passed through to the hook
+    int64_t ndof() const { return 3LL * nb[0] * nb[1] * nb[2]; }
+    int64_t nelem() const { return (int64_t)ne[0] * ne[1] * ne[2]; }
+};
+
+// Gauss-Legendre on [0,1], ascending (src/quadrature.c:381-429 values).
+int gauss01(int n, double *x, double *w);
+
+// Cox-de Boor value / k-th derivative of basis i (mesh/nurbs/evalN.c:7-62).
+double evalN(const double *kv, int nk, int k, int i, int p, double xi);
+
+// Number of AIJ nonzeros (snesiga_init.c:341-359 row counts summed).
+int64_t pattern_nnz(const int nb[3]);
+
+// Dirichlet row list in the order of snesiga_Tangent.c:214-267.
+std::vector<int32_t> dirichlet_rows(const Mesh &m);
+
+// Final Dirichlet values per row after the INSERT sequence of
+// snesiga_Residual.c:331-409 (later faces overwrite shared rows):
+// mask[r] = 1 and g[r] = value for every row in the list.
+void dirichlet_values(const Mesh &m, std::vector<unsigned char> &mask, std::vector<double> &g);
+
+// Neumann load vector (snesiga_Residual.c:188-320); empty if hook is NULL or
+// no face carries a Neumann flag.
+std::vector<double> neumann_load(const Mesh &m, int nquad, igap4h_neumann_fn hook);
+
+// (A36, A37, A38, e0, e345, le^2) from par_mat (phys/ge/residual.c:45-47).
+void coef_from_par(const double *par, double coef[6]);
+
+} // namespace igap4h

@@ -1,0 +1,207 @@
+/*
+ * igap4h -- MI355X (gfx950) residual and tangent assembly for C1 quadratic
+ * B-spline isogeometric gradient elasticity, as a C-ABI shared library
+ * (libigap4h.so).  Plain pointers and sizes; no PETSc, no torch types.
+ *
+ * Drop-in boundary.  The reference assembles through two PETSc SNES
+ * callbacks,
+ *     PetscErrorCode snesiga_Residual(SNES, Vec U, Vec R, void *app)
+ *     PetscErrorCode snesiga_Tangent(SNES, Vec U, Mat J, Mat P, void *app)
+ * (include/snesiga.h:8-9; src/snesiga_Residual.c:13, src/snesiga_Tangent.c:13)
+ * with `app` an IGA* (include/iga.h:8-25).  The entry points below take the
+ * same IGA* at set-up and the raw arrays behind U, R and the AIJ values of P
+ * per call, so a PETSc shim around them is a few lines.
+ *
+ * The structs Phys, Nurbs, BC and IGA are declared here with the reference's
+ * layouts (include/phys.h:4-26, include/nurbs.h:4-24, include/bc.h:4-11,
+ * include/iga.h:8-25) under the reference's include guards, so this header
+ * can be included before or after the reference's own headers.
+ *
+ * Single GPU, single rank: globalIdx must be the identity and nbasis =
+ * nelem + 2 in every direction (one MPI rank owns the whole patch).
+ * Periodic directions are not supported.
+ *
+ * Status codes: 0 = success, negative = error (igap4h_last_error() has the
+ * message).  No entry point exits the process.
+ */
+#ifndef IGAP4H_H
+#define IGAP4H_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference structs (same layout; include/phys.h:4-26) ------------- */
+#if !defined(PHYS)
+#define PHYS
+typedef void (*residual_ptr)(double *residual_, double *u, double *par);
+typedef void (*tangent_ptr)(double *tangent_, double *u, double *par);
+typedef void (*field_ptr)(double *field, double *u, double *par);
+typedef void (*density_ptr)(double h[], double u[], double *par);
+typedef void (*densityb_ptr)(double h[], double u[], double traction, double *par, int order);
+typedef void (*assert_par_mat_ptr)(double *par);
+typedef struct {
+    int ndim;
+    int ndof;
+    int torder;
+    int sorder;
+    double *par_mat;
+    int ndensity;
+    int nfield;
+    residual_ptr residual;
+    tangent_ptr tangent;
+    field_ptr field;
+    density_ptr density;
+    densityb_ptr densityb;
+    assert_par_mat_ptr assert_par_mat;
+} Phys;
+#endif
+
+/* include/nurbs.h:4-24 */
+#if !defined(NURBS)
+#define NURBS
+typedef struct {
+    int porder;
+    int *nelem;
+    int *nbasis;
+    int *nknot;
+    double **knotVector;
+    int nboun;
+    int *boun;
+    int nsendPart;
+    int *sendPart;
+    int nrecvPart;
+    int *recvPart;
+    int *sendBlock;
+    int *recvBlock;
+    int *universalIdx_temp;
+    int *bc_periodic;
+    int *nelem_global;
+    double **knotVector_global;
+} Nurbs;
+#endif
+
+/* include/bc.h:4-11 */
+#if !defined(BC_)
+#define BC_
+typedef struct {
+    int *bc_type;
+    double *par_periodic;
+    double *par_dirichlet;
+    double *par_neumann;
+} BC;
+#endif
+
+/* include/iga.h:8-25 */
+#if !defined(IGA_)
+#define IGA_
+typedef struct {
+    Phys *phys;
+    Nurbs *nurbs;
+    BC *bc;
+    double *U_hist;
+    int nselfIdx;
+    int *selfIdx;
+    int nsendIdx;
+    int *sendIdx;
+    int *sendPtr;
+    int nrecvIdx;
+    int *recvIdx;
+    int *recvPtr;
+    int *globalIdx;
+} IGA;
+#endif
+
+/* ---- library types ---------------------------------------------------- */
+typedef struct igap4h_ctx igap4h_ctx;
+
+enum {
+    IGAP4H_OK = 0,
+    IGAP4H_EINVAL = -1,      /* bad argument or unsupported problem        */
+    IGAP4H_ENOMEM = -2,      /* device or host allocation failed            */
+    IGAP4H_EHIP = -3,        /* a HIP runtime call failed                   */
+    IGAP4H_ENODEV = -4       /* no gfx950 device visible                    */
+};
+
+/* physics on the path (phys/ge, phys/getime).  The reference dispatches
+ * through host function pointers (include/phys.h:20-25) that a GPU cannot
+ * call; the device code is selected by this enum instead.                 */
+enum { IGAP4H_PHYS_GE = 0, IGAP4H_PHYS_GETIME = 1, IGAP4H_PHYS_FROM_TORDER = -1 };
+
+/* precision of the tangent contraction K_e = sum_q w B^T D B             */
+enum { IGAP4H_FP64 = 0, IGAP4H_FP32 = 1 };
+
+/* element type of the CSR values array                                    */
+enum { IGAP4H_VALS_F64 = 0, IGAP4H_VALS_F32 = 1 };
+
+/* Same signature as the reference app hook (include/bc.h:14). */
+typedef void (*igap4h_neumann_fn)(double *t, double par_neumann[], int face_id, int bc_order, int idof,
+                                  double xq[]);
+
+typedef struct {
+    int physics;              /* IGAP4H_PHYS_*; FROM_TORDER: torder 0 -> ge, 2 -> getime */
+    int nquad;                /* Gauss points per direction, 1..4; 0 -> 4 (reference)   */
+    const double *elem_coef;  /* NULL: every element uses phys->par_mat; else host array
+                                 n_elem x 6 of (A36, A37, A38, e0, e345, le^2), elements
+                                 in the reference order (x slowest, z fastest)           */
+    igap4h_neumann_fn neumann;/* traction hook (NULL: zero traction); evaluated on the
+                                 host at every face Gauss point once, at create          */
+} igap4h_opts;
+
+/* ---- life cycle ------------------------------------------------------- */
+/* Replaces the per-callback set-up of snesiga_Residual.c:16-102 and
+ * snesiga_init.c:80-124 (index maps, AIJ pattern): builds the CSR pattern,
+ * Dirichlet row list and values, Neumann load and per-element coefficients
+ * on the current HIP device.  opts may be NULL (all defaults).           */
+int igap4h_create(const IGA *app, const igap4h_opts *opts, igap4h_ctx **ctx);
+void igap4h_destroy(igap4h_ctx *ctx);
+const char *igap4h_last_error(void);
+
+/* ndof = 3 prod(nbasis); nnz of the AIJ pattern; ndirichlet = length of the
+ * Dirichlet row list with duplicates (snesiga_Tangent.c:206-213).          */
+int igap4h_sizes(const igap4h_ctx *ctx, int64_t *ndof, int64_t *nnz, int64_t *ndirichlet, int64_t *nelem,
+                 int *nhist);
+
+/* ---- index sets (bit-exact with the reference) ------------------------ */
+/* AIJ pattern of P (snesiga_init.c:112-124 with iga_setnz :341-359): rows in
+ * globalIdx order, columns ascending.  Host outputs (ndof+1, nnz).        */
+int igap4h_pattern(igap4h_ctx *ctx, int64_t *rowptr, int32_t *col);
+/* Device copies owned by the context (valid until igap4h_destroy).        */
+int igap4h_pattern_dev(igap4h_ctx *ctx, const int64_t **d_rowptr, const int32_t **d_col);
+/* Rows passed to MatZeroRowsColumns (snesiga_Tangent.c:214-270), same order,
+ * duplicates kept.  Host output of length ndirichlet.                     */
+int igap4h_dirichlet_rows(igap4h_ctx *ctx, int32_t *rows);
+
+/* ---- assembly, host arrays (PCIe-inclusive) --------------------------- */
+/* snesiga_Residual (src/snesiga_Residual.c:13-417).  U: current iterate
+ * (ndof); U_hist: (torder+1) x ndof, slice 0 is overwritten with U as the
+ * reference does (:60), the other slices are the history; R: ndof output,
+ * fully overwritten (volume + Neumann, then Dirichlet rows U - g).        */
+int igap4h_residual(igap4h_ctx *ctx, const double *U, double *U_hist, double *R);
+/* snesiga_Tangent (src/snesiga_Tangent.c:13-284): values of P on the
+ * pattern, Dirichlet rows and columns zeroed with 1 on the diagonal.
+ * prec: IGAP4H_FP64 / IGAP4H_FP32 contraction; vals_type: IGAP4H_VALS_*.  */
+int igap4h_tangent(igap4h_ctx *ctx, const double *U, double *U_hist, int prec, int vals_type, void *vals);
+
+/* ---- assembly, device arrays (inputs resident in HBM) ------------------ */
+/* d_U_hist: device (torder+1) x ndof (slice 0 = current iterate); d_R:
+ * device ndof; stream: hipStream_t or NULL.  Asynchronous on the stream. */
+int igap4h_residual_dev(igap4h_ctx *ctx, const double *d_U_hist, double *d_R, void *stream);
+int igap4h_tangent_dev(igap4h_ctx *ctx, const double *d_U_hist, int prec, int vals_type, void *d_vals,
+                       void *stream);
+
+/* ---- material coefficients ------------------------------------------- */
+/* Replace the per-element coefficients (host array n_elem x 6).          */
+int igap4h_set_coef(igap4h_ctx *ctx, const double *elem_coef);
+
+/* ---- energy ----------------------------------------------------------- */
+/* Volume energy Pi = sum_e sum_q w W(u) (src/iga_intdensity.c:10-293,
+ * volume part, phys/ge density hh[0]); ge only.  d_U: device ndof.        */
+int igap4h_energy_dev(igap4h_ctx *ctx, const double *d_U, double *energy, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IGAP4H_H */
