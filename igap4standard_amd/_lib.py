@@ -74,6 +74,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libigap4h.so is not built: %s (run __graft_entry__.build())" % LIB_PATH)
+        # torch ships its own libamdhip64.so.7 (same soname as /opt/rocm's).
+        # Loading torch first makes libigap4h.so bind to that one runtime;
+        # loading this library first would map two HIP runtimes into the
+        # process and torch could no longer see the GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         vp = ctypes.c_void_p
         L.igap4h_create.argtypes = [ctypes.POINTER(IGA), ctypes.POINTER(Opts), ctypes.POINTER(vp)]

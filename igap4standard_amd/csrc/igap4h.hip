@@ -87,6 +87,12 @@ __device__ __forceinline__ f4 mfma16(float a, float b, f4 c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
+// Output row of accumulator register r in lane l (column is l % 16 for both):
+// measured on gfx950, f64 16x16x4 interleaves rows (l/16 + 4r), f32
+// 16x16x4 keeps them contiguous (4 (l/16) + r).  Operands: A row = l % 16,
+// B column = l % 16, k = l / 16 in both.
+__device__ __forceinline__ int acc_row(double, int l, int r) { return (l >> 4) + 4 * r; }
+__device__ __forceinline__ int acc_row(float, int l, int r) { return 4 * (l >> 4) + r; }
 
 // 1D basis tables of one element: phi[(d*4 + q)*9 + a*3 + k], k = value,
 // first, second derivative of function a at Gauss point q in direction d.
@@ -282,6 +288,64 @@ constexpr int TPW = 6;    // output tiles per wave (24 tiles of 16x16 over 4 wav
 __device__ __forceinline__ int ctile_block(int ct) { return ct < 6 ? 0 : (ct < 10 ? 1 : 2); }
 __device__ __forceinline__ int block_off(int i) { return i == 0 ? 0 : (i == 1 ? 96 : 160); }
 
+// Read-modify-write of the staged 81x81 element matrix into the CSR values,
+// all loads of a batch issued before its stores.
+template <class VAL, class ACC>
+__device__ __forceinline__ void scatter_rows(const ACC *sKe, const long long *sRowBase, const int *sRowSx,
+                                             const int *sRowSy, const unsigned char *sMask, VAL *vals, int tid)
+{
+    constexpr int NB = 13; // values per thread per batch (2 batches cover 81*81 with 256 threads)
+    for (int base = 0; base < 81 * 81; base += NB * TNT) {
+        long long pos[NB];
+        VAL add[NB], old[NB];
+#pragma unroll
+        for (int m = 0; m < NB; m++) {
+            int idx = base + m * TNT + tid;
+            pos[m] = -1;
+            if (idx < 81 * 81) {
+                int r = idx / 81, cc = idx % 81;
+                if (!(sMask[r] | sMask[cc])) {
+                    int B = cc / 3, j = cc % 3;
+                    pos[m] = sRowBase[r] + (B / 9) * sRowSx[r] + ((B / 3) % 3) * sRowSy[r] + 3 * (B % 3) + j;
+                    add[m] = (VAL)sKe[idx];
+                }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < NB; m++)
+            if (pos[m] >= 0) old[m] = vals[pos[m]];
+#pragma unroll
+        for (int m = 0; m < NB; m++)
+            if (pos[m] >= 0) vals[pos[m]] = old[m] + add[m];
+    }
+}
+
+// One (Gauss point, basis function B, dof block j) task of the T phase:
+// T_ij[s, B] for i <= j written straight into the LDS strip.
+template <class T, class ACC, int J, int NS, int S0>
+__device__ __forceinline__ void tangent_task(const Compact<T> *sCmp, const double *sNd, ACC *sT, const Coef &c,
+                                             double mass, const double *sw, int ql, int B, int qp, int nqp)
+{
+    double out[J + 1][10];
+    if (qp < nqp) {
+        double n[10];
+#pragma unroll
+        for (int s = 0; s < 10; s++) n[s] = sNd[(ql * 27 + B) * 10 + s];
+        tangent_apply<T, J>(sCmp[qp], c, n, mass, sw[qp], out);
+    } else {
+#pragma unroll
+        for (int i = 0; i <= J; i++)
+#pragma unroll
+            for (int s = 0; s < 10; s++) out[i][s] = 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i <= J; i++) {
+        ACC *col = sT + block_off(i) + (J - i) * 27 + B;
+#pragma unroll
+        for (int s = S0; s < 10; s++) col[(ql * NS + s - S0) * 192] = (ACC)out[i][s];
+    }
+}
+
 // One workgroup per element of one colour (ex = cx + 3 bx, ...): elements of
 // a colour share no basis function, so the read-modify-write scatter into the
 // CSR values needs no atomics and the sum order is fixed.
@@ -307,6 +371,9 @@ __global__ __launch_bounds__(TNT) void k_tangent(MeshArgs m, const double *__res
     __shared__ Compact<T> sCmp[64];
     __shared__ double sNd[QC * 27 * 10];
     __shared__ __attribute__((aligned(16))) ACC sBuf[NBUF];
+    __shared__ long long sRowBase[81]; // CSR position of column node e in row r
+    __shared__ int sRowSx[81], sRowSy[81];
+    __shared__ unsigned char sMask[81];
     ACC *sNk = sBuf;
     ACC *sT = sBuf + NKA;
 
@@ -325,6 +392,19 @@ __global__ __launch_bounds__(TNT) void k_tangent(MeshArgs m, const double *__res
         sU[idx] = Uh[h * m.ndof + 3 * nd + i];
     }
     fill_phi(m, ev, sphi, tid, TNT);
+    if (tid < 81) {
+        // row r = 3A + i of the element: CSR position of column (B, j) is
+        // base + Bx sx + By sy + 3 Bz + j (columns ascending, stencil lo..hi)
+        int A = tid / 3, i = tid % 3;
+        int a0 = ev[0] + A / 9, a1 = ev[1] + (A / 3) % 3, a2 = ev[2] + A % 3;
+        long long grow = 3 * node_of(m, a0, a1, a2) + i;
+        int lo0 = max(0, a0 - 2), lo1 = max(0, a1 - 2), lo2 = max(0, a2 - 2);
+        int w1 = min(m.nb[1] - 1, a1 + 2) - lo1 + 1, w2 = min(m.nb[2] - 1, a2 + 2) - lo2 + 1;
+        sRowBase[tid] = rowptr[grow] + 3LL * (((long long)(ev[0] - lo0) * w1 + (ev[1] - lo1)) * w2 + (ev[2] - lo2));
+        sRowSx[tid] = 3 * w1 * w2;
+        sRowSy[tid] = 3 * w2;
+        sMask[tid] = dmask[grow];
+    }
     __syncthreads();
     const Coef c = load_coef(coef, e);
 
@@ -368,23 +448,15 @@ __global__ __launch_bounds__(TNT) void k_tangent(MeshArgs m, const double *__res
             if (s >= S0) sNk[(ql * NS + s - S0) * 32 + A] = (ACC)v;
         }
         __syncthreads();
-        // (b) B operand T_ij[k][B] = w sum_t D[(i,s),(j,t)] N_{B,t}, i <= j
+        // (b) B operand T_ij[k][B] = w sum_t D[(i,s),(j,t)] N_{B,t}, i <= j.
+        // Tasks ordered j = 2, 1, 0 (costliest first) so the ragged second
+        // round gets the cheapest ones.
         for (int task = tid; task < QC * 81; task += TNT) {
-            int ql = task / 81, rem = task % 81, j = rem / 27, B = rem % 27;
+            int j = 2 - task / (QC * 27), rem = task % (QC * 27), ql = rem / 27, B = rem % 27;
             int qp = ch * QC + ql;
-            double out[3][10];
-            if (qp < nqp) {
-                double n[10];
-                for (int s = 0; s < 10; s++) n[s] = sNd[(ql * 27 + B) * 10 + s];
-                tangent_apply<T>(sCmp[qp], c, j, n, mass, sw[qp], out);
-            } else {
-                for (int i = 0; i < 3; i++)
-                    for (int s = 0; s < 10; s++) out[i][s] = 0.0;
-            }
-            for (int i = 0; i <= j; i++) {
-                ACC *col = sT + block_off(i) + (j - i) * 27 + B;
-                for (int s = S0; s < 10; s++) col[(ql * NS + s - S0) * 192] = (ACC)out[i][s];
-            }
+            if (j == 2) tangent_task<T, ACC, 2, NS, S0>(sCmp, sNd, sT, c, mass, sw, ql, B, qp, nqp);
+            else if (j == 1) tangent_task<T, ACC, 1, NS, S0>(sCmp, sNd, sT, c, mass, sw, ql, B, qp, nqp);
+            else tangent_task<T, ACC, 0, NS, S0>(sCmp, sNd, sT, c, mass, sw, ql, B, qp, nqp);
         }
         __syncthreads();
         // (c) K_i,(j>=i) += N^T T on MFMA
@@ -411,30 +483,20 @@ __global__ __launch_bounds__(TNT) void k_tangent(MeshArgs m, const double *__res
         if (cl >= (3 - i) * 27) continue;
         int j = i + cl / 27, B = cl % 27;
         for (int r = 0; r < 4; r++) {
-            int A = at * 16 + 4 * (lane >> 4) + r;
+            int A = at * 16 + acc_row((ACC)0, lane, r);
             if (A >= 27) continue;
             sKe[(3 * A + i) * 81 + 3 * B + j] = acc[t][r];
             if (i != j) sKe[(3 * B + j) * 81 + 3 * A + i] = acc[t][r];
         }
     }
     __syncthreads();
-    // scatter-add into the CSR values; Dirichlet rows/columns skipped
-    for (int idx = tid; idx < 81 * 81; idx += TNT) {
-        int r = idx / 81, cc = idx % 81;
-        int A = r / 3, i = r % 3, B = cc / 3, j = cc % 3;
-        int a0 = ev[0] + A / 9, a1 = ev[1] + (A / 3) % 3, a2 = ev[2] + A % 3;
-        int b0 = ev[0] + B / 9, b1 = ev[1] + (B / 3) % 3, b2 = ev[2] + B % 3;
-        long long grow = 3 * node_of(m, a0, a1, a2) + i;
-        long long gcol = 3 * node_of(m, b0, b1, b2) + j;
-        if (dmask[grow] | dmask[gcol]) continue;
-        int lo0 = max(0, a0 - 2), lo1 = max(0, a1 - 2), lo2 = max(0, a2 - 2);
-        int w1 = min(m.nb[1] - 1, a1 + 2) - lo1 + 1, w2 = min(m.nb[2] - 1, a2 + 2) - lo2 + 1;
-        long long pos = rowptr[grow] + 3LL * (((b0 - lo0) * w1 + (b1 - lo1)) * w2 + (b2 - lo2)) + j;
-        if (vals_f32)
-            ((float *)vals)[pos] += (float)sKe[idx];
-        else
-            ((double *)vals)[pos] += (double)sKe[idx];
-    }
+    // scatter-add into the CSR values (Dirichlet rows/columns skipped);
+    // consecutive threads take consecutive columns of a row, which are runs
+    // of 9 consecutive CSR positions
+    if (vals_f32)
+        scatter_rows<float>(sKe, sRowBase, sRowSx, sRowSy, sMask, (float *)vals, tid);
+    else
+        scatter_rows<double>(sKe, sRowBase, sRowSx, sRowSy, sMask, (double *)vals, tid);
 }
 
 // diagonal of Dirichlet rows = 1 (MatZeroRowsColumns(..., 1.0, ...))

@@ -288,12 +288,14 @@ IGAP4H_HD void compact_of(const T *u, const Coef &c, Compact<T> &cp)
     }
 }
 
-// T_ij[s, B] for i in [0, j], one basis function with slot values n[10].
-// Writes out[i][s] for s = 0..9 (value slot only when mass != 0).
-template <class T>
-IGAP4H_HD void tangent_apply(const Compact<T> &cp, const Coef &c, int j, const double n[10], double mass,
-                             double w, double out[3][10])
+// T_ij[s, B] for i in [0, J], one basis function with slot values n[10].
+// Writes out[i][s] for s = 0..9 (value slot only when mass != 0).  J is a
+// compile-time constant so that out[][] stays in registers.
+template <class T, int JB>
+IGAP4H_HD void tangent_apply(const Compact<T> &cp, const Coef &c, const double n[10], double mass, double w,
+                             double out[JB + 1][10])
 {
+    constexpr int j = JB;
     // beta = sym(F_j^T grad n) in (00,11,22,01,02,12)
     T b[6];
     b[0] = cp.F[j][0] * n[1];
@@ -319,6 +321,7 @@ IGAP4H_HD void tangent_apply(const Compact<T> &cp, const Coef &c, int j, const d
         for (int I = 0; I < 3; I++) mu[I][K] = (nu[I] - m) * (2.0 * c.lsq);
     }
     const T Sm[3][3] = {{cp.S[0], cp.S[3], cp.S[4]}, {cp.S[3], cp.S[1], cp.S[5]}, {cp.S[4], cp.S[5], cp.S[2]}};
+#pragma unroll
     for (int i = 0; i <= j; i++) {
         T tf[3], th[6];
         for (int J = 0; J < 3; J++) {
