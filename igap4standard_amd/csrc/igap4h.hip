@@ -290,17 +290,17 @@ __device__ __forceinline__ int block_off(int i) { return i == 0 ? 0 : (i == 1 ? 
 
 // Read-modify-write of the staged 81x81 element matrix into the CSR values,
 // all loads of a batch issued before its stores.
-template <class VAL, class ACC>
+template <class VAL, class ACC, int NTHR = TNT>
 __device__ __forceinline__ void scatter_rows(const ACC *sKe, const long long *sRowBase, const int *sRowSx,
                                              const int *sRowSy, const unsigned char *sMask, VAL *vals, int tid)
 {
-    constexpr int NB = 13; // values per thread per batch (2 batches cover 81*81 with 256 threads)
-    for (int base = 0; base < 81 * 81; base += NB * TNT) {
+    constexpr int NB = 13; // values per thread per batch (13 x 512 >= 81 x 81)
+    for (int base = 0; base < 81 * 81; base += NB * NTHR) {
         long long pos[NB];
         VAL add[NB], old[NB];
 #pragma unroll
         for (int m = 0; m < NB; m++) {
-            int idx = base + m * TNT + tid;
+            int idx = base + m * NTHR + tid;
             pos[m] = -1;
             if (idx < 81 * 81) {
                 int r = idx / 81, cc = idx % 81;
@@ -497,6 +497,207 @@ __global__ __launch_bounds__(TNT) void k_tangent(MeshArgs m, const double *__res
         scatter_rows<float>(sKe, sRowBase, sRowSx, sRowSy, sMask, (float *)vals, tid);
     else
         scatter_rows<double>(sKe, sRowBase, sRowSx, sRowSy, sMask, (double *)vals, tid);
+}
+
+// ---------------------------------------------------------------- ge, high occupancy
+// phys/ge tangent, 4 waves per element with a small LDS footprint so that
+// two or more elements share a CU and one element's VALU phase overlaps
+// another's MFMA phase.  Per chunk of QC Gauss points:
+//   VALU  waves 0-2 build T_ij (dof block j = 2 - wave, every lane the same
+//         code path), wave 3 builds the A operand N^T;
+//   MFMA  all 4 waves contract their 6 tiles over the chunk.
+// The per-point compact data and the T strip share one LDS region that is
+// reused to stage the 81x81 element matrix for a coalesced scatter.
+constexpr int NCMP = 48; // doubles of Compact<double>
+
+template <int QC>
+struct GeTile {
+    static constexpr int KC = QC * 9;                 // contraction depth per chunk
+    static constexpr int KCP = (KC + 3) / 4 * 4;      // padded to the MFMA k of 4
+    static constexpr int NKA = KCP * 32;              // A operand N^T[k][A]
+    static constexpr int NTS = KCP * 192;             // T strip
+};
+
+__device__ __forceinline__ void store_compact_soa(double *sC, int qp, const Compact<double> &cp)
+{
+    const double *v = reinterpret_cast<const double *>(&cp);
+#pragma unroll
+    for (int f = 0; f < NCMP; f++) sC[f * 64 + qp] = v[f];
+}
+__device__ __forceinline__ void load_compact_soa(const double *sC, int qp, Compact<double> &cp)
+{
+    double *v = reinterpret_cast<double *>(&cp);
+#pragma unroll
+    for (int f = 0; f < NCMP; f++) v[f] = sC[f * 64 + qp];
+}
+
+template <int J, class ACC>
+__device__ __forceinline__ void ge_task(const double *sCmp, const double *sphi, const int *sQ, const double *sw,
+                                        ACC *sT, const Coef &c, int ql, int B, int qp, int nqp)
+{
+    double out[J + 1][10];
+    if (qp < nqp) {
+        double n[10];
+        int q = sQ[qp];
+        basis_slots(sphi, q & 15, (q >> 4) & 15, (q >> 8) & 15, B, n);
+        Compact<double> cp;
+        load_compact_soa(sCmp, qp, cp);
+        tangent_apply<double, J>(cp, c, n, 0.0, sw[qp], out);
+    } else {
+#pragma unroll
+        for (int i = 0; i <= J; i++)
+#pragma unroll
+            for (int s = 0; s < 10; s++) out[i][s] = 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i <= J; i++) {
+        ACC *col = sT + block_off(i) + (J - i) * 27 + B;
+#pragma unroll
+        for (int s = 1; s < 10; s++) col[(ql * 9 + s - 1) * 192] = (ACC)out[i][s];
+    }
+}
+
+template <class ACC, int QC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_tangent_ge(MeshArgs m, const double *__restrict__ Uh,
+                                                    const double *__restrict__ coef,
+                                                    const long long *__restrict__ rowptr,
+                                                    const unsigned char *__restrict__ dmask, void *__restrict__ vals,
+                                                    int vals_f32, int cx, int cy, int cz, int my, int mz)
+{
+    using G = GeTile<QC>;
+    using V = typename V4<ACC>::t;
+    // region: [compact SoA | A operand | T strip], reused for the element matrix
+    constexpr int CMP_BYTES = NCMP * 64 * 8;
+    constexpr int OP_BYTES = (G::NKA + G::NTS) * (int)sizeof(ACC);
+    constexpr int KE_BYTES = 81 * 81 * (int)sizeof(ACC);
+    constexpr int REGION = (CMP_BYTES + OP_BYTES) > KE_BYTES ? (CMP_BYTES + OP_BYTES) : KE_BYTES;
+    __shared__ __attribute__((aligned(16))) unsigned char sRegion[REGION];
+    __shared__ double sU[81];
+    __shared__ double sphi[3 * 4 * 9];
+    __shared__ double sw[64];
+    __shared__ int sQ[64];
+    __shared__ long long sRowBase[81];
+    __shared__ int sRowSx[81], sRowSy[81];
+    __shared__ unsigned char sMask[81];
+    double *sCmp = reinterpret_cast<double *>(sRegion);
+    ACC *sNk = reinterpret_cast<ACC *>(sRegion + CMP_BYTES);
+    ACC *sT = sNk + G::NKA;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int b = blockIdx.x;
+    int ev[3];
+    ev[2] = cz + 3 * (b % mz);
+    ev[1] = cy + 3 * ((b / mz) % my);
+    ev[0] = cx + 3 * (b / (mz * my));
+    const long long e = ((long long)ev[0] * m.ne[1] + ev[1]) * m.ne[2] + ev[2];
+    const int nq = m.nq, nqp = nq * nq * nq;
+
+    if (tid < 81) {
+        int A = tid / 3, i = tid % 3;
+        int a0 = ev[0] + A / 9, a1 = ev[1] + (A / 3) % 3, a2 = ev[2] + A % 3;
+        long long grow = 3 * node_of(m, a0, a1, a2) + i;
+        sU[tid] = Uh[grow];
+        int lo0 = max(0, a0 - 2), lo1 = max(0, a1 - 2), lo2 = max(0, a2 - 2);
+        int w1 = min(m.nb[1] - 1, a1 + 2) - lo1 + 1, w2 = min(m.nb[2] - 1, a2 + 2) - lo2 + 1;
+        sRowBase[tid] = rowptr[grow] + 3LL * (((long long)(ev[0] - lo0) * w1 + (ev[1] - lo1)) * w2 + (ev[2] - lo2));
+        sRowSx[tid] = 3 * w1 * w2;
+        sRowSy[tid] = 3 * w2;
+        sMask[tid] = dmask[grow];
+    } else if (tid >= 128 && tid < 192) {
+        int qp = tid - 128;
+        sQ[qp] = qp < nqp ? ((qp / (nq * nq)) | (((qp / nq) % nq) << 4) | ((qp % nq) << 8)) : 0;
+    }
+    // padded contraction rows stay zero in both operands
+    for (int idx = tid; idx < (G::KCP - G::KC) * 224; idx += 256) {
+        int k = G::KC + idx / 224, col = idx % 224;
+        if (col < 32) sNk[k * 32 + col] = (ACC)0;
+        else sT[k * 192 + col - 32] = (ACC)0;
+    }
+    fill_phi(m, ev, sphi, tid, 256);
+    __syncthreads();
+    const Coef c = load_coef(coef, e);
+
+    if (tid < nqp) {
+        int q = sQ[tid];
+        double u[1][30], w;
+        interp_slots<1>(m, sU, sphi, ev, q & 15, (q >> 4) & 15, (q >> 8) & 15, u, &w);
+        Compact<double> cp;
+        compact_of<double>(u[0], c, cp);
+        store_compact_soa(sCmp, tid, cp);
+        sw[tid] = w;
+    }
+    __syncthreads();
+
+    V acc[6];
+#pragma unroll
+    for (int t = 0; t < 6; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[t][r] = (ACC)0;
+
+    const int nchunk = (nqp + QC - 1) / QC;
+    for (int ch = 0; ch < nchunk; ch++) {
+        if (wave < 3) {
+            // T_ij for dof block j = 2 - wave; lane = (ql, B)
+            for (int t = lane; t < QC * 27; t += 64) {
+                int ql = t / 27, B = t % 27, qp = ch * QC + ql;
+                if (wave == 0) ge_task<2, ACC>(sCmp, sphi, sQ, sw, sT, c, ql, B, qp, nqp);
+                else if (wave == 1) ge_task<1, ACC>(sCmp, sphi, sQ, sw, sT, c, ql, B, qp, nqp);
+                else ge_task<0, ACC>(sCmp, sphi, sQ, sw, sT, c, ql, B, qp, nqp);
+            }
+        } else {
+            for (int idx = lane; idx < G::KC * 27; idx += 64) {
+                int k = idx / 27, A = idx % 27, ql = k / 9, s = 1 + k % 9;
+                int qp = ch * QC + ql;
+                double v = 0.0;
+                if (qp < nqp) {
+                    int q = sQ[qp];
+                    const double *px = sphi + (0 * 4 + (q & 15)) * 9 + (A / 9) * 3;
+                    const double *py = sphi + (1 * 4 + ((q >> 4) & 15)) * 9 + ((A / 3) % 3) * 3;
+                    const double *pz = sphi + (2 * 4 + ((q >> 8) & 15)) * 9 + (A % 3) * 3;
+                    v = px[kSlotD[s][0]] * py[kSlotD[s][1]] * pz[kSlotD[s][2]];
+                }
+                sNk[k * 32 + A] = (ACC)v;
+            }
+        }
+        __syncthreads();
+        const int ct0 = 3 * wave;
+#pragma unroll
+        for (int ks = 0; ks < G::KCP / 4; ks++) {
+            int k = 4 * ks + (lane >> 4);
+            ACC a0 = sNk[k * 32 + (lane & 15)];
+            ACC a1 = sNk[k * 32 + 16 + (lane & 15)];
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+                ACC bb = sT[k * 192 + (ct0 + t) * 16 + (lane & 15)];
+                acc[2 * t] = mfma16(a0, bb, acc[2 * t]);
+                acc[2 * t + 1] = mfma16(a1, bb, acc[2 * t + 1]);
+            }
+        }
+        __syncthreads();
+    }
+
+    // stage the 81x81 element matrix (mirror the i < j blocks), then scatter
+    ACC *sKe = reinterpret_cast<ACC *>(sRegion);
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+        int ct = 3 * wave + t / 2, at = t & 1;
+        int i = ctile_block(ct);
+        int cl = ct * 16 + (lane & 15) - block_off(i);
+        if (cl >= (3 - i) * 27) continue;
+        int j = i + cl / 27, B = cl % 27;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            int A = at * 16 + acc_row((ACC)0, lane, r);
+            if (A >= 27) continue;
+            sKe[(3 * A + i) * 81 + 3 * B + j] = acc[t][r];
+            if (i != j) sKe[(3 * B + j) * 81 + 3 * A + i] = acc[t][r];
+        }
+    }
+    __syncthreads();
+    if (vals_f32)
+        scatter_rows<float, ACC, 256>(sKe, sRowBase, sRowSx, sRowSy, sMask, (float *)vals, tid);
+    else
+        scatter_rows<double, ACC, 256>(sKe, sRowBase, sRowSx, sRowSy, sMask, (double *)vals, tid);
 }
 
 // diagonal of Dirichlet rows = 1 (MatZeroRowsColumns(..., 1.0, ...))
@@ -953,6 +1154,29 @@ static int launch_tangent(igap4h_ctx *c, const MeshArgs &m, const double *d_Uh, 
     return 0;
 }
 
+// Gauss points per chunk: 2 for FP64 (keeps two elements per CU), 4 for FP32.
+template <class ACC>
+struct GeQC { static constexpr int v = 2; };
+template <>
+struct GeQC<float> { static constexpr int v = 4; };
+
+template <class ACC>
+static int launch_tangent_ge(igap4h_ctx *c, const MeshArgs &m, const double *d_Uh, void *d_vals, int f32, hipStream_t st)
+{
+    for (int cx = 0; cx < 3; cx++)
+        for (int cy = 0; cy < 3; cy++)
+            for (int cz = 0; cz < 3; cz++) {
+                int mx = (m.ne[0] - cx + 2) / 3, my = (m.ne[1] - cy + 2) / 3, mz = (m.ne[2] - cz + 2) / 3;
+                long long nb = (long long)mx * my * mz;
+                if (nb <= 0) continue;
+                hipLaunchKernelGGL((k_tangent_ge<ACC, GeQC<ACC>::v>), dim3((unsigned)nb), dim3(256), 0, st, m, d_Uh,
+                                   (const double *)c->d_coef, (const long long *)c->d_rowptr,
+                                   (const unsigned char *)c->d_mask, d_vals, f32, cx, cy, cz, my, mz);
+                HIPCHK(hipGetLastError());
+            }
+    return 0;
+}
+
 extern "C" int igap4h_tangent_dev(igap4h_ctx *c, const double *d_Uh, int prec, int vals_type, void *d_vals, void *stream)
 {
     if (!c || !d_Uh || !d_vals) return fail(IGAP4H_EINVAL, "NULL argument");
@@ -967,8 +1191,8 @@ extern "C" int igap4h_tangent_dev(igap4h_ctx *c, const double *d_Uh, int prec, i
     double mass = c->physics == IGAP4H_PHYS_GETIME ? c->par_mat[5] / (c->par_mat[7] * c->par_mat[7]) + c->par_mat[6] / (2. * c->par_mat[7]) : 0.0;
     int rc;
     if (c->physics == IGAP4H_PHYS_GE)
-        rc = prec == IGAP4H_FP64 ? launch_tangent<0, double>(c, m, d_Uh, mass, d_vals, f32, st)
-                                 : launch_tangent<0, float>(c, m, d_Uh, mass, d_vals, f32, st);
+        rc = prec == IGAP4H_FP64 ? launch_tangent_ge<double>(c, m, d_Uh, d_vals, f32, st)
+                                 : launch_tangent_ge<float>(c, m, d_Uh, d_vals, f32, st);
     else
         rc = prec == IGAP4H_FP64 ? launch_tangent<1, double>(c, m, d_Uh, mass, d_vals, f32, st)
                                  : launch_tangent<1, float>(c, m, d_Uh, mass, d_vals, f32, st);
