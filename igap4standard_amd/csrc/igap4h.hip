@@ -442,7 +442,7 @@ __global__ __launch_bounds__(TNT) void k_tangent(MeshArgs m, const double *__res
                 const double *px = sphi + (0 * 4 + qx) * 9 + (A / 9) * 3;
                 const double *py = sphi + (1 * 4 + qy) * 9 + ((A / 3) % 3) * 3;
                 const double *pz = sphi + (2 * 4 + qz) * 9 + (A % 3) * 3;
-                v = px[kSlotD[s][0]] * py[kSlotD[s][1]] * pz[kSlotD[s][2]];
+                v = px[slot_d(s, 0)] * py[slot_d(s, 1)] * pz[slot_d(s, 2)];
             }
             sNd[idx] = v;
             if (s >= S0) sNk[(ql * NS + s - S0) * 32 + A] = (ACC)v;
@@ -514,7 +514,7 @@ template <int QC>
 struct GeTile {
     static constexpr int KC = QC * 9;                 // contraction depth per chunk
     static constexpr int KCP = (KC + 3) / 4 * 4;      // padded to the MFMA k of 4
-    static constexpr int NKA = KCP * 32;              // A operand N^T[k][A]
+    static constexpr int NKA = 0;                     // A operand formed in registers
     static constexpr int NTS = KCP * 192;             // T strip
 };
 
@@ -566,7 +566,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 {
     using G = GeTile<QC>;
     using V = typename V4<ACC>::t;
-    // region: [compact SoA | A operand | T strip], reused for the element matrix
+    // region: [compact SoA | T strip], reused for the element matrix
     constexpr int CMP_BYTES = NCMP * 64 * 8;
     constexpr int OP_BYTES = (G::NKA + G::NTS) * (int)sizeof(ACC);
     constexpr int KE_BYTES = 81 * 81 * (int)sizeof(ACC);
@@ -607,12 +607,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         int qp = tid - 128;
         sQ[qp] = qp < nqp ? ((qp / (nq * nq)) | (((qp / nq) % nq) << 4) | ((qp % nq) << 8)) : 0;
     }
-    // padded contraction rows stay zero in both operands
-    for (int idx = tid; idx < (G::KCP - G::KC) * 224; idx += 256) {
-        int k = G::KC + idx / 224, col = idx % 224;
-        if (col < 32) sNk[k * 32 + col] = (ACC)0;
-        else sT[k * 192 + col - 32] = (ACC)0;
-    }
+    // padded contraction rows of the T strip stay zero
+    for (int idx = tid; idx < (G::KCP - G::KC) * 192; idx += 256) sT[G::KC * 192 + idx] = (ACC)0;
     fill_phi(m, ev, sphi, tid, 256);
     __syncthreads();
     const Coef c = load_coef(coef, e);
@@ -634,6 +630,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
         for (int r = 0; r < 4; r++) acc[t][r] = (ACC)0;
 
+    // A-operand coordinates of this lane: rows A0 = lane % 16 and A1 = A0 + 16
+    // (valid when < 27) of N^T, basis-table offsets per direction
+    const int A0 = lane & 15, A1 = A0 + 16, krow = lane >> 4;
+    const int ox0 = (A0 / 9) * 3, oy0 = ((A0 / 3) % 3) * 3, oz0 = (A0 % 3) * 3;
+    const int ox1 = (A1 < 27 ? A1 / 9 : 0) * 3, oy1 = (A1 < 27 ? (A1 / 3) % 3 : 0) * 3, oz1 = (A1 < 27 ? A1 % 3 : 0) * 3;
+    const double a1mask = A1 < 27 ? 1.0 : 0.0;
+
     const int nchunk = (nqp + QC - 1) / QC;
     for (int ch = 0; ch < nchunk; ch++) {
         if (wave < 3) {
@@ -644,28 +647,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 else if (wave == 1) ge_task<1, ACC>(sCmp, sphi, sQ, sw, sT, c, ql, B, qp, nqp);
                 else ge_task<0, ACC>(sCmp, sphi, sQ, sw, sT, c, ql, B, qp, nqp);
             }
-        } else {
-            for (int idx = lane; idx < G::KC * 27; idx += 64) {
-                int k = idx / 27, A = idx % 27, ql = k / 9, s = 1 + k % 9;
-                int qp = ch * QC + ql;
-                double v = 0.0;
-                if (qp < nqp) {
-                    int q = sQ[qp];
-                    const double *px = sphi + (0 * 4 + (q & 15)) * 9 + (A / 9) * 3;
-                    const double *py = sphi + (1 * 4 + ((q >> 4) & 15)) * 9 + ((A / 3) % 3) * 3;
-                    const double *pz = sphi + (2 * 4 + ((q >> 8) & 15)) * 9 + (A % 3) * 3;
-                    v = px[kSlotD[s][0]] * py[kSlotD[s][1]] * pz[kSlotD[s][2]];
-                }
-                sNk[k * 32 + A] = (ACC)v;
-            }
         }
         __syncthreads();
         const int ct0 = 3 * wave;
 #pragma unroll
         for (int ks = 0; ks < G::KCP / 4; ks++) {
-            int k = 4 * ks + (lane >> 4);
-            ACC a0 = sNk[k * 32 + (lane & 15)];
-            ACC a1 = sNk[k * 32 + 16 + (lane & 15)];
+            const int k = 4 * ks + krow;
+            // A fragments N_{A,s}(q) from the per-element basis tables
+            double v0 = 0.0, v1 = 0.0;
+            const int ql = k / 9, sl = 1 + k % 9, qp = ch * QC + ql;
+            if (k < G::KC && qp < nqp) {
+                const int q = sQ[qp];
+                const double *px = sphi + (q & 15) * 9 + slot_d(sl, 0);
+                const double *py = sphi + (4 + ((q >> 4) & 15)) * 9 + slot_d(sl, 1);
+                const double *pz = sphi + (8 + ((q >> 8) & 15)) * 9 + slot_d(sl, 2);
+                v0 = px[ox0] * py[oy0] * pz[oz0];
+                v1 = a1mask * (px[ox1] * py[oy1] * pz[oz1]);
+            }
+            const ACC a0 = (ACC)v0, a1 = (ACC)v1;
 #pragma unroll
             for (int t = 0; t < 3; t++) {
                 ACC bb = sT[k * 192 + (ct0 + t) * 16 + (lane & 15)];

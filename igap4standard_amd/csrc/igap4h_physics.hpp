@@ -376,8 +376,12 @@ IGAP4H_HD void bspline1d(const double *k, double x, double v[3], double d1[3], d
     }
 }
 
-// derivative orders (x, y, z) of slot s
-__device__ __constant__ const int kSlotD[10][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0},
-                                                   {1, 1, 0}, {1, 0, 1}, {0, 2, 0}, {0, 1, 1}, {0, 0, 2}};
+// derivative order in direction d (0..2) of slot s (0..9): slots (value, x,
+// y, z, xx, xy, xz, yy, yz, zz) packed 2 bits per direction, 6 bits per
+// slot, in an immediate (a per-lane table lookup would be a memory load)
+IGAP4H_HD int slot_d(int s, int d)
+{
+    return (int)((0x814211142404040ULL >> (6 * s + 2 * d)) & 3ULL);
+}
 
 } // namespace igap4h
