@@ -27,6 +27,19 @@ FLOP_KE_PER_QP = 9 * (2 * 10 * 10 * 27 + 2 * 27 * 10 * 27)
 PEAK = {"f64": 78.6, "f32": 157.3}   # MI355X dense MFMA TFLOP/s
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE runs, scaled by a plain copy)."""
+    path = os.path.join(ROOT, "profiles", "k_tangent_ge_64cube_traffic.txt")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch") and kernel in line:
+                return float(line.split("=")[1].split()[0])
+    except OSError:
+        pass
+    return None
+
+
 def metric_name():
     try:
         with open(os.path.join(ROOT, "BASELINE.json")) as f:
@@ -70,7 +83,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--forms", default="fp64,mixed", help="comma list of fp64, mixed")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.calib_copy:
+        import torch
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes": a.numel() * 8}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the assembly path is single-GPU in this build "
               "(multi-GPU sharding is a design note, not implemented); run with --gpus 1", file=sys.stderr)
@@ -150,8 +174,9 @@ def main():
                                "clamp/stretch a=0.01, Lame +-10%% per element" % n,
                    "nelem": pb.n_elem, "ndof": app.ndof, "nnz": app.nnz, "nquad": 4, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK["f64"], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK["f64"], "traffic": None,
-                     "kernel": "k_tangent<ge,double> (27 colour launches per tangent)",
+                     "frac": achieved / PEAK["f64"], "traffic": profiled_traffic("k_tangent_ge<double,2>"),
+                     "traffic_source": "profiles/k_tangent_ge_64cube_traffic.txt (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1)",
+                     "kernel": "k_tangent_ge<double,2> (27 colour launches per tangent)",
                      "algorithmic": "11,508,480 flop per element (9 x (2*10*10*27 + 2*27*10*27) per Gauss point)"},
         "mixed": None if "mixed" not in res else {
             **res["mixed"], "roofline_frac_f32": res["mixed"]["tangent_tflops"] / PEAK["f32"],
