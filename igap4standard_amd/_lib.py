@@ -58,11 +58,38 @@ class Opts(ctypes.Structure):
 IGAP4H_FP64, IGAP4H_FP32 = 0, 1
 IGAP4H_VALS_F64, IGAP4H_VALS_F32 = 0, 1
 IGAP4H_PHYS_GE, IGAP4H_PHYS_GETIME, IGAP4H_PHYS_FROM_TORDER = 0, 1, -1
+IGAP4H_PC_NONE, IGAP4H_PC_BJACOBI3, IGAP4H_PC_SGS, IGAP4H_PC_MG = 0, 1, 2, 3
+IGAP4H_KSP_GMRES, IGAP4H_KSP_CG = 0, 1
+
+
+class NewtonOpts(ctypes.Structure):
+    """igap4h_newton_opts (include/igap4h.h)."""
+    _fields_ = [("atol", ctypes.c_double), ("rtol", ctypes.c_double), ("max_newton", ctypes.c_int),
+                ("tangent_prec", ctypes.c_int), ("vals_type", ctypes.c_int), ("ksp_type", ctypes.c_int),
+                ("restart", ctypes.c_int), ("ksp_rtol", ctypes.c_double), ("ksp_max_it", ctypes.c_int),
+                ("ksp_guess", ctypes.c_int), ("pc", ctypes.c_int), ("line_search", ctypes.c_int),
+                ("stagnation", ctypes.c_int), ("dirichlet_lift", ctypes.c_int)]
+
+
+class NewtonInfo(ctypes.Structure):
+    """igap4h_newton_info (include/igap4h.h)."""
+    _fields_ = [("converged", ctypes.c_int), ("newton_its", ctypes.c_int), ("ksp_its", ctypes.c_int),
+                ("fnorm0", ctypes.c_double), ("fnorm", ctypes.c_double), ("t_residual", ctypes.c_double),
+                ("t_tangent", ctypes.c_double), ("t_solve", ctypes.c_double), ("t_total", ctypes.c_double),
+                ("fnorm_hist", ctypes.c_double * 64), ("ksp_its_hist", ctypes.c_int * 64)]
+
+    def as_dict(self):
+        k = min(self.newton_its + 1, 64)
+        return {"converged": self.converged, "newton_its": self.newton_its, "ksp_its": self.ksp_its,
+                "fnorm0": self.fnorm0, "fnorm": self.fnorm, "t_residual": self.t_residual,
+                "t_tangent": self.t_tangent, "t_solve": self.t_solve, "t_total": self.t_total,
+                "fnorm_hist": list(self.fnorm_hist[:k]), "ksp_its_hist": list(self.ksp_its_hist[:k - 1])}
 
 # every entry point declared in include/igap4h.h
 SYMBOLS = ("igap4h_create", "igap4h_destroy", "igap4h_last_error", "igap4h_sizes", "igap4h_pattern",
            "igap4h_pattern_dev", "igap4h_dirichlet_rows", "igap4h_residual", "igap4h_tangent",
-           "igap4h_residual_dev", "igap4h_tangent_dev", "igap4h_set_coef", "igap4h_energy_dev")
+           "igap4h_residual_dev", "igap4h_tangent_dev", "igap4h_set_coef", "igap4h_energy_dev",
+           "igap4h_newton_default_opts", "igap4h_newton_solve_dev", "igap4h_spmv_dev", "igap4h_linear_solve_dev")
 
 _lib = None
 
@@ -98,6 +125,11 @@ def lib():
         L.igap4h_tangent_dev.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp, vp]
         L.igap4h_set_coef.argtypes = [vp, c_double_p]
         L.igap4h_energy_dev.argtypes = [vp, vp, c_double_p, vp]
+        L.igap4h_newton_default_opts.argtypes = [ctypes.POINTER(NewtonOpts)]
+        L.igap4h_newton_solve_dev.argtypes = [vp, vp, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonInfo), vp]
+        L.igap4h_spmv_dev.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp]
+        L.igap4h_linear_solve_dev.argtypes = [vp, vp, ctypes.c_int, vp, vp, ctypes.POINTER(NewtonOpts), c_int_p,
+                                              c_double_p, vp]
         for name in SYMBOLS:
             if name not in ("igap4h_destroy", "igap4h_last_error"):
                 getattr(L, name).restype = ctypes.c_int

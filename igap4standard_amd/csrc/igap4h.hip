@@ -20,12 +20,13 @@
 #include "../../include/igap4h.h"
 #include "igap4h_physics.hpp"
 #include "igap4h_setup.hpp"
+#include "igap4h_internal.hpp"
 
 using namespace igap4h;
 
 // ============================================================== errors
 static thread_local std::string g_err;
-static int fail(int code, const char *fmt, ...)
+int igap4h::fail(int code, const char *fmt, ...)
 {
     char buf[512];
     va_list ap;
@@ -35,11 +36,6 @@ static int fail(int code, const char *fmt, ...)
     g_err = buf;
     return code;
 }
-#define HIPCHK(call)                                                                          \
-    do {                                                                                      \
-        hipError_t e_ = (call);                                                               \
-        if (e_ != hipSuccess) return fail(IGAP4H_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
-    } while (0)
 
 extern "C" const char *igap4h_last_error(void) { return g_err.c_str(); }
 
@@ -629,6 +625,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     for (int t = 0; t < 6; t++)
 #pragma unroll
         for (int r = 0; r < 4; r++) acc[t][r] = (ACC)0;
+    // FP32 MFMA partial sums are added into FP64 after every chunk, so the
+    // FP32 rounding of the accumulation spans one chunk, not the element
+    double accd[6][4];
+#pragma unroll
+    for (int t = 0; t < 6; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) accd[t][r] = 0.0;
 
     // A-operand coordinates of this lane: rows A0 = lane % 16 and A1 = A0 + 16
     // (valid when < 27) of N^T, basis-table offsets per direction
@@ -672,7 +675,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 acc[2 * t + 1] = mfma16(a1, bb, acc[2 * t + 1]);
             }
         }
+        if (sizeof(ACC) == 4) {
+#pragma unroll
+            for (int t = 0; t < 6; t++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    accd[t][r] += (double)acc[t][r];
+                    acc[t][r] = (ACC)0;
+                }
+        }
         __syncthreads();
+    }
+    if (sizeof(ACC) == 4) {
+#pragma unroll
+        for (int t = 0; t < 6; t++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) acc[t][r] = (ACC)accd[t][r];
     }
 
     // stage the 81x81 element matrix (mirror the i < j blocks), then scatter
@@ -816,31 +834,6 @@ __global__ __launch_bounds__(64) void k_energy_elem(MeshArgs m, const double *__
 }
 
 // ============================================================== context
-struct igap4h_ctx {
-    Mesh mesh;
-    int physics, nquad, nhist;
-    int device;
-    double par_mat[9];
-    double cq[4], wq[4];
-    int64_t nnz;
-    std::vector<int32_t> drows;
-    // device
-    double *d_knot[3] = {nullptr, nullptr, nullptr};
-    double *d_coef = nullptr;
-    long long *d_rowptr = nullptr;
-    int *d_col = nullptr;
-    unsigned char *d_mask = nullptr;
-    double *d_g = nullptr;
-    double *d_fneu = nullptr;
-    int *d_drows = nullptr;
-    double *d_Re = nullptr;      // n_elem x 81
-    double *d_Uh = nullptr;      // staging for the host API
-    double *d_R = nullptr;
-    void *d_vals = nullptr;      // staging for the host API
-    size_t d_vals_bytes = 0;
-    double *d_Pe = nullptr;
-    double *U_hist_app = nullptr;
-};
 
 static MeshArgs mesh_args(const igap4h_ctx *c)
 {
@@ -863,17 +856,6 @@ static MeshArgs mesh_args(const igap4h_ctx *c)
     return m;
 }
 
-template <class X>
-static int dev_alloc(X **p, size_t n)
-{
-    if (n == 0) n = 1;
-    if (hipMalloc((void **)p, n * sizeof(X)) != hipSuccess) {
-        *p = nullptr;
-        return fail(IGAP4H_ENOMEM, "hipMalloc of %zu bytes failed", n * sizeof(X));
-    }
-    return 0;
-}
-
 static void ctx_free(igap4h_ctx *c)
 {
     if (!c) return;
@@ -893,6 +875,7 @@ static void ctx_free(igap4h_ctx *c)
     (void)hipFree(c->d_R);
     (void)hipFree(c->d_vals);
     (void)hipFree(c->d_Pe);
+    solver_free(c);
     if (cur >= 0) (void)hipSetDevice(cur);
     delete c;
 }

@@ -1,6 +1,7 @@
 // Host-side set-up of the assembly context (see igap4h_setup.hpp).
 #include "igap4h_setup.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -266,4 +267,68 @@ void coef_from_par(const double *par, double coef[6])
     coef[5] = par[4] * par[4];
 }
 
-} // namespace igap4h
+std::vector<double> coarsen_knots(const std::vector<double> &kf, int nef)
+{
+    std::vector<double> kc;
+    kc.push_back(kf[0]);
+    kc.push_back(kf[1]);
+    for (int e = 0; e <= nef; e += 2) kc.push_back(kf[2 + e]);  // element boundaries 0, 2, ..., nef
+    kc.push_back(kf[nef + 3]);
+    kc.push_back(kf[nef + 4]);
+    return kc;
+}
+
+int prolongation_1d(const std::vector<double> &kf, int nef, const std::vector<double> &kc, int nec, std::vector<double> &P)
+{
+    const int p = 2, nbf = nef + p, nbc = nec + p;
+    const int nkf = nef + 5, nkc = nec + 5;
+    // collocation points: Greville abscissae, the two ends moved inside
+    std::vector<double> x(nbf);
+    for (int a = 0; a < nbf; a++) x[a] = 0.5 * (kf[a + 1] + kf[a + 2]);
+    double h0 = kf[3] - kf[2], h1 = kf[nef + 2] - kf[nef + 1];
+    x[0] = kf[2] + 0.25 * h0;
+    x[nbf - 1] = kf[nef + 2] - 0.25 * h1;
+    // B[a][f] = N^f_f(x_a), C[a][A] = N^c_A(x_a); solve B P = C
+    std::vector<double> B((size_t)nbf * nbf), C((size_t)nbf * nbc);
+    for (int a = 0; a < nbf; a++) {
+        for (int f = 0; f < nbf; f++) B[(size_t)a * nbf + f] = evalN(kf.data(), nkf, 0, f, p, x[a]);
+        for (int A = 0; A < nbc; A++) C[(size_t)a * nbc + A] = evalN(kc.data(), nkc, 0, A, p, x[a]);
+    }
+    for (int k = 0; k < nbf; k++) {  // Gaussian elimination, partial pivoting
+        int piv = k;
+        for (int r = k + 1; r < nbf; r++)
+            if (std::fabs(B[(size_t)r * nbf + k]) > std::fabs(B[(size_t)piv * nbf + k])) piv = r;
+        if (std::fabs(B[(size_t)piv * nbf + k]) < 1e-300) return -1;
+        if (piv != k) {
+            for (int c = 0; c < nbf; c++) std::swap(B[(size_t)k * nbf + c], B[(size_t)piv * nbf + c]);
+            for (int c = 0; c < nbc; c++) std::swap(C[(size_t)k * nbc + c], C[(size_t)piv * nbc + c]);
+        }
+        for (int r = 0; r < nbf; r++) {
+            if (r == k) continue;
+            double f = B[(size_t)r * nbf + k] / B[(size_t)k * nbf + k];
+            if (f == 0.0) continue;
+            for (int c = k; c < nbf; c++) B[(size_t)r * nbf + c] -= f * B[(size_t)k * nbf + c];
+            for (int c = 0; c < nbc; c++) C[(size_t)r * nbc + c] -= f * C[(size_t)k * nbc + c];
+        }
+    }
+    P.assign((size_t)nbf * nbc, 0.0);
+    for (int a = 0; a < nbf; a++)
+        for (int A = 0; A < nbc; A++) {
+            double v = C[(size_t)a * nbc + A] / B[(size_t)a * nbf + a];
+            P[(size_t)a * nbc + A] = std::fabs(v) < 1e-13 ? 0.0 : v;
+        }
+    // nested: the coarse functions are reproduced at a second point set
+    std::vector<double> Nf(nbf);
+    for (int a = 0; a < nbf; a++) {
+        double xm = 0.5 * (x[a] + (a + 1 < nbf ? x[a + 1] : kf[nef + 2]));
+        for (int f = 0; f < nbf; f++) Nf[f] = evalN(kf.data(), nkf, 0, f, p, xm);
+        for (int A = 0; A < nbc; A++) {
+            double s = 0.0;
+            for (int f = 0; f < nbf; f++) s += P[(size_t)f * nbc + A] * Nf[f];
+            if (std::fabs(s - evalN(kc.data(), nkc, 0, A, p, xm)) > 1e-10) return -1;
+        }
+    }
+    return 0;
+}
+
+}  // namespace igap4h

@@ -47,4 +47,17 @@ std::vector<double> neumann_load(const Mesh &m, int nquad, igap4h_neumann_fn hoo
 // (A36, A37, A38, e0, e345, le^2) from par_mat (phys/ge/residual.c:45-47).
 void coef_from_par(const double *par, double coef[6]);
 
+// Dyadic coarsening of an open quadratic knot vector: kf has nef + 5 knots;
+// kc keeps every other element boundary (nef even), nef / 2 + 5 knots.
+std::vector<double> coarsen_knots(const std::vector<double> &kf, int nef);
+
+// Knot-insertion prolongation between nested quadratic spline spaces:
+// P (nbf x nbc, row-major) with N^c_A = sum_a P[a][A] N^f_a, so coarse
+// control values c map to fine ones P c (the effect of inserting the
+// missing knots, mesh/nurbs/nurbs_knotinsert.c:7-107).  Computed exactly by
+// collocation at one point inside each fine function's support.  Returns 0,
+// or -1 if the spaces are not nested.
+int prolongation_1d(const std::vector<double> &kf, int nef, const std::vector<double> &kc, int nec,
+                    std::vector<double> &P);
+
 } // namespace igap4h

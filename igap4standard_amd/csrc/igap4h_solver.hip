@@ -1,0 +1,1462 @@
+// Newton-Krylov driver on one GPU: the counterpart of the PETSc SNES/KSP set-up
+// of src/snes_init.c:8-47 and the outer loop of src/snes_solve.c:10-36, for the
+// residual and tangent of this library.  C-ABI in include/igap4h.h.
+//
+// Matrix: the AIJ pattern of the assembly (rows (a0,a1,a2,i), columns the
+// 3 dofs of every node b with |a_d - b_d| <= 2, ascending).  The three rows of
+// a node share one column list of length L = 3 w0 w1 w2 and sit back to back
+// (rowptr[3n+i] = rowptr[3n] + i L), so column indices are computed, not read:
+// an SpMV streams only the values (8 or 4 bytes per nonzero).
+//
+// Every kernel here is one wave per node (row work) or a grid-stride vector
+// pass; reductions write per-block partials that a second kernel sums in a
+// fixed order, so results do not depend on scheduling.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "igap4h_internal.hpp"
+
+using namespace igap4h;
+
+namespace {
+
+constexpr int NRB = 1024;  // blocks of a reduction pass = partials per sum
+constexpr int JT = 8;      // vectors per tile of the block dot product
+
+struct SpArgs {
+    int nb[3];
+    long long nnode;
+    const long long *rowptr;
+};
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// node -> grid coordinates and the stencil box [lo, lo + w) per direction
+__device__ __forceinline__ void node_box(const SpArgs &s, long long node, int a[3], int lo[3], int w[3])
+{
+    a[2] = (int)(node % s.nb[2]);
+    a[1] = (int)((node / s.nb[2]) % s.nb[1]);
+    a[0] = (int)(node / ((long long)s.nb[2] * s.nb[1]));
+    for (int d = 0; d < 3; d++) {
+        lo[d] = max(0, a[d] - 2);
+        w[d] = min(s.nb[d] - 1, a[d] + 2) - lo[d] + 1;
+    }
+}
+
+// acc[i] = sum_k K[3n+i, col_k] x[col_k] over the node's rows (one wave)
+template <class VT>
+__device__ __forceinline__ void node_rows(const SpArgs &s, long long node, const VT *__restrict__ vals,
+                                          const double *__restrict__ x, int lane, double acc[3])
+{
+    int a[3], lo[3], w[3];
+    node_box(s, node, a, lo, w);
+    const long long base = s.rowptr[3 * node];
+    const int L = 3 * w[0] * w[1] * w[2];
+    const float r2 = 1.0f / (float)w[2], r1 = 1.0f / (float)w[1];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int k = lane; k < L; k += 64) {
+        int t = (k * 43691) >> 17;  // k / 3 for k < 131072
+        int j = k - 3 * t;
+        int t2 = (int)(((float)t + 0.5f) * r2);  // t / w2 (exact: t < 128, w2 <= 5)
+        int o2 = t - t2 * w[2];
+        int o0 = (int)(((float)t2 + 0.5f) * r1);
+        int o1 = t2 - o0 * w[1];
+        long long cn = ((long long)(lo[0] + o0) * s.nb[1] + (lo[1] + o1)) * s.nb[2] + (lo[2] + o2);
+        double xv = x[3 * cn + j];
+        s0 += (double)vals[base + k] * xv;
+        s1 += (double)vals[base + L + k] * xv;
+        s2 += (double)vals[base + 2 * L + k] * xv;
+    }
+    acc[0] = wave_sum(s0);
+    acc[1] = wave_sum(s1);
+    acc[2] = wave_sum(s2);
+}
+
+// y = K x (RES = 0) or y = b - K x (RES = 1); with DOT, partial[block] =
+// sum over the block's rows of x . y (for CG's p.Kp)
+template <class VT, int RES, int DOT>
+__global__ __launch_bounds__(256) void k_spmv(SpArgs s, const VT *__restrict__ vals, const double *__restrict__ x,
+                                              const double *__restrict__ b, double *__restrict__ y,
+                                              double *__restrict__ partial)
+{
+    __shared__ double sd[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const long long node = (long long)blockIdx.x * 4 + wv;
+    double d = 0.0;
+    if (node < s.nnode) {
+        double acc[3];
+        node_rows<VT>(s, node, vals, x, lane, acc);
+        if (lane == 0) {
+            for (int i = 0; i < 3; i++) {
+                double v = RES ? b[3 * node + i] - acc[i] : acc[i];
+                y[3 * node + i] = v;
+                if (DOT) d += x[3 * node + i] * v;
+            }
+        }
+    }
+    if (DOT) {
+        if (lane == 0) sd[wv] = d;
+        __syncthreads();
+        if (threadIdx.x == 0) partial[blockIdx.x] = ((sd[0] + sd[1]) + sd[2]) + sd[3];
+    }
+}
+
+__device__ __forceinline__ void inv3(const double A[9], double B[9])
+{
+    double c00 = A[4] * A[8] - A[5] * A[7], c01 = A[5] * A[6] - A[3] * A[8], c02 = A[3] * A[7] - A[4] * A[6];
+    double det = A[0] * c00 + A[1] * c01 + A[2] * c02;
+    if (det == 0.0 || !isfinite(det)) {
+        for (int k = 0; k < 9; k++) B[k] = (k % 4 == 0) ? 1.0 : 0.0;
+        return;
+    }
+    double id = 1.0 / det;
+    B[0] = c00 * id;
+    B[1] = (A[2] * A[7] - A[1] * A[8]) * id;
+    B[2] = (A[1] * A[5] - A[2] * A[4]) * id;
+    B[3] = c01 * id;
+    B[4] = (A[0] * A[8] - A[2] * A[6]) * id;
+    B[5] = (A[2] * A[3] - A[0] * A[5]) * id;
+    B[6] = c02 * id;
+    B[7] = (A[1] * A[6] - A[0] * A[7]) * id;
+    B[8] = (A[0] * A[4] - A[1] * A[3]) * id;
+}
+
+// nodal 3x3 diagonal blocks of K and their inverses (9 doubles per node, row-major)
+template <class VT>
+__global__ void k_block_diag(SpArgs s, const VT *__restrict__ vals, double *__restrict__ Dblk, double *__restrict__ Dinv)
+{
+    long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= s.nnode) return;
+    int a[3], lo[3], w[3];
+    node_box(s, node, a, lo, w);
+    long long L = 3LL * w[0] * w[1] * w[2];
+    long long off = 3LL * (((a[0] - lo[0]) * w[1] + (a[1] - lo[1])) * w[2] + (a[2] - lo[2]));
+    long long base = s.rowptr[3 * node];
+    double A[9], B[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) A[3 * i + j] = (double)vals[base + i * L + off + j];
+    inv3(A, B);
+    for (int k = 0; k < 9; k++) {
+        Dblk[9 * node + k] = A[k];
+        Dinv[9 * node + k] = B[k];
+    }
+}
+
+// z = Dinv r per node
+__global__ void k_bjacobi(long long nnode, const double *__restrict__ Dinv, const double *__restrict__ r,
+                          double *__restrict__ z)
+{
+    long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nnode) return;
+    const double *B = Dinv + 9 * n;
+    double r0 = r[3 * n], r1 = r[3 * n + 1], r2 = r[3 * n + 2];
+    z[3 * n] = B[0] * r0 + B[1] * r1 + B[2] * r2;
+    z[3 * n + 1] = B[3] * r0 + B[4] * r1 + B[5] * r2;
+    z[3 * n + 2] = B[6] * r0 + B[7] * r1 + B[8] * r2;
+}
+
+// One colour of a block Gauss-Seidel sweep.  v holds the sweep's output so
+// far and is zero on nodes not yet visited, including this colour's, so the
+// full row product K v is the off-diagonal sum over visited colours.
+//   forward  (BWD = 0): v_n = Dinv_n (r_n - (K v)_n)
+//   backward (BWD = 1): v_n = Dinv_n (D_n y_n - (K v)_n)
+// Nodes of one colour (a_d = c_d mod 3) share no matrix entry.
+template <class VT, int BWD>
+__global__ __launch_bounds__(256) void k_sgs_colour(SpArgs s, int c0, int c1, int c2, int m1, int m2, long long ncol,
+                                                    const VT *__restrict__ vals, const double *__restrict__ Dblk,
+                                                    const double *__restrict__ Dinv, const double *__restrict__ rhs,
+                                                    double *v)
+{
+    const int lane = threadIdx.x & 63;
+    const long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= ncol) return;
+    long long a2 = c2 + 3 * (t % m2), a1 = c1 + 3 * ((t / m2) % m1), a0 = c0 + 3 * (t / ((long long)m1 * m2));
+    long long node = (a0 * s.nb[1] + a1) * s.nb[2] + a2;
+    double acc[3];
+    node_rows<VT>(s, node, vals, v, lane, acc);
+    if (lane == 0) {
+        double q[3];
+        const double *D = Dblk + 9 * node;
+        for (int i = 0; i < 3; i++) {
+            double rh = BWD ? D[3 * i] * rhs[3 * node] + D[3 * i + 1] * rhs[3 * node + 1] + D[3 * i + 2] * rhs[3 * node + 2]
+                            : rhs[3 * node + i];
+            q[i] = rh - acc[i];
+        }
+        const double *B = Dinv + 9 * node;
+        for (int i = 0; i < 3; i++) v[3 * node + i] = B[3 * i] * q[0] + B[3 * i + 1] * q[1] + B[3 * i + 2] * q[2];
+    }
+}
+
+__global__ void k_zero(double *x, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        x[i] = 0.0;
+}
+
+// out = x + a y   (a from the host)
+__global__ void k_waxpy(long long n, double a, const double *__restrict__ x, const double *__restrict__ y,
+                        double *__restrict__ out)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        out[i] = x[i] + a * y[i];
+}
+
+// x *= 1 / sc[0]  (device scalar)
+__global__ void k_scale_inv(long long n, const double *__restrict__ sc, double *__restrict__ x)
+{
+    double f = 1.0 / sc[0];
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        x[i] *= f;
+}
+
+// deterministic block sum of a per-thread value into partial[blockIdx.x]
+__device__ __forceinline__ void block_partial(double v, double *sred, double *partial_slot)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    v = wave_sum(v);
+    if (lane == 0) sred[wv] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) *partial_slot = ((sred[0] + sred[1]) + sred[2]) + sred[3];
+    __syncthreads();
+}
+
+// partial dots of up to two pairs: partial[p * NRB + b]
+__global__ __launch_bounds__(256) void k_dot2(long long n, const double *__restrict__ x0, const double *__restrict__ y0,
+                                              const double *__restrict__ x1, const double *__restrict__ y1,
+                                              double *__restrict__ partial)
+{
+    __shared__ double sred[4];
+    double a = 0.0, b = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        a += x0[i] * y0[i];
+        if (x1) b += x1[i] * y1[i];
+    }
+    block_partial(a, sred, &partial[blockIdx.x]);
+    if (x1) block_partial(b, sred, &partial[NRB + blockIdx.x]);
+}
+
+// out[j] = sum_b partial[j * nparts + b]; SQRT: out[j] = sqrt(...)
+template <int SQRT>
+__global__ __launch_bounds__(256) void k_finish(const double *__restrict__ partial, int nparts, double *__restrict__ out)
+{
+    __shared__ double sred[4];
+    const double *p = partial + (long long)blockIdx.x * nparts;
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) v += p[i];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    v = wave_sum(v);
+    if (lane == 0) sred[wv] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = ((sred[0] + sred[1]) + sred[2]) + sred[3];
+        out[blockIdx.x] = SQRT ? sqrt(s) : s;
+    }
+}
+
+// partial[j * NRB + b] = sum over block b's range of V_j . w, j < k
+// (V row-major k x ld).  Tiles of JT vectors share one read of w.
+__global__ __launch_bounds__(256) void k_mdot(const double *__restrict__ V, long long ld, int k, const double *__restrict__ w,
+                                              long long n, double *__restrict__ partial)
+{
+    __shared__ double sred[4][JT];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int j0 = 0; j0 < k; j0 += JT) {
+        double acc[JT];
+#pragma unroll
+        for (int t = 0; t < JT; t++) acc[t] = 0.0;
+        for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+            double wi = w[i];
+#pragma unroll
+            for (int t = 0; t < JT; t++)
+                if (j0 + t < k) acc[t] += V[(long long)(j0 + t) * ld + i] * wi;
+        }
+#pragma unroll
+        for (int t = 0; t < JT; t++) {
+            double v = wave_sum(acc[t]);
+            if (lane == 0) sred[wv][t] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < JT && j0 + (int)threadIdx.x < k) {
+            int t = threadIdx.x;
+            partial[(long long)(j0 + t) * NRB + blockIdx.x] = ((sred[0][t] + sred[1][t]) + sred[2][t]) + sred[3][t];
+        }
+        __syncthreads();
+    }
+}
+
+// w += alpha * sum_{j<k} V_j h[j]   (h in device memory)
+__global__ __launch_bounds__(256) void k_mgemv(const double *__restrict__ V, long long ld, int k, const double *__restrict__ h,
+                                               double alpha, double *__restrict__ w, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int j = 0; j < k; j++) s += V[(long long)j * ld + i] * h[j];
+        w[i] += alpha * s;
+    }
+}
+
+// CG update with device scalars: alpha = sc[irz] / sc[ipq];
+// x += alpha p, r -= alpha q; with BJ: z = Dinv r and partials of r.z, r.r
+// (launched with NRB blocks, like k_dot2, so that every partial is written)
+template <int BJ>
+__global__ __launch_bounds__(256) void k_cg_update(long long nnode, const double *__restrict__ sc, int irz, int ipq,
+                                                   const double *__restrict__ p, const double *__restrict__ q,
+                                                   double *__restrict__ x, double *__restrict__ r,
+                                                   const double *__restrict__ Dinv, double *__restrict__ z,
+                                                   double *__restrict__ partial)
+{
+    __shared__ double sred[4];
+    const double alpha = sc[irz] / sc[ipq];
+    double rz = 0.0, rr = 0.0;
+    for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < nnode; n += (long long)gridDim.x * blockDim.x) {
+        double rv[3];
+        for (int i = 0; i < 3; i++) {
+            long long g = 3 * n + i;
+            x[g] += alpha * p[g];
+            rv[i] = r[g] - alpha * q[g];
+            r[g] = rv[i];
+            rr += rv[i] * rv[i];
+        }
+        if (BJ) {
+            const double *B = Dinv + 9 * n;
+            for (int i = 0; i < 3; i++) {
+                double zi = B[3 * i] * rv[0] + B[3 * i + 1] * rv[1] + B[3 * i + 2] * rv[2];
+                z[3 * n + i] = zi;
+                rz += rv[i] * zi;
+            }
+        }
+    }
+    block_partial(rz, sred, &partial[blockIdx.x]);
+    block_partial(rr, sred, &partial[NRB + blockIdx.x]);
+}
+
+// p = z + beta p, beta = sc[inew] / sc[iold]
+__global__ void k_cg_dir(long long n, const double *__restrict__ sc, int inew, int iold, const double *__restrict__ z,
+                         double *__restrict__ p)
+{
+    const double beta = sc[inew] / sc[iold];
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = z[i] + beta * p[i];
+}
+
+// U = g on the Dirichlet rows (mask)
+__global__ void k_lift(long long n, const unsigned char *__restrict__ mask, const double *__restrict__ g, double *__restrict__ U)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        if (mask[i]) U[i] = g[i];
+}
+
+inline unsigned grid_for(long long n, int per_block) { return (unsigned)((n + per_block - 1) / per_block); }
+inline unsigned vgrid(long long n) { return (unsigned)std::min<long long>(NRB, (n + 255) / 256); }
+
+
+// ====================================================================== multigrid
+// Stencil storage of a level matrix ("SM"): for node n, row i, stencil offset
+// o = ((d0+2)*5 + (d1+2))*5 + (d2+2), column dof j: sm[n*1125 + i*375 + o*3 + j]
+// = K[(n,i),(n+d,j)], zero where n+d is outside the grid.  The AIJ pattern
+// of every level is this stencil cut at the boundary, so the Galerkin
+// operator P^T K P of a level keeps it.
+constexpr int MGW = 4;  // nonzeros per row/column of a 1D transfer
+
+struct Xfer {                 // one direction: fine nf <-> coarse nc
+    const int *c2f_lo;        // [nc]: first fine index of coarse A's support
+    const double *c2f_w;      // [nc * MGW]: P[c2f_lo + k][A]
+    const int *f2c_lo;        // [nf]
+    const double *f2c_w;      // [nf * MGW]: P[a][f2c_lo + k]
+};
+struct Xfer3 {
+    Xfer d[3];
+    int nbf[3], nbc[3];
+};
+
+__device__ __forceinline__ void decode_node(const int nb[3], long long node, int a[3])
+{
+    a[2] = (int)(node % nb[2]);
+    a[1] = (int)((node / nb[2]) % nb[1]);
+    a[0] = (int)(node / ((long long)nb[2] * nb[1]));
+}
+
+// acc[i] = sum over the stencil of K[(n,i),(n+d,j)] x[n+d, j] (one wave)
+__device__ __forceinline__ void node_rows_sm(const SpArgs &s, long long node, const double *__restrict__ sm,
+                                             const double *__restrict__ x, int lane, double acc[3])
+{
+    int a[3];
+    decode_node(s.nb, node, a);
+    const double *v = sm + node * 1125;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int k = lane; k < 375; k += 64) {
+        int o = (k * 43691) >> 17;     // k / 3
+        int j = k - 3 * o;
+        int q0 = (o * 1311) >> 15;     // o / 25 (o < 125)
+        int r0 = o - 25 * q0;
+        int q1 = (r0 * 13108) >> 16;   // r0 / 5
+        int b0 = a[0] + q0 - 2, b1 = a[1] + q1 - 2, b2 = a[2] + (r0 - 5 * q1) - 2;
+        if (b0 < 0 || b0 >= s.nb[0] || b1 < 0 || b1 >= s.nb[1] || b2 < 0 || b2 >= s.nb[2]) continue;
+        double xv = x[3 * (((long long)b0 * s.nb[1] + b1) * s.nb[2] + b2) + j];
+        s0 += v[k] * xv;
+        s1 += v[375 + k] * xv;
+        s2 += v[750 + k] * xv;
+    }
+    acc[0] = wave_sum(s0);
+    acc[1] = wave_sum(s1);
+    acc[2] = wave_sum(s2);
+}
+
+template <int SM, class VT>
+__device__ __forceinline__ void level_rows(const SpArgs &s, long long node, const VT *__restrict__ vals,
+                                           const double *__restrict__ x, int lane, double acc[3])
+{
+    if (SM) node_rows_sm(s, node, (const double *)vals, x, lane, acc);
+    else node_rows<VT>(s, node, vals, x, lane, acc);
+}
+
+// y = b - K x on a level in stencil storage
+__global__ __launch_bounds__(256) void k_resid_sm(SpArgs s, const double *__restrict__ sm, const double *__restrict__ x,
+                                                  const double *__restrict__ b, double *__restrict__ y)
+{
+    const int lane = threadIdx.x & 63;
+    const long long node = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (node >= s.nnode) return;
+    double acc[3];
+    node_rows_sm(s, node, sm, x, lane, acc);
+    if (lane == 0)
+        for (int i = 0; i < 3; i++) y[3 * node + i] = b[3 * node + i] - acc[i];
+}
+
+// one colour of block Gauss-Seidel: x_n += Dinv_n (b_n - (K x)_n)
+template <int SM, class VT>
+__global__ __launch_bounds__(256) void k_gs_colour(SpArgs s, int c0, int c1, int c2, int m1, int m2, long long ncol,
+                                                   const VT *__restrict__ vals, const double *__restrict__ Dinv,
+                                                   const double *__restrict__ b, double *x)
+{
+    const int lane = threadIdx.x & 63;
+    const long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= ncol) return;
+    long long a2 = c2 + 3 * (t % m2), a1 = c1 + 3 * ((t / m2) % m1), a0 = c0 + 3 * (t / ((long long)m1 * m2));
+    long long node = (a0 * s.nb[1] + a1) * s.nb[2] + a2;
+    double acc[3];
+    level_rows<SM, VT>(s, node, vals, x, lane, acc);
+    if (lane == 0) {
+        double q[3];
+        for (int i = 0; i < 3; i++) q[i] = b[3 * node + i] - acc[i];
+        const double *B = Dinv + 9 * node;
+        for (int i = 0; i < 3; i++) x[3 * node + i] += B[3 * i] * q[0] + B[3 * i + 1] * q[1] + B[3 * i + 2] * q[2];
+    }
+}
+
+// diagonal 3x3 blocks of a level in stencil storage, and their inverses
+__global__ void k_block_diag_sm(long long nnode, const double *__restrict__ sm, double *__restrict__ Dblk, double *__restrict__ Dinv)
+{
+    long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= nnode) return;
+    double A[9], B[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) A[3 * i + j] = sm[node * 1125 + i * 375 + 62 * 3 + j];
+    inv3(A, B);
+    for (int k = 0; k < 9; k++) {
+        Dblk[9 * node + k] = A[k];
+        Dinv[9 * node + k] = B[k];
+    }
+}
+
+// One direction d of the Galerkin product: out = (R_d (x) I) in (R_d (x) I)^T
+// with R_d = P_d^T, the other directions unchanged.  in: stencil storage on
+// grid nbi (IN_CSR: the assembled AIJ values of the fine level); out:
+// stencil storage on nbo (nbo[d] coarse).  One thread per (node, offset).
+template <int IN_CSR, class VT>
+__global__ __launch_bounds__(256) void k_galerkin(int d, int3 nbi, int3 nbo, const long long *__restrict__ rowptr,
+                                                  const VT *__restrict__ in, Xfer x, double *__restrict__ out)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nbi_[3] = {nbi.x, nbi.y, nbi.z}, nbo_[3] = {nbo.x, nbo.y, nbo.z};
+    const long long nout = (long long)nbo.x * nbo.y * nbo.z;
+    if (t >= nout * 125) return;
+    const long long N = t / 125;
+    const int o = (int)(t - N * 125);
+    int A[3], D[3] = {o / 25 - 2, (o / 5) % 5 - 2, o % 5 - 2};
+    decode_node(nbo_, N, A);
+    double acc[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) acc[k] = 0.0;
+    bool ok = true;
+    for (int k = 0; k < 3; k++) ok = ok && A[k] + D[k] >= 0 && A[k] + D[k] < nbo_[k];
+    if (ok) {
+        const int Ad = A[d], Bd = A[d] + D[d];
+        const int alo = x.c2f_lo[Ad], blo = x.c2f_lo[Bd];
+        for (int ka = 0; ka < MGW; ka++) {
+            const double wa = x.c2f_w[Ad * MGW + ka];
+            if (wa == 0.0) continue;
+            for (int kb = 0; kb < MGW; kb++) {
+                const double wb = x.c2f_w[Bd * MGW + kb];
+                const int off = (blo + kb) - (alo + ka);
+                if (wb == 0.0 || off < -2 || off > 2) continue;
+                int Ai[3] = {A[0], A[1], A[2]}, Di[3] = {D[0], D[1], D[2]};
+                Ai[d] = alo + ka;
+                Di[d] = off;
+                const long long nin = ((long long)Ai[0] * nbi_[1] + Ai[1]) * nbi_[2] + Ai[2];
+                const double w = wa * wb;
+                if (IN_CSR) {
+                    int lo[3], wd[3];
+                    for (int k = 0; k < 3; k++) {
+                        lo[k] = max(0, Ai[k] - 2);
+                        wd[k] = min(nbi_[k] - 1, Ai[k] + 2) - lo[k] + 1;
+                    }
+                    const long long L = 3LL * wd[0] * wd[1] * wd[2];
+                    const long long pos = rowptr[3 * nin] +
+                        3LL * (((Ai[0] + Di[0] - lo[0]) * wd[1] + (Ai[1] + Di[1] - lo[1])) * wd[2] + (Ai[2] + Di[2] - lo[2]));
+#pragma unroll
+                    for (int i = 0; i < 3; i++)
+#pragma unroll
+                        for (int j = 0; j < 3; j++) acc[3 * i + j] += w * (double)in[pos + i * L + j];
+                } else {
+                    const int oi = ((Di[0] + 2) * 5 + (Di[1] + 2)) * 5 + (Di[2] + 2);
+                    const VT *v = in + nin * 1125 + oi * 3;
+#pragma unroll
+                    for (int i = 0; i < 3; i++)
+#pragma unroll
+                        for (int j = 0; j < 3; j++) acc[3 * i + j] += w * (double)v[i * 375 + j];
+                }
+            }
+        }
+    }
+    double *po = out + N * 1125 + o * 3;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) po[i * 375 + j] = acc[3 * i + j];
+}
+
+// bc = P^T rf (restriction), one thread per coarse node
+__global__ void k_restrict(Xfer3 x, const double *__restrict__ rf, double *__restrict__ bc)
+{
+    const long long nc = (long long)x.nbc[0] * x.nbc[1] * x.nbc[2];
+    const long long C = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (C >= nc) return;
+    int A[3];
+    decode_node(x.nbc, C, A);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    const int l0 = x.d[0].c2f_lo[A[0]], l1 = x.d[1].c2f_lo[A[1]], l2 = x.d[2].c2f_lo[A[2]];
+    for (int k0 = 0; k0 < MGW; k0++) {
+        double w0 = x.d[0].c2f_w[A[0] * MGW + k0];
+        if (w0 == 0.0) continue;
+        for (int k1 = 0; k1 < MGW; k1++) {
+            double w1 = w0 * x.d[1].c2f_w[A[1] * MGW + k1];
+            if (w1 == 0.0) continue;
+            for (int k2 = 0; k2 < MGW; k2++) {
+                double w = w1 * x.d[2].c2f_w[A[2] * MGW + k2];
+                if (w == 0.0) continue;
+                long long f = (((long long)(l0 + k0)) * x.nbf[1] + (l1 + k1)) * x.nbf[2] + (l2 + k2);
+                s0 += w * rf[3 * f];
+                s1 += w * rf[3 * f + 1];
+                s2 += w * rf[3 * f + 2];
+            }
+        }
+    }
+    bc[3 * C] = s0;
+    bc[3 * C + 1] = s1;
+    bc[3 * C + 2] = s2;
+}
+
+// xf += P xc (prolongation), one thread per fine node
+__global__ void k_prolong_add(Xfer3 x, const double *__restrict__ xc, double *__restrict__ xf)
+{
+    const long long nf = (long long)x.nbf[0] * x.nbf[1] * x.nbf[2];
+    const long long F = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (F >= nf) return;
+    int a[3];
+    decode_node(x.nbf, F, a);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    const int l0 = x.d[0].f2c_lo[a[0]], l1 = x.d[1].f2c_lo[a[1]], l2 = x.d[2].f2c_lo[a[2]];
+    for (int k0 = 0; k0 < MGW; k0++) {
+        double w0 = x.d[0].f2c_w[a[0] * MGW + k0];
+        if (w0 == 0.0) continue;
+        for (int k1 = 0; k1 < MGW; k1++) {
+            double w1 = w0 * x.d[1].f2c_w[a[1] * MGW + k1];
+            if (w1 == 0.0) continue;
+            for (int k2 = 0; k2 < MGW; k2++) {
+                double w = w1 * x.d[2].f2c_w[a[2] * MGW + k2];
+                if (w == 0.0) continue;
+                long long C = (((long long)(l0 + k0)) * x.nbc[1] + (l1 + k1)) * x.nbc[2] + (l2 + k2);
+                s0 += w * xc[3 * C];
+                s1 += w * xc[3 * C + 1];
+                s2 += w * xc[3 * C + 2];
+            }
+        }
+    }
+    xf[3 * F] += s0;
+    xf[3 * F + 1] += s1;
+    xf[3 * F + 2] += s2;
+}
+
+// x = Ainv b (dense n x n, one block)
+__global__ __launch_bounds__(256) void k_dense_mv(int n, const double *__restrict__ Ainv, const double *__restrict__ b,
+                                                  double *__restrict__ x)
+{
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+        double s = 0.0;
+        for (int c = 0; c < n; c++) s += Ainv[(long long)r * n + c] * b[c];
+        x[r] = s;
+    }
+}
+
+}  // namespace
+
+// ====================================================================== work space
+namespace {
+struct MGLevel {
+    int ne[3], nb[3];
+    long long nnode = 0, ndof = 0;
+    double *sm = nullptr;                 // stencil storage (levels >= 1)
+    double *Dblk = nullptr, *Dinv = nullptr;
+    double *x = nullptr, *b = nullptr, *r = nullptr;
+    double *stage[2] = {nullptr, nullptr};  // Galerkin intermediates towards the next level
+    int *xfer_i = nullptr;                // transfer tables to the next level (device)
+    double *xfer_w = nullptr;
+    Xfer3 T;                              // views into xfer_i / xfer_w
+    double *Ainv = nullptr;               // coarsest level: dense inverse
+    bool dense = false;
+};
+}  // namespace
+
+struct igap4h::Solver {
+    std::vector<MGLevel> mg;  // mg[0] is the fine level (its matrix is the assembled AIJ values)
+    long long ndof = 0, nnode = 0;
+    int restart = 0;
+    size_t vals_bytes = 0;
+    void *vals = nullptr;  // tangent values used by the Newton solve
+    double *Dblk = nullptr, *Dinv = nullptr;
+    double *V = nullptr;   // (restart + 1) x ndof
+    double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *w = nullptr, *t = nullptr;
+    double *Y = nullptr, *U0 = nullptr, *R = nullptr;
+    double *partial = nullptr;  // (restart + 2) x NRB
+    double *pspmv = nullptr;    // one partial per SpMV block
+    double *hd = nullptr;       // GMRES: two Gram-Schmidt columns and y, 3 (restart + 1)
+    double *hh = nullptr;       // pinned host mirror of hd
+    double *sc = nullptr;       // device scalars
+    double *hp = nullptr;       // pinned host mirror of sc
+};
+
+void igap4h::solver_free(igap4h_ctx *c)
+{
+    Solver *s = c ? c->solver : nullptr;
+    if (!s) return;
+    (void)hipFree(s->vals);
+    for (double *p : {s->Dblk, s->Dinv, s->V, s->r, s->z, s->p, s->q, s->w, s->t, s->Y, s->U0, s->R, s->partial,
+                      s->pspmv, s->hd, s->sc})
+        (void)hipFree(p);
+    if (s->hp) (void)hipHostFree(s->hp);
+    if (s->hh) (void)hipHostFree(s->hh);
+    for (MGLevel &L : s->mg) {
+        for (double *p : {L.sm, L.Dblk, L.Dinv, L.x, L.b, L.r, L.stage[0], L.stage[1], L.xfer_w, L.Ainv})
+            (void)hipFree(p);
+        (void)hipFree(L.xfer_i);
+    }
+    delete s;
+    c->solver = nullptr;
+}
+
+static int solver_ws(igap4h_ctx *c, int restart, size_t vals_bytes)
+{
+    if (!c->solver) {
+        c->solver = new Solver();
+        Solver *s = c->solver;
+        s->ndof = c->mesh.ndof();
+        s->nnode = s->ndof / 3;
+        size_t n = (size_t)s->ndof;
+        int rc;
+        if ((rc = dev_alloc(&s->Dblk, 9 * (size_t)s->nnode)) || (rc = dev_alloc(&s->Dinv, 9 * (size_t)s->nnode)) ||
+            (rc = dev_alloc(&s->r, n)) || (rc = dev_alloc(&s->z, n)) || (rc = dev_alloc(&s->p, n)) ||
+            (rc = dev_alloc(&s->q, n)) || (rc = dev_alloc(&s->w, n)) || (rc = dev_alloc(&s->t, n)) ||
+            (rc = dev_alloc(&s->Y, n)) || (rc = dev_alloc(&s->U0, n)) || (rc = dev_alloc(&s->R, n)) ||
+            (rc = dev_alloc(&s->sc, 64)) || (rc = dev_alloc(&s->pspmv, (size_t)(s->nnode + 3) / 4)))
+            return rc;
+        if (hipHostMalloc((void **)&s->hp, 64 * sizeof(double)) != hipSuccess) {
+            s->hp = nullptr;
+            return fail(IGAP4H_ENOMEM, "hipHostMalloc of 512 bytes failed");
+        }
+    }
+    Solver *s = c->solver;
+    if (restart > s->restart) {
+        (void)hipFree(s->V);
+        (void)hipFree(s->partial);
+        (void)hipFree(s->hd);
+        if (s->hh) (void)hipHostFree(s->hh);
+        s->V = s->partial = s->hd = s->hh = nullptr;
+        s->restart = 0;
+        int rc;
+        if ((rc = dev_alloc(&s->V, (size_t)(restart + 1) * s->ndof))) return rc;
+        if ((rc = dev_alloc(&s->partial, (size_t)(restart + 2) * NRB))) return rc;
+        if ((rc = dev_alloc(&s->hd, 3 * (size_t)(restart + 1)))) return rc;
+        if (hipHostMalloc((void **)&s->hh, 3 * (size_t)(restart + 1) * sizeof(double)) != hipSuccess) {
+            s->hh = nullptr;
+            return fail(IGAP4H_ENOMEM, "hipHostMalloc failed");
+        }
+        s->restart = restart;
+    }
+    if (vals_bytes > s->vals_bytes) {
+        (void)hipFree(s->vals);
+        s->vals = nullptr;
+        s->vals_bytes = 0;
+        if (hipMalloc(&s->vals, vals_bytes) != hipSuccess) return fail(IGAP4H_ENOMEM, "hipMalloc of %zu bytes failed", vals_bytes);
+        s->vals_bytes = vals_bytes;
+    }
+    return 0;
+}
+
+static SpArgs sp_args(const igap4h_ctx *c)
+{
+    SpArgs a;
+    for (int d = 0; d < 3; d++) a.nb[d] = c->mesh.nb[d];
+    a.nnode = c->mesh.ndof() / 3;
+    a.rowptr = c->d_rowptr;
+    return a;
+}
+
+// ====================================================================== multigrid set-up
+// Levels by dyadic coarsening of the knot vectors while every direction has
+// an even element count and the level has more than MG_DENSE dofs; the
+// coarsest level is solved with a dense inverse when it has at most MG_DENSE
+// dofs, else by symmetric Gauss-Seidel sweeps.
+constexpr long long MG_DENSE = 400;
+
+static int mg_build(igap4h_ctx *c, Solver *s)
+{
+    if (!s->mg.empty()) return 0;
+    std::vector<double> kf[3];
+    MGLevel L0;
+    for (int d = 0; d < 3; d++) {
+        L0.ne[d] = c->mesh.ne[d];
+        L0.nb[d] = c->mesh.nb[d];
+        kf[d] = c->mesh.knot[d];
+    }
+    L0.nnode = (long long)L0.nb[0] * L0.nb[1] * L0.nb[2];
+    L0.ndof = 3 * L0.nnode;
+    s->mg.push_back(L0);
+    int rc;
+    if ((rc = dev_alloc(&s->mg[0].r, (size_t)L0.ndof))) return rc;
+    while (s->mg.size() < 12) {
+        MGLevel &F = s->mg.back();
+        bool can = F.ndof > MG_DENSE;
+        for (int d = 0; d < 3; d++) can = can && F.ne[d] % 2 == 0 && F.ne[d] >= 2;
+        if (!can) break;
+        MGLevel C;
+        std::vector<double> kc[3], P[3];
+        for (int d = 0; d < 3; d++) {
+            C.ne[d] = F.ne[d] / 2;
+            C.nb[d] = C.ne[d] + 2;
+            kc[d] = coarsen_knots(kf[d], F.ne[d]);
+            if (prolongation_1d(kf[d], F.ne[d], kc[d], C.ne[d], P[d]))
+                return fail(IGAP4H_EINVAL, "multigrid: knot vectors of direction %d are not nested", d);
+        }
+        C.nnode = (long long)C.nb[0] * C.nb[1] * C.nb[2];
+        C.ndof = 3 * C.nnode;
+        // transfer tables: [c2f_lo d0 d1 d2 | f2c_lo d0 d1 d2], weights alike
+        size_t ni = 0;
+        for (int d = 0; d < 3; d++) ni += (size_t)C.nb[d] + F.nb[d];
+        std::vector<int> hi(ni, 0);
+        std::vector<double> hw(ni * MGW, 0.0);
+        size_t oi = 0;
+        size_t off_c[3], off_f[3];
+        for (int d = 0; d < 3; d++) {
+            const int nbf = F.nb[d], nbc = C.nb[d];
+            off_c[d] = oi;
+            for (int A = 0; A < nbc; A++, oi++) {
+                int lo = -1, hi_ = -1;
+                for (int a = 0; a < nbf; a++)
+                    if (P[d][(size_t)a * nbc + A] != 0.0) { if (lo < 0) lo = a; hi_ = a; }
+                if (lo < 0 || hi_ - lo >= MGW) return fail(IGAP4H_EINVAL, "multigrid: transfer wider than %d", MGW);
+                hi[oi] = lo;
+                for (int k = 0; lo + k <= hi_; k++) hw[oi * MGW + k] = P[d][(size_t)(lo + k) * nbc + A];
+            }
+        }
+        for (int d = 0; d < 3; d++) {
+            const int nbf = F.nb[d], nbc = C.nb[d];
+            off_f[d] = oi;
+            for (int a = 0; a < nbf; a++, oi++) {
+                int lo = -1, hi_ = -1;
+                for (int A = 0; A < nbc; A++)
+                    if (P[d][(size_t)a * nbc + A] != 0.0) { if (lo < 0) lo = A; hi_ = A; }
+                if (lo < 0 || hi_ - lo >= MGW) return fail(IGAP4H_EINVAL, "multigrid: transfer wider than %d", MGW);
+                hi[oi] = lo;
+                for (int k = 0; lo + k <= hi_; k++) hw[oi * MGW + k] = P[d][(size_t)a * nbc + lo + k];
+            }
+        }
+        if ((rc = dev_alloc(&F.xfer_i, ni)) || (rc = dev_alloc(&F.xfer_w, ni * MGW))) return rc;
+        HIPCHK(hipMemcpy(F.xfer_i, hi.data(), ni * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(F.xfer_w, hw.data(), ni * MGW * sizeof(double), hipMemcpyHostToDevice));
+        for (int d = 0; d < 3; d++) {
+            F.T.d[d].c2f_lo = F.xfer_i + off_c[d];
+            F.T.d[d].c2f_w = F.xfer_w + off_c[d] * MGW;
+            F.T.d[d].f2c_lo = F.xfer_i + off_f[d];
+            F.T.d[d].f2c_w = F.xfer_w + off_f[d] * MGW;
+            F.T.nbf[d] = F.nb[d];
+            F.T.nbc[d] = C.nb[d];
+        }
+        const size_t st0 = (size_t)C.nb[0] * F.nb[1] * F.nb[2] * 1125, st1 = (size_t)C.nb[0] * C.nb[1] * F.nb[2] * 1125;
+        if ((rc = dev_alloc(&F.stage[0], st0)) || (rc = dev_alloc(&F.stage[1], st1))) return rc;
+        if ((rc = dev_alloc(&C.sm, (size_t)C.nnode * 1125)) || (rc = dev_alloc(&C.Dblk, 9 * (size_t)C.nnode)) ||
+            (rc = dev_alloc(&C.Dinv, 9 * (size_t)C.nnode)) || (rc = dev_alloc(&C.x, (size_t)C.ndof)) ||
+            (rc = dev_alloc(&C.b, (size_t)C.ndof)) || (rc = dev_alloc(&C.r, (size_t)C.ndof)))
+            return rc;
+        for (int d = 0; d < 3; d++) kf[d] = kc[d];
+        s->mg.push_back(C);
+    }
+    MGLevel &B = s->mg.back();
+    if (s->mg.size() > 1 && B.ndof <= MG_DENSE) {
+        B.dense = true;
+        if ((rc = dev_alloc(&B.Ainv, (size_t)B.ndof * B.ndof))) return rc;
+    }
+    return 0;
+}
+
+static SpArgs level_args(const MGLevel &L)
+{
+    SpArgs a;
+    for (int d = 0; d < 3; d++) a.nb[d] = L.nb[d];
+    a.nnode = L.nnode;
+    a.rowptr = nullptr;
+    return a;
+}
+
+// Gauss-Jordan inverse with partial pivoting (row-major n x n, in place into inv)
+static bool dense_inverse(std::vector<double> &A, int n, std::vector<double> &inv)
+{
+    inv.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) inv[(size_t)i * n + i] = 1.0;
+    for (int k = 0; k < n; k++) {
+        int piv = k;
+        for (int r = k + 1; r < n; r++)
+            if (std::fabs(A[(size_t)r * n + k]) > std::fabs(A[(size_t)piv * n + k])) piv = r;
+        double pv = A[(size_t)piv * n + k];
+        if (!(std::fabs(pv) > 0.0) || !std::isfinite(pv)) return false;
+        if (piv != k)
+            for (int c2 = 0; c2 < n; c2++) {
+                std::swap(A[(size_t)k * n + c2], A[(size_t)piv * n + c2]);
+                std::swap(inv[(size_t)k * n + c2], inv[(size_t)piv * n + c2]);
+            }
+        double ip = 1.0 / A[(size_t)k * n + k];
+        for (int c2 = 0; c2 < n; c2++) {
+            A[(size_t)k * n + c2] *= ip;
+            inv[(size_t)k * n + c2] *= ip;
+        }
+        for (int r = 0; r < n; r++) {
+            if (r == k) continue;
+            double f = A[(size_t)r * n + k];
+            if (f == 0.0) continue;
+            for (int c2 = 0; c2 < n; c2++) {
+                A[(size_t)r * n + c2] -= f * A[(size_t)k * n + c2];
+                inv[(size_t)r * n + c2] -= f * inv[(size_t)k * n + c2];
+            }
+        }
+    }
+    return true;
+}
+
+// ====================================================================== operators
+namespace {
+
+struct Op {
+    igap4h_ctx *c;
+    Solver *s;
+    SpArgs sp;
+    const void *vals;
+    int f32;
+    int pc;
+    hipStream_t st;
+
+    int spmv(const double *x, double *y, const double *b = nullptr) const
+    {
+        unsigned g = grid_for(sp.nnode, 4);
+        if (f32) {
+            if (b) hipLaunchKernelGGL((k_spmv<float, 1, 0>), dim3(g), dim3(256), 0, st, sp, (const float *)vals, x, b, y, nullptr);
+            else hipLaunchKernelGGL((k_spmv<float, 0, 0>), dim3(g), dim3(256), 0, st, sp, (const float *)vals, x, b, y, nullptr);
+        } else {
+            if (b) hipLaunchKernelGGL((k_spmv<double, 1, 0>), dim3(g), dim3(256), 0, st, sp, (const double *)vals, x, b, y, nullptr);
+            else hipLaunchKernelGGL((k_spmv<double, 0, 0>), dim3(g), dim3(256), 0, st, sp, (const double *)vals, x, b, y, nullptr);
+        }
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    // y = K x and partial[block] of x.y; returns the number of partials
+    int spmv_dot(const double *x, double *y, double *partial, int *nparts) const
+    {
+        unsigned g = grid_for(sp.nnode, 4);
+        if (f32) hipLaunchKernelGGL((k_spmv<float, 0, 1>), dim3(g), dim3(256), 0, st, sp, (const float *)vals, x, nullptr, y, partial);
+        else hipLaunchKernelGGL((k_spmv<double, 0, 1>), dim3(g), dim3(256), 0, st, sp, (const double *)vals, x, nullptr, y, partial);
+        HIPCHK(hipGetLastError());
+        *nparts = (int)g;
+        return 0;
+    }
+    int gs_sweep(size_t l, bool bwd, const double *b, double *x) const
+    {
+        const MGLevel &L = s->mg[l];
+        SpArgs a = l ? level_args(L) : sp;
+        const double *Dinv = l ? L.Dinv : s->Dinv;
+        for (int cc = 0; cc < 27; cc++) {
+            int col = bwd ? 26 - cc : cc;
+            int c0 = col / 9, c1 = (col / 3) % 3, c2 = col % 3;
+            int m0 = (a.nb[0] - c0 + 2) / 3, m1 = (a.nb[1] - c1 + 2) / 3, m2 = (a.nb[2] - c2 + 2) / 3;
+            long long ncol = (long long)m0 * m1 * m2;
+            if (ncol <= 0) continue;
+            dim3 g(grid_for(ncol, 4));
+            if (l) hipLaunchKernelGGL((k_gs_colour<1, double>), g, dim3(256), 0, st, a, c0, c1, c2, m1, m2, ncol, (const double *)L.sm, Dinv, b, x);
+            else if (f32) hipLaunchKernelGGL((k_gs_colour<0, float>), g, dim3(256), 0, st, a, c0, c1, c2, m1, m2, ncol, (const float *)vals, Dinv, b, x);
+            else hipLaunchKernelGGL((k_gs_colour<0, double>), g, dim3(256), 0, st, a, c0, c1, c2, m1, m2, ncol, (const double *)vals, Dinv, b, x);
+            HIPCHK(hipGetLastError());
+        }
+        return 0;
+    }
+    int mg_setup() const
+    {
+        int rc;
+        for (size_t l = 0; l + 1 < s->mg.size(); l++) {
+            const MGLevel &F = s->mg[l];
+            const MGLevel &C = s->mg[l + 1];
+            int3 g0 = make_int3(F.nb[0], F.nb[1], F.nb[2]);
+            int3 g1 = make_int3(C.nb[0], F.nb[1], F.nb[2]);
+            int3 g2 = make_int3(C.nb[0], C.nb[1], F.nb[2]);
+            int3 g3 = make_int3(C.nb[0], C.nb[1], C.nb[2]);
+            auto nthr = [](int3 g) { return grid_for((long long)g.x * g.y * g.z * 125, 256); };
+            if (l == 0) {
+                if (f32) hipLaunchKernelGGL((k_galerkin<1, float>), dim3(nthr(g1)), dim3(256), 0, st, 0, g0, g1, sp.rowptr, (const float *)vals, F.T.d[0], F.stage[0]);
+                else hipLaunchKernelGGL((k_galerkin<1, double>), dim3(nthr(g1)), dim3(256), 0, st, 0, g0, g1, sp.rowptr, (const double *)vals, F.T.d[0], F.stage[0]);
+            } else {
+                hipLaunchKernelGGL((k_galerkin<0, double>), dim3(nthr(g1)), dim3(256), 0, st, 0, g0, g1, (const long long *)nullptr, (const double *)F.sm, F.T.d[0], F.stage[0]);
+            }
+            HIPCHK(hipGetLastError());
+            hipLaunchKernelGGL((k_galerkin<0, double>), dim3(nthr(g2)), dim3(256), 0, st, 1, g1, g2, (const long long *)nullptr, (const double *)F.stage[0], F.T.d[1], F.stage[1]);
+            hipLaunchKernelGGL((k_galerkin<0, double>), dim3(nthr(g3)), dim3(256), 0, st, 2, g2, g3, (const long long *)nullptr, (const double *)F.stage[1], F.T.d[2], C.sm);
+            HIPCHK(hipGetLastError());
+            hipLaunchKernelGGL(k_block_diag_sm, dim3(grid_for(C.nnode, 256)), dim3(256), 0, st, C.nnode, (const double *)C.sm, C.Dblk, C.Dinv);
+            HIPCHK(hipGetLastError());
+        }
+        MGLevel &B = s->mg.back();
+        if (B.dense) {
+            std::vector<double> h((size_t)B.nnode * 1125), A((size_t)B.ndof * B.ndof, 0.0), inv;
+            HIPCHK(hipMemcpyAsync(h.data(), B.sm, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (long long n = 0; n < B.nnode; n++) {
+                int a[3] = {(int)(n / ((long long)B.nb[2] * B.nb[1])), (int)((n / B.nb[2]) % B.nb[1]), (int)(n % B.nb[2])};
+                for (int o = 0; o < 125; o++) {
+                    int b0 = a[0] + o / 25 - 2, b1 = a[1] + (o / 5) % 5 - 2, b2 = a[2] + o % 5 - 2;
+                    if (b0 < 0 || b0 >= B.nb[0] || b1 < 0 || b1 >= B.nb[1] || b2 < 0 || b2 >= B.nb[2]) continue;
+                    long long m = ((long long)b0 * B.nb[1] + b1) * B.nb[2] + b2;
+                    for (int i = 0; i < 3; i++)
+                        for (int j = 0; j < 3; j++)
+                            A[(size_t)(3 * n + i) * B.ndof + 3 * m + j] = h[(size_t)n * 1125 + i * 375 + o * 3 + j];
+                }
+            }
+            if (!dense_inverse(A, (int)B.ndof, inv)) return fail(IGAP4H_EINVAL, "multigrid: coarsest operator is singular");
+            HIPCHK(hipMemcpyAsync(B.Ainv, inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice, st));
+            HIPCHK(hipStreamSynchronize(st));  // inv is freed on return
+        }
+        (void)rc;
+        return 0;
+    }
+    // x = V-cycle(b) on level l (x overwritten)
+    int vcycle(size_t l, const double *b, double *x) const
+    {
+        const MGLevel &L = s->mg[l];
+        const long long n = L.ndof;
+        int rc;
+        if (l + 1 == s->mg.size()) {
+            if (L.dense) {
+                hipLaunchKernelGGL(k_dense_mv, dim3(1), dim3(256), 0, st, (int)n, (const double *)L.Ainv, b, x);
+                HIPCHK(hipGetLastError());
+                return 0;
+            }
+            hipLaunchKernelGGL(k_zero, dim3(vgrid(n)), dim3(256), 0, st, x, n);
+            HIPCHK(hipGetLastError());
+            for (int it = 0; it < 4; it++)
+                if ((rc = gs_sweep(l, false, b, x)) || (rc = gs_sweep(l, true, b, x))) return rc;
+            return 0;
+        }
+        const MGLevel &C = s->mg[l + 1];
+        hipLaunchKernelGGL(k_zero, dim3(vgrid(n)), dim3(256), 0, st, x, n);
+        HIPCHK(hipGetLastError());
+        if ((rc = gs_sweep(l, false, b, x))) return rc;
+        if (l) {
+            hipLaunchKernelGGL(k_resid_sm, dim3(grid_for(L.nnode, 4)), dim3(256), 0, st, level_args(L), (const double *)L.sm, (const double *)x, b, L.r);
+            HIPCHK(hipGetLastError());
+        } else if ((rc = spmv(x, L.r, b))) {
+            return rc;
+        }
+        hipLaunchKernelGGL(k_restrict, dim3(grid_for(C.nnode, 256)), dim3(256), 0, st, L.T, (const double *)L.r, C.b);
+        HIPCHK(hipGetLastError());
+        if ((rc = vcycle(l + 1, C.b, C.x))) return rc;
+        hipLaunchKernelGGL(k_prolong_add, dim3(grid_for(L.nnode, 256)), dim3(256), 0, st, L.T, (const double *)C.x, x);
+        HIPCHK(hipGetLastError());
+        return gs_sweep(l, true, b, x);
+    }
+    int setup_pc() const
+    {
+        if (pc == IGAP4H_PC_NONE) return 0;
+        unsigned g = grid_for(sp.nnode, 256);
+        if (f32) hipLaunchKernelGGL((k_block_diag<float>), dim3(g), dim3(256), 0, st, sp, (const float *)vals, s->Dblk, s->Dinv);
+        else hipLaunchKernelGGL((k_block_diag<double>), dim3(g), dim3(256), 0, st, sp, (const double *)vals, s->Dblk, s->Dinv);
+        HIPCHK(hipGetLastError());
+        if (pc == IGAP4H_PC_MG) {
+            int rc = mg_build(c, s);
+            if (rc) return rc;
+            return mg_setup();
+        }
+        return 0;
+    }
+    template <class VT, int BWD>
+    int sgs_pass(const double *rhs, double *v) const
+    {
+        for (int cc = 0; cc < 27; cc++) {
+            int col = BWD ? 26 - cc : cc;
+            int c0 = col / 9, c1 = (col / 3) % 3, c2 = col % 3;
+            int m0 = (sp.nb[0] - c0 + 2) / 3, m1 = (sp.nb[1] - c1 + 2) / 3, m2 = (sp.nb[2] - c2 + 2) / 3;
+            long long ncol = (long long)m0 * m1 * m2;
+            if (ncol <= 0) continue;
+            hipLaunchKernelGGL((k_sgs_colour<VT, BWD>), dim3(grid_for(ncol, 4)), dim3(256), 0, st, sp, c0, c1, c2, m1, m2, ncol,
+                               (const VT *)vals, (const double *)s->Dblk, (const double *)s->Dinv, rhs, v);
+            HIPCHK(hipGetLastError());
+        }
+        return 0;
+    }
+    // z = M^{-1} r (z != r)
+    int apply_pc(const double *r, double *z) const
+    {
+        long long n = 3 * sp.nnode;
+        if (pc == IGAP4H_PC_NONE) {
+            HIPCHK(hipMemcpyAsync(z, r, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+            return 0;
+        }
+        if (pc == IGAP4H_PC_MG) return vcycle(0, r, z);
+        if (pc == IGAP4H_PC_BJACOBI3) {
+            hipLaunchKernelGGL(k_bjacobi, dim3(grid_for(sp.nnode, 256)), dim3(256), 0, st, sp.nnode, (const double *)s->Dinv, r, z);
+            HIPCHK(hipGetLastError());
+            return 0;
+        }
+        // symmetric Gauss-Seidel: forward into s->t, backward into z
+        double *y = s->t;
+        hipLaunchKernelGGL(k_zero, dim3(vgrid(n)), dim3(256), 0, st, y, n);
+        hipLaunchKernelGGL(k_zero, dim3(vgrid(n)), dim3(256), 0, st, z, n);
+        HIPCHK(hipGetLastError());
+        int rc = f32 ? sgs_pass<float, 0>(r, y) : sgs_pass<double, 0>(r, y);
+        if (rc) return rc;
+        return f32 ? sgs_pass<float, 1>(y, z) : sgs_pass<double, 1>(y, z);
+    }
+    // out[0..np) = sums of np consecutive partial rows of nparts each
+    int finish(const double *partial, int np, int nparts, double *out, bool sq = false) const
+    {
+        if (sq) hipLaunchKernelGGL(k_finish<1>, dim3(np), dim3(256), 0, st, partial, nparts, out);
+        else hipLaunchKernelGGL(k_finish<0>, dim3(np), dim3(256), 0, st, partial, nparts, out);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    int norm(const double *x, double *out_dev) const
+    {
+        long long n = 3 * sp.nnode;
+        hipLaunchKernelGGL(k_dot2, dim3(NRB), dim3(256), 0, st, n, x, x, (const double *)nullptr, (const double *)nullptr, s->partial);
+        HIPCHK(hipGetLastError());
+        return finish(s->partial, 1, NRB, out_dev, true);
+    }
+    int read_scalars(int i0, int cnt) const
+    {
+        HIPCHK(hipMemcpyAsync(s->hp + i0, s->sc + i0, cnt * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return 0;
+    }
+};
+
+// right-preconditioned GMRES(m), classical Gram-Schmidt with one
+// re-orthogonalisation; stop at ||b - K x|| <= rtol ||b|| (the least-squares
+// estimate inside a cycle, the true residual at each restart)
+int gmres(const Op &A, const double *b, double *x, int m, double rtol, int maxit, int *its_out, double *relres)
+{
+    Solver *s = A.s;
+    const long long n = s->ndof;
+    hipStream_t st = A.st;
+    double *sc = s->sc, *hp = s->hp;
+    double *h1 = s->hd, *h2 = s->hd + (m + 1), *ydev = s->hd + 2 * (m + 1);  // device
+    double *hh = s->hh;                                                       // pinned host
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+    int rc;
+    if ((rc = A.norm(b, sc + 0))) return rc;
+    if ((rc = A.read_scalars(0, 1))) return rc;
+    const double bnorm = hp[0];
+    *its_out = 0;
+    *relres = 0.0;
+    if (bnorm == 0.0) {
+        hipLaunchKernelGGL(k_zero, dim3(vgrid(n)), dim3(256), 0, st, x, n);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    const double tol = rtol * bnorm;
+    int its = 0;
+    double res = 0.0;
+    while (true) {
+        double *V0 = s->V;
+        if ((rc = A.spmv(x, V0, b))) return rc;  // r0 = b - K x
+        if ((rc = A.norm(V0, sc + 1))) return rc;
+        if ((rc = A.read_scalars(1, 1))) return rc;
+        const double beta = hp[1];
+        res = beta;
+        if (beta <= tol || its >= maxit) break;
+        hipLaunchKernelGGL(k_scale_inv, dim3(vgrid(n)), dim3(256), 0, st, n, (const double *)(sc + 1), V0);
+        HIPCHK(hipGetLastError());
+        std::fill(H.begin(), H.end(), 0.0);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int j = 0;
+        for (; j < m;) {
+            double *Vj = s->V + (long long)j * n, *W = s->V + (long long)(j + 1) * n;
+            if ((rc = A.apply_pc(Vj, s->z))) return rc;
+            if ((rc = A.spmv(s->z, W))) return rc;
+            for (int pass = 0; pass < 2; pass++) {  // h = V^T w; w -= V h
+                double *hc = pass ? h2 : h1;
+                hipLaunchKernelGGL(k_mdot, dim3(NRB), dim3(256), 0, st, (const double *)s->V, n, j + 1, (const double *)W, n, s->partial);
+                HIPCHK(hipGetLastError());
+                if ((rc = A.finish(s->partial, j + 1, NRB, hc))) return rc;
+                hipLaunchKernelGGL(k_mgemv, dim3(vgrid(n)), dim3(256), 0, st, (const double *)s->V, n, j + 1, (const double *)hc, -1.0, W, n);
+                HIPCHK(hipGetLastError());
+            }
+            if ((rc = A.norm(W, sc + 2))) return rc;
+            HIPCHK(hipMemcpyAsync(hh, s->hd, 2 * (m + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+            if ((rc = A.read_scalars(2, 1))) return rc;  // also waits for hh
+            const double hn = hp[2];
+            double *h = &H[(size_t)j * (m + 1)];
+            for (int i = 0; i <= j; i++) h[i] = hh[i] + hh[m + 1 + i];
+            h[j + 1] = hn;
+            if (hn > 0.0) {
+                hipLaunchKernelGGL(k_scale_inv, dim3(vgrid(n)), dim3(256), 0, st, n, (const double *)(sc + 2), W);
+                HIPCHK(hipGetLastError());
+            }
+            for (int i = 0; i < j; i++) {  // previous Givens rotations
+                double t0 = cs[i] * h[i] + sn[i] * h[i + 1];
+                h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+                h[i] = t0;
+            }
+            double den = std::hypot(h[j], h[j + 1]);
+            cs[j] = den == 0.0 ? 1.0 : h[j] / den;
+            sn[j] = den == 0.0 ? 0.0 : h[j + 1] / den;
+            h[j] = den;
+            h[j + 1] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            its++;
+            j++;
+            res = std::fabs(g[j]);
+            if (res <= tol || its >= maxit || hn == 0.0) break;
+        }
+        // y = R^{-1} g; x += M^{-1} V y
+        for (int i = j - 1; i >= 0; i--) {
+            double v = g[i];
+            for (int k = i + 1; k < j; k++) v -= H[(size_t)k * (m + 1) + i] * y[k];
+            double d = H[(size_t)i * (m + 1) + i];
+            y[i] = d == 0.0 ? 0.0 : v / d;
+        }
+        for (int i = 0; i < j; i++) hh[2 * (m + 1) + i] = y[i];
+        HIPCHK(hipMemcpyAsync(ydev, hh + 2 * (m + 1), j * sizeof(double), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_zero, dim3(vgrid(n)), dim3(256), 0, st, s->w, n);
+        hipLaunchKernelGGL(k_mgemv, dim3(vgrid(n)), dim3(256), 0, st, (const double *)s->V, n, j, (const double *)ydev, 1.0, s->w, n);
+        HIPCHK(hipGetLastError());
+        if ((rc = A.apply_pc(s->w, s->z))) return rc;
+        hipLaunchKernelGGL(k_waxpy, dim3(vgrid(n)), dim3(256), 0, st, n, 1.0, (const double *)x, (const double *)s->z, x);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));  // hh is reused by the next cycle
+        if (res <= tol || its >= maxit) break;
+    }
+    *its_out = its;
+    *relres = res / bnorm;
+    return 0;
+}
+
+// preconditioned conjugate gradients; device scalars sc[0..1] (r.z, two
+// slots alternating), sc[2] (p.Kp), sc[3] (r.r)
+int pcg(const Op &A, const double *b, double *x, double rtol, int maxit, int *its_out, double *relres)
+{
+    Solver *s = A.s;
+    const long long n = s->ndof, nn = s->nnode;
+    hipStream_t st = A.st;
+    double *sc = s->sc, *hp = s->hp;
+    int rc;
+    if ((rc = A.norm(b, sc + 4))) return rc;
+    if ((rc = A.read_scalars(4, 1))) return rc;
+    const double bnorm = hp[4];
+    *its_out = 0;
+    *relres = 0.0;
+    if (bnorm == 0.0) {
+        hipLaunchKernelGGL(k_zero, dim3(vgrid(n)), dim3(256), 0, st, x, n);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    const double tol = rtol * bnorm;
+    if ((rc = A.spmv(x, s->r, b))) return rc;  // r = b - K x
+    if ((rc = A.apply_pc(s->r, s->z))) return rc;
+    hipLaunchKernelGGL(k_dot2, dim3(NRB), dim3(256), 0, st, n, (const double *)s->r, (const double *)s->z,
+                       (const double *)s->r, (const double *)s->r, s->partial);
+    HIPCHK(hipGetLastError());
+    if ((rc = A.finish(s->partial, 1, NRB, sc + 0))) return rc;
+    if ((rc = A.finish(s->partial + NRB, 1, NRB, sc + 3))) return rc;
+    HIPCHK(hipMemcpyAsync(s->p, s->z, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if ((rc = A.read_scalars(3, 1))) return rc;
+    double res = std::sqrt(hp[3]);
+    int its = 0, cur = 0;
+    const bool bj = A.pc == IGAP4H_PC_BJACOBI3;
+    while (res > tol && its < maxit) {
+        int np;
+        if ((rc = A.spmv_dot(s->p, s->q, s->pspmv, &np))) return rc;
+        // p.Kp from the spmv's per-block partials (np of them)
+        if ((rc = A.finish(s->pspmv, 1, np, sc + 2))) return rc;
+        if (bj) {
+            hipLaunchKernelGGL(k_cg_update<1>, dim3(NRB), dim3(256), 0, st, nn, (const double *)sc, cur, 2,
+                               (const double *)s->p, (const double *)s->q, x, s->r, (const double *)s->Dinv, s->z, s->partial);
+            HIPCHK(hipGetLastError());
+        } else {
+            hipLaunchKernelGGL(k_cg_update<0>, dim3(NRB), dim3(256), 0, st, nn, (const double *)sc, cur, 2,
+                               (const double *)s->p, (const double *)s->q, x, s->r, (const double *)s->Dinv, s->z, s->partial);
+            HIPCHK(hipGetLastError());
+            if ((rc = A.apply_pc(s->r, s->z))) return rc;
+            hipLaunchKernelGGL(k_dot2, dim3(NRB), dim3(256), 0, st, n, (const double *)s->r, (const double *)s->z,
+                               (const double *)nullptr, (const double *)nullptr, s->partial);
+            HIPCHK(hipGetLastError());
+        }
+        const int nxt = cur ^ 1;
+        if ((rc = A.finish(s->partial, 1, NRB, sc + nxt))) return rc;
+        if ((rc = A.finish(s->partial + NRB, 1, NRB, sc + 3))) return rc;
+        hipLaunchKernelGGL(k_cg_dir, dim3(vgrid(n)), dim3(256), 0, st, n, (const double *)sc, nxt, cur, (const double *)s->z, s->p);
+        HIPCHK(hipGetLastError());
+        its++;
+        cur = nxt;
+        if ((rc = A.read_scalars(2, 2))) return rc;
+        res = std::sqrt(hp[3]);
+        if (!(hp[2] > 0.0)) {  // K (or M) not positive definite along p
+            *its_out = its;
+            *relres = res / bnorm;
+            return fail(IGAP4H_EINVAL, "CG breakdown: p.Kp = %g at iteration %d (matrix not SPD; use GMRES)", hp[2], its);
+        }
+    }
+    *its_out = its;
+    *relres = res / bnorm;
+    return 0;
+}
+
+int check_opts(const igap4h_newton_opts *o)
+{
+    if (!o) return fail(IGAP4H_EINVAL, "NULL options");
+    if (o->ksp_type != IGAP4H_KSP_GMRES && o->ksp_type != IGAP4H_KSP_CG) return fail(IGAP4H_EINVAL, "bad ksp_type %d", o->ksp_type);
+    if (o->pc < IGAP4H_PC_NONE || o->pc > IGAP4H_PC_MG) return fail(IGAP4H_EINVAL, "bad pc %d", o->pc);
+    if (o->ksp_type == IGAP4H_KSP_GMRES && (o->restart < 1 || o->restart > 4096))
+        return fail(IGAP4H_EINVAL, "GMRES restart %d outside 1..4096", o->restart);
+    if (o->ksp_max_it < 1) return fail(IGAP4H_EINVAL, "ksp_max_it must be >= 1");
+    if (o->vals_type != IGAP4H_VALS_F64 && o->vals_type != IGAP4H_VALS_F32) return fail(IGAP4H_EINVAL, "bad vals_type %d", o->vals_type);
+    if (o->tangent_prec != IGAP4H_FP64 && o->tangent_prec != IGAP4H_FP32) return fail(IGAP4H_EINVAL, "bad tangent_prec %d", o->tangent_prec);
+    return 0;
+}
+
+int linear_solve(const Op &A, const double *b, double *x, const igap4h_newton_opts *o, int *its, double *relres)
+{
+    if (o->ksp_type == IGAP4H_KSP_CG) return pcg(A, b, x, o->ksp_rtol, o->ksp_max_it, its, relres);
+    return gmres(A, b, x, o->restart, o->ksp_rtol, o->ksp_max_it, its, relres);
+}
+
+using clk = std::chrono::steady_clock;
+inline double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+}  // namespace
+
+// ====================================================================== C-ABI
+extern "C" int igap4h_newton_default_opts(igap4h_newton_opts *o)
+{
+    if (!o) return fail(IGAP4H_EINVAL, "NULL options");
+    std::memset(o, 0, sizeof(*o));
+    o->atol = 1e-10;  // snes_init.c:18
+    o->rtol = 0.0;
+    o->max_newton = 50;
+    o->tangent_prec = IGAP4H_FP64;
+    o->vals_type = IGAP4H_VALS_F64;
+    o->ksp_type = IGAP4H_KSP_GMRES;  // snes_init.c:32
+    o->restart = 200;
+    o->ksp_rtol = 1e-5;               // PETSc default (snes_init.c:26 keeps it)
+    o->ksp_max_it = 20000;
+    o->ksp_guess = 1;                 // snes_init.c:25
+    o->pc = IGAP4H_PC_BJACOBI3;
+    o->line_search = 1;               // snes_init.c:21-22
+    o->stagnation = 1;                // snes_solve.c:30
+    return 0;
+}
+
+extern "C" int igap4h_spmv_dev(igap4h_ctx *c, const void *d_vals, int vals_type, const double *d_x, double *d_y, void *stream)
+{
+    if (!c || !d_vals || !d_x || !d_y) return fail(IGAP4H_EINVAL, "NULL argument");
+    if (vals_type != IGAP4H_VALS_F64 && vals_type != IGAP4H_VALS_F32) return fail(IGAP4H_EINVAL, "bad vals type %d", vals_type);
+    Op A{c, nullptr, sp_args(c), d_vals, vals_type == IGAP4H_VALS_F32, IGAP4H_PC_NONE, (hipStream_t)stream};
+    return A.spmv(d_x, d_y);
+}
+
+extern "C" int igap4h_linear_solve_dev(igap4h_ctx *c, const void *d_vals, int vals_type, const double *d_b, double *d_x,
+                                       const igap4h_newton_opts *o, int *its, double *relres, void *stream)
+{
+    if (!c || !d_vals || !d_b || !d_x || !its || !relres) return fail(IGAP4H_EINVAL, "NULL argument");
+    int rc = check_opts(o);
+    if (rc) return rc;
+    if (vals_type != IGAP4H_VALS_F64 && vals_type != IGAP4H_VALS_F32) return fail(IGAP4H_EINVAL, "bad vals type %d", vals_type);
+    HIPCHK(hipSetDevice(c->device));
+    if ((rc = solver_ws(c, o->ksp_type == IGAP4H_KSP_GMRES ? o->restart : 1, 0))) return rc;
+    Op A{c, c->solver, sp_args(c), d_vals, vals_type == IGAP4H_VALS_F32, o->pc, (hipStream_t)stream};
+    if ((rc = A.setup_pc())) return rc;
+    return linear_solve(A, d_b, d_x, o, its, relres);
+}
+
+extern "C" int igap4h_newton_solve_dev(igap4h_ctx *c, double *d_Uh, const igap4h_newton_opts *o, igap4h_newton_info *info,
+                                       void *stream)
+{
+    if (!c || !d_Uh || !info) return fail(IGAP4H_EINVAL, "NULL argument");
+    int rc = check_opts(o);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(c->device));
+    std::memset(info, 0, sizeof(*info));
+    const int f32 = o->vals_type == IGAP4H_VALS_F32;
+    if ((rc = solver_ws(c, o->ksp_type == IGAP4H_KSP_GMRES ? o->restart : 1, (size_t)c->nnz * (f32 ? 4 : 8)))) return rc;
+    Solver *s = c->solver;
+    hipStream_t st = (hipStream_t)stream;
+    const long long n = s->ndof;
+    Op A{c, s, sp_args(c), s->vals, f32, o->pc, st};
+    double *U = d_Uh;  // slice 0: current iterate
+
+    auto t_all = clk::now();
+    if (o->dirichlet_lift && !c->drows.empty()) {
+        hipLaunchKernelGGL(k_lift, dim3(vgrid(n)), dim3(256), 0, st, n, (const unsigned char *)c->d_mask, (const double *)c->d_g, U);
+        HIPCHK(hipGetLastError());
+    }
+    auto residual = [&](double *fn) -> int {
+        auto t0 = clk::now();
+        int r = igap4h_residual_dev(c, d_Uh, s->R, st);
+        if (r) return r;
+        if ((r = A.norm(s->R, s->sc + 5))) return r;
+        if ((r = A.read_scalars(5, 1))) return r;
+        *fn = s->hp[5];
+        info->t_residual += secs(t0, clk::now());
+        return 0;
+    };
+    double fnorm;
+    if ((rc = residual(&fnorm))) return rc;
+    info->fnorm0 = fnorm;
+    info->fnorm_hist[0] = fnorm;
+    hipLaunchKernelGGL(k_zero, dim3(vgrid(n)), dim3(256), 0, st, s->Y, n);
+    HIPCHK(hipGetLastError());
+    double fold = 999.0;  // snes_solve.c:23
+    info->converged = 0;
+    for (int it = 0; it < o->max_newton; it++) {
+        if (!std::isfinite(fnorm)) {
+            info->converged = -3;
+            break;
+        }
+        if ((o->atol > 0.0 && fnorm < o->atol) || (o->rtol > 0.0 && fnorm < o->rtol * info->fnorm0)) {
+            info->converged = 1;
+            break;
+        }
+        auto t0 = clk::now();
+        if ((rc = igap4h_tangent_dev(c, d_Uh, o->tangent_prec, o->vals_type, s->vals, st))) return rc;
+        HIPCHK(hipStreamSynchronize(st));
+        auto t1 = clk::now();
+        info->t_tangent += secs(t0, t1);
+        if (!o->ksp_guess) {
+            hipLaunchKernelGGL(k_zero, dim3(vgrid(n)), dim3(256), 0, st, s->Y, n);
+            HIPCHK(hipGetLastError());
+        }
+        if ((rc = A.setup_pc())) return rc;
+        int kits = 0;
+        double relres = 0.0;
+        rc = linear_solve(A, s->R, s->Y, o, &kits, &relres);
+        HIPCHK(hipStreamSynchronize(st));
+        info->t_solve += secs(t1, clk::now());
+        info->ksp_its += kits;
+        if (it < 64) info->ksp_its_hist[it] = kits;
+        if (rc) {
+            info->converged = -2;
+            info->newton_its = it;
+            info->fnorm = fnorm;
+            info->t_total = secs(t_all, clk::now());
+            return rc;
+        }
+        // U_new = U - lambda Y with backtracking on f = |F|^2 / 2
+        HIPCHK(hipMemcpyAsync(s->U0, U, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        auto trial = [&](double lam, double *gn) -> int {
+            hipLaunchKernelGGL(k_waxpy, dim3(vgrid(n)), dim3(256), 0, st, n, -lam, (const double *)s->U0, (const double *)s->Y, U);
+            HIPCHK(hipGetLastError());
+            return residual(gn);
+        };
+        double gnorm;
+        if (!o->line_search) {
+            if ((rc = trial(1.0, &gnorm))) return rc;
+        } else {
+            // slope of f along -Y: -(F, K Y)   (Dennis & Schnabel A6.3.1)
+            if ((rc = A.spmv(s->Y, s->w))) return rc;
+            hipLaunchKernelGGL(k_dot2, dim3(NRB), dim3(256), 0, st, n, (const double *)s->R, (const double *)s->w,
+                               (const double *)nullptr, (const double *)nullptr, s->partial);
+            HIPCHK(hipGetLastError());
+            if ((rc = A.finish(s->partial, 1, NRB, s->sc + 6))) return rc;
+            if ((rc = A.read_scalars(6, 1))) return rc;
+            double slope = s->hp[6];
+            if (slope > 0.0) slope = -slope;
+            if (slope == 0.0) slope = -1.0;
+            const double alpha = 1e-4, f0 = 0.5 * fnorm * fnorm;
+            double lam = 1.0;
+            if ((rc = trial(lam, &gnorm))) return rc;
+            bool ok = std::isfinite(gnorm) && 0.5 * gnorm * gnorm <= f0 + lam * alpha * slope;
+            double lprev = lam, gprev = gnorm;
+            if (!ok) {  // quadratic model
+                double lt = -slope / (gnorm * gnorm - fnorm * fnorm - 2.0 * lam * slope);
+                if (!std::isfinite(gnorm) || !std::isfinite(lt)) lt = 0.1 * lam;
+                lprev = lam;
+                gprev = gnorm;
+                lam = lt > 0.5 * lam ? 0.5 * lam : (lt <= 0.1 * lam ? 0.1 * lam : lt);
+                if ((rc = trial(lam, &gnorm))) return rc;
+                ok = std::isfinite(gnorm) && 0.5 * gnorm * gnorm < f0 + lam * alpha * slope;
+            }
+            for (int cnt = 0; !ok && cnt < 40; cnt++) {  // cubic model
+                if (lam < 1e-12) break;
+                double t1c = 0.5 * (gnorm * gnorm - fnorm * fnorm) - lam * slope;
+                double t2c = 0.5 * (gprev * gprev - fnorm * fnorm) - lprev * slope;
+                double a = (t1c / (lam * lam) - t2c / (lprev * lprev)) / (lam - lprev);
+                double bb = (-lprev * t1c / (lam * lam) + lam * t2c / (lprev * lprev)) / (lam - lprev);
+                double d = std::max(bb * bb - 3.0 * a * slope, 0.0);
+                double lt = a == 0.0 ? -slope / (2.0 * bb) : (-bb + std::sqrt(d)) / (3.0 * a);
+                if (!std::isfinite(lt) || !std::isfinite(gnorm)) lt = 0.1 * lam;
+                lprev = lam;
+                gprev = gnorm;
+                lam = lt > 0.5 * lam ? 0.5 * lam : (lt <= 0.1 * lam ? 0.1 * lam : lt);
+                if ((rc = trial(lam, &gnorm))) return rc;
+                ok = std::isfinite(gnorm) && 0.5 * gnorm * gnorm < f0 + lam * alpha * slope;
+            }
+            if (!ok) {  // keep the last accepted iterate
+                HIPCHK(hipMemcpyAsync(U, s->U0, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+                double f;
+                if ((rc = residual(&f))) return rc;
+                info->converged = -3;
+                info->newton_its = it + 1;
+                fnorm = f;
+                break;
+            }
+        }
+        fnorm = gnorm;
+        info->newton_its = it + 1;
+        if (it + 1 < 64) info->fnorm_hist[it + 1] = fnorm;
+        if (o->stagnation && std::fabs(fold - fnorm) / fold < 1e-6 &&
+            !((o->atol > 0.0 && fnorm < o->atol) || (o->rtol > 0.0 && fnorm < o->rtol * info->fnorm0))) {
+            info->converged = -1;
+            break;
+        }
+        fold = fnorm;
+        if ((o->atol > 0.0 && fnorm < o->atol) || (o->rtol > 0.0 && fnorm < o->rtol * info->fnorm0)) {
+            info->converged = 1;
+            break;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    info->fnorm = fnorm;
+    info->t_total = secs(t_all, clk::now());
+    return 0;
+}

@@ -181,6 +181,21 @@ def stretch_problem(n, a=0.01, lam=1.0, mu=0.5, ell=0.1, noise=0.0, seed=0x5EED0
     return pb
 
 
+def affine_guess(pb, nhist=None):
+    """Control values of the affine field u_i = sum_J a_iJ X_J with
+    a_iJ = par_dirichlet[3 i + J] (the field src/iga_dirichlet.c:42-44
+    imposes on the Dirichlet faces) at every control point: a smooth initial
+    guess that meets the Dirichlet rows of faces whose data it is (quadratic
+    B-splines reproduce affine fields with control values at the Greville
+    abscissae).  History slices (getime) are copies."""
+    g = pb.greville()
+    A = np.asarray(pb.par_dirichlet, dtype=np.float64).reshape(3, 3)
+    X = np.stack(np.meshgrid(*g, indexing="ij"), axis=-1)       # (nb0, nb1, nb2, 3)
+    U = np.einsum("abcJ,iJ->abci", X, A).reshape(-1)
+    nh = pb.nhist if nhist is None else nhist
+    return np.tile(U, nh)
+
+
 def smooth_state(pb, amp=1e-3, noise=1e-6, seed=0x5EED0012, nhist=None):
     """Non-equilibrium displacement at the control points:
     U_i = amp sin(pi X) sin(pi Y) sin(pi Z) (1, .5, .25)_i + noise U(-1,1),

@@ -166,6 +166,42 @@ class App:
         L.check(L.lib().igap4h_tangent_dev(self.ctx, ctypes.c_void_p(d_Uh), prec, vals_type,
                                            ctypes.c_void_p(d_vals), ctypes.c_void_p(stream or 0)))
 
+    # ---- Newton-Krylov driver (device arrays)
+    @staticmethod
+    def newton_opts(**kw):
+        """igap4h_newton_opts with the library defaults (reference SNES/KSP
+        settings, src/snes_init.c:18-39) overridden by keyword."""
+        o = L.NewtonOpts()
+        L.check(L.lib().igap4h_newton_default_opts(ctypes.byref(o)))
+        for k, v in kw.items():
+            if not hasattr(o, k):
+                raise TypeError("unknown Newton option %r" % k)
+            setattr(o, k, v)
+        return o
+
+    def newton_solve_dev(self, d_Uh, opts=None, stream=None, **kw):
+        """Solve R(U) = 0 from slice 0 of the device array d_Uh (updated in
+        place).  Returns the igap4h_newton_info fields as a dict."""
+        o = opts if opts is not None else self.newton_opts(**kw)
+        info = L.NewtonInfo()
+        L.check(L.lib().igap4h_newton_solve_dev(self.ctx, ctypes.c_void_p(d_Uh), ctypes.byref(o),
+                                                ctypes.byref(info), ctypes.c_void_p(stream or 0)))
+        return info.as_dict()
+
+    def spmv_dev(self, d_vals, vals_type, d_x, d_y, stream=None):
+        L.check(L.lib().igap4h_spmv_dev(self.ctx, ctypes.c_void_p(d_vals), vals_type, ctypes.c_void_p(d_x),
+                                        ctypes.c_void_p(d_y), ctypes.c_void_p(stream or 0)))
+
+    def linear_solve_dev(self, d_vals, vals_type, d_b, d_x, opts=None, stream=None, **kw):
+        """K x = b on the device; returns (iterations, relative residual)."""
+        o = opts if opts is not None else self.newton_opts(**kw)
+        its = ctypes.c_int()
+        rel = ctypes.c_double()
+        L.check(L.lib().igap4h_linear_solve_dev(self.ctx, ctypes.c_void_p(d_vals), vals_type, ctypes.c_void_p(d_b),
+                                                ctypes.c_void_p(d_x), ctypes.byref(o), ctypes.byref(its),
+                                                ctypes.byref(rel), ctypes.c_void_p(stream or 0)))
+        return its.value, rel.value
+
     def energy_dev(self, d_U, stream=None):
         e = ctypes.c_double()
         L.check(L.lib().igap4h_energy_dev(self.ctx, ctypes.c_void_p(d_U), ctypes.byref(e),

@@ -201,6 +201,73 @@ int igap4h_set_coef(igap4h_ctx *ctx, const double *elem_coef);
  * volume part, phys/ge density hh[0]); ge only.  d_U: device ndof.        */
 int igap4h_energy_dev(igap4h_ctx *ctx, const double *d_U, double *energy, void *stream);
 
+/* ---- Newton-Krylov driver ------------------------------------------- */
+/* The reference leaves the solve to PETSc (src/snes_init.c:8-47,
+ * src/snes_solve.c:10-36: NEWTONLS + BT line search, GMRES with right
+ * preconditioning, one Newton step per SNESSolve in an outer loop stopped at
+ * ||F|| < 1e-10 or when |dF|/F < 1e-6).  This driver runs the same loop on
+ * the GPU: residual and tangent from this library, GMRES(restart) with
+ * classical Gram-Schmidt twice (in place of modified Gram-Schmidt), right-
+ * preconditioned by geometric multigrid, nodal 3x3 block Jacobi or multi-
+ * colour block symmetric Gauss-Seidel (GPU replacements of the ASM/ILU(0) of
+ * snes_init.c:38-39), or preconditioned CG.  Every reduction is a fixed-order two-pass sum, so a
+ * solve is bitwise reproducible on one device.                            */
+enum { IGAP4H_PC_NONE = 0, IGAP4H_PC_BJACOBI3 = 1, IGAP4H_PC_SGS = 2, IGAP4H_PC_MG = 3 };
+enum { IGAP4H_KSP_GMRES = 0, IGAP4H_KSP_CG = 1 };
+
+typedef struct {
+    double atol;       /* stop when ||R||_2 < atol (reference 1e-10); <= 0 off  */
+    double rtol;       /* stop when ||R||_2 / ||R0||_2 < rtol; <= 0 off         */
+    int max_newton;    /* Newton iterations                                     */
+    int tangent_prec;  /* IGAP4H_FP64 / IGAP4H_FP32 contraction                */
+    int vals_type;     /* matrix values used by the Krylov solve (F64/F32)     */
+    int ksp_type;      /* IGAP4H_KSP_GMRES (reference, snes_init.c:32) or _CG (the
+                          alternative left commented at snes_init.c:40-43)      */
+    int restart;       /* GMRES restart length                                  */
+    double ksp_rtol;   /* linear stop: ||b - K x|| < ksp_rtol ||b|| (PETSc 1e-5) */
+    int ksp_max_it;    /* linear iterations per Newton step                    */
+    int ksp_guess;     /* 1: start from the previous Newton direction
+                          (KSPSetInitialGuessNonzero, snes_init.c:25); 0: zero  */
+    int pc;            /* IGAP4H_PC_BJACOBI3: 3x3 nodal block Jacobi;
+                          IGAP4H_PC_SGS: 27-colour nodal block symmetric
+                          Gauss-Seidel; IGAP4H_PC_MG: one V-cycle of geometric
+                          multigrid (knot-insertion prolongation, Galerkin
+                          coarse operators P^T K P, block Gauss-Seidel
+                          smoothing, dense coarsest solve); IGAP4H_PC_NONE     */
+    int line_search;   /* 0 full step, 1 backtracking (quadratic then cubic)   */
+    int stagnation;    /* 1: stop (as diverged) when |dF|/F < 1e-6 (snes_solve.c:30) */
+    int dirichlet_lift;/* 1: set U = g on the Dirichlet rows before the first step, so
+                          every Newton increment vanishes there and the zeroed columns
+                          of the tangent (snesiga_Tangent.c:270) lose nothing; 0: start
+                          from U as given (reference)                                */
+} igap4h_newton_opts;
+
+typedef struct {
+    int converged;     /* 1 converged, 0 iteration limit, -1 stagnated, -2 linear solve failed,
+                          -3 line search failed */
+    int newton_its;
+    int ksp_its;       /* summed over Newton steps */
+    double fnorm0, fnorm;
+    double t_residual, t_tangent, t_solve, t_total; /* seconds (line-search residuals
+                                                        count in t_residual) */
+    double fnorm_hist[64];  /* ||R|| before each Newton step and after the last */
+    int ksp_its_hist[64];   /* Krylov iterations of each Newton step            */
+} igap4h_newton_info;
+
+int igap4h_newton_default_opts(igap4h_newton_opts *opts);
+/* d_U_hist: device (torder+1) x ndof; slice 0 is the initial guess on entry
+ * and the solution on return (history slices are read only).              */
+int igap4h_newton_solve_dev(igap4h_ctx *ctx, double *d_U_hist, const igap4h_newton_opts *opts,
+                            igap4h_newton_info *info, void *stream);
+/* y = K x on the AIJ pattern; d_vals f64 or f32 per vals_type.            */
+int igap4h_spmv_dev(igap4h_ctx *ctx, const void *d_vals, int vals_type, const double *d_x, double *d_y,
+                    void *stream);
+/* K x = b with the options' ksp_type/restart/ksp_rtol/ksp_max_it/pc; x holds
+ * the initial guess on entry.  relres = ||b - K x|| / ||b|| as the method
+ * tracks it (GMRES: the least-squares estimate).                           */
+int igap4h_linear_solve_dev(igap4h_ctx *ctx, const void *d_vals, int vals_type, const double *d_b, double *d_x,
+                            const igap4h_newton_opts *opts, int *its, double *relres, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
