@@ -1,12 +1,19 @@
-"""Assembly benchmark on one MI355X: residual + tangent of phys/ge at 64^3
-quadratic C1 B-spline elements (nq = 4, 16.8 M Gauss points), clamped at
-x = 0 and stretched u_x = 0.01 X at x = 1, per-element Lame fields with
-seeded +-10 % noise (lambda = 1, mu = 0.5, l = 0.1), non-equilibrium state
-U = 1e-3 sin(pi X) sin(pi Y) sin(pi Z) (1, .5, .25) + 1e-6 noise.
+"""Newton-step benchmark on one MI355X: phys/ge at 64^3 quadratic C1 B-spline
+elements (nq = 4, 16.8 M Gauss points), clamped at x = 0 and stretched
+u_x = 0.01 X at x = 1, per-element Lame fields with seeded +-10 % noise
+(lambda = 1, mu = 0.5, l = 0.1).
 
-A step is one Newton-step assembly: R (FP64) and K on the AIJ pattern, with
-inputs resident in HBM.  Two forms are timed: plain FP64 (the primary line)
-and mixed (FP64 residual, FP32-MFMA tangent with FP32 values).
+A step is one full Newton solve on the GPU from the affine guess u = 0.01 X
+(Dirichlet rows set to their values) to ||R|| / ||R0|| < 1e-10: per Newton
+iteration the FP64 residual, the tangent on the AIJ pattern, GMRES(30) with a
+multigrid V-cycle and the backtracking line search, all with inputs resident
+in HBM.  Two forms: plain FP64 (the primary line) and mixed (FP32-MFMA
+tangent with FP64 accumulation over Gauss-point chunks, FP32 matrix values
+in the Krylov solve and the multigrid hierarchy's fine level).
+
+Also timed: the assembly alone (residual + tangent at a non-equilibrium state
+U = 1e-3 sin(pi X) sin(pi Y) sin(pi Z) (1, .5, .25) + 1e-6 noise) and the
+fine-level row pass (SpMV) that the Krylov and multigrid passes share.
 
     python bench.py [--steps K] [--warmup W] [--n 64] [--no-cpu]
 
@@ -25,18 +32,19 @@ sys.path.insert(0, ROOT)
 # (2*10*10*27 + 2*27*10*27) = 179,820  (i.e. 11,508,480 per element at nq = 4)
 FLOP_KE_PER_QP = 9 * (2 * 10 * 10 * 27 + 2 * 27 * 10 * 27)
 PEAK = {"f64": 78.6, "f32": 157.3}   # MI355X dense MFMA TFLOP/s
+PEAK_HBM = 8000.0                    # GB/s
 
 
 def profiled_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE and WRITE_SIZE runs, scaled by a plain copy)."""
-    path = os.path.join(ROOT, "profiles", "k_tangent_ge_64cube_traffic.txt")
-    try:
-        for line in open(path):
-            if line.startswith("traffic_bytes_per_launch") and kernel in line:
-                return float(line.split("=")[1].split()[0])
-    except OSError:
-        pass
+    summaries (FETCH_SIZE and WRITE_SIZE runs, scaled by a plain copy)."""
+    for name in ("k_tangent_ge_64cube_traffic.txt", "k_spmv_64cube_traffic.txt"):
+        try:
+            for line in open(os.path.join(ROOT, "profiles", name)):
+                if line.startswith("traffic_bytes_per_launch") and kernel in line:
+                    return float(line.split("=")[1].split()[0])
+        except OSError:
+            pass
     return None
 
 
@@ -45,7 +53,7 @@ def metric_name():
         with open(os.path.join(ROOT, "BASELINE.json")) as f:
             return json.load(f)["metric"]
     except Exception:
-        return "Newton-step assembly elements/s, 64^3 3D ge"
+        return "Newton-step s & elements/s, 64^3 3D ge"
 
 
 def cpu_baseline(pb, U, budget_s=20.0):
@@ -69,20 +77,24 @@ def cpu_baseline(pb, U, budget_s=20.0):
     O.residual_elements_timed(pb, U, R, 0, n_r)
     t_r = (time.perf_counter() - t0) / n_r
     return {"value": 1.0 / (t_k + t_r), "unit": "elements/s", "cores": 1, "kind": "port",
-            "sample": "oracle/ C restatement of snesiga_Tangent+snesiga_Residual loops, one core, "
-                      "%d tangent + %d residual elements of the same 64^3 workload (%.1f s)" % (n_k, n_r, n_k * t_k + n_r * t_r),
+            "sample": "oracle/ C restatement of the snesiga_Tangent + snesiga_Residual loops, one core, "
+                      "%d tangent + %d residual elements of the same 64^3 workload (%.1f s). Assembly only: "
+                      "an upper bound on a CPU Newton step's elements/s (the CPU linear solve is not timed)"
+                      % (n_k, n_r, n_k * t_k + n_r * t_r),
             "tangent_s_per_element": t_k, "residual_s_per_element": t_r}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=3, help="timed Newton solves per form")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--asm-steps", type=int, default=5, help="timed assembly steps per form")
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--forms", default="fp64,mixed", help="comma list of fp64, mixed")
+    ap.add_argument("--no-newton", action="store_true", help="assembly and SpMV only")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -107,12 +119,15 @@ def main():
     from igap4standard_amd import problem as P
     from igap4standard_amd.snesiga import App
 
+    def say(*a):
+        print("bench:", *a, file=sys.stderr, flush=True)
+
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     n = args.n
     pb = P.stretch_problem(n, a=0.01, lam=1.0, mu=0.5, ell=0.1, noise=0.1, seed=0x5EED0002, nquad=4)
     U = P.smooth_state(pb, amp=1e-3, noise=1e-6, seed=0x5EED0012)
-    print("bench: building context for %d^3 (%d dofs)" % (n, pb.ndof), file=sys.stderr, flush=True)
+    say("building context for %d^3 (%d dofs)" % (n, pb.ndof))
     app = App(pb)
     stream = torch.cuda.current_stream(dev)
     sp = stream.cuda_stream
@@ -120,26 +135,18 @@ def main():
     dR = torch.empty(app.ndof, device=dev, dtype=torch.float64)
     dK64 = torch.empty(app.nnz, device=dev, dtype=torch.float64)
     dK32 = torch.empty(app.nnz, device=dev, dtype=torch.float32)
+    forms = {"fp64": (L.IGAP4H_FP64, L.IGAP4H_VALS_F64, dK64), "mixed": (L.IGAP4H_FP32, L.IGAP4H_VALS_F32, dK32)}
+    wanted = [f for f in args.forms.split(",") if f in forms]
 
-    forms = {
-        "fp64": (L.IGAP4H_FP64, L.IGAP4H_VALS_F64, dK64),
-        "mixed": (L.IGAP4H_FP32, L.IGAP4H_VALS_F32, dK32),
-    }
-    res = {}
-    wanted = args.forms.split(",")
-    for name, (prec, vt, dK) in forms.items():
-        if name not in wanted:
-            continue
-        def step():
+    # ---- assembly alone (residual + tangent), HIP events on the launch stream
+    asm = {}
+    for name in wanted:
+        prec, vt, dK = forms[name]
+        for _ in range(2):
             app.residual_dev(dUh.data_ptr(), dR.data_ptr(), sp)
             app.tangent_dev(dUh.data_ptr(), dK.data_ptr(), prec, vt, sp)
-
-        for _ in range(args.warmup):
-            step()
         torch.cuda.synchronize()
-        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
-               torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-        torch.cuda.synchronize()
+        ev = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(3)) for _ in range(args.asm_steps)]
         t0 = time.perf_counter()
         for e0, e1, e2 in ev:
             e0.record(stream)
@@ -149,43 +156,114 @@ def main():
             e2.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        r_ms = sum(a.elapsed_time(b) for a, b, _ in ev) / args.steps
-        k_ms = sum(b.elapsed_time(c) for _, b, c in ev) / args.steps
-        ms = wall * 1000.0 / args.steps
-        res[name] = {"ms_per_step": ms, "elements_per_s": pb.n_elem / (ms / 1000.0), "residual_ms": r_ms,
-                     "tangent_ms": k_ms, "tangent_tflops": pb.n_elem * FLOP_KE_PER_QP * pb.nquad ** 3 / (k_ms * 1e-3) / 1e12}
-        print("bench: %s %s" % (name, json.dumps(res[name])), file=sys.stderr, flush=True)
-    main_form = res.get("fp64") or res["mixed"]
-    achieved = main_form["tangent_tflops"]
+        r_ms = sum(a.elapsed_time(b) for a, b, _ in ev) / args.asm_steps
+        k_ms = sum(b.elapsed_time(c) for _, b, c in ev) / args.asm_steps
+        ms = wall * 1000.0 / args.asm_steps
+        asm[name] = {"ms_per_assembly": ms, "elements_per_s": pb.n_elem / (ms / 1000.0), "residual_ms": r_ms,
+                     "tangent_ms": k_ms,
+                     "tangent_tflops": pb.n_elem * FLOP_KE_PER_QP * pb.nquad ** 3 / (k_ms * 1e-3) / 1e12}
+        say("assembly", name, json.dumps(asm[name]))
+
+    # ---- fine-level row pass (y = K x), the traversal every Krylov/multigrid pass makes
+    spmv = {}
+    x = torch.tensor(np.random.default_rng(1).standard_normal(app.ndof), device=dev)
+    y = torch.empty_like(x)
+    for name in wanted:
+        prec, vt, dK = forms[name]
+        app.tangent_dev(dUh.data_ptr(), dK.data_ptr(), prec, vt, sp)
+        for _ in range(3):
+            app.spmv_dev(dK.data_ptr(), vt, x.data_ptr(), y.data_ptr(), sp)
+        nrep = 20
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(nrep):
+            app.spmv_dev(dK.data_ptr(), vt, x.data_ptr(), y.data_ptr(), sp)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1000.0 / nrep
+        vb = 8 if vt == L.IGAP4H_VALS_F64 else 4
+        alg = app.nnz * vb + 3 * app.ndof * 8 + (app.ndof + 1) * 8   # values, x, y, rowptr
+        spmv[name] = {"us": us, "algorithmic_bytes": alg, "GBps": alg / (us * 1e-6) / 1e9}
+        say("spmv", name, json.dumps(spmv[name]))
+
+    # ---- Newton solves (the timed step)
+    newton = {}
+    if not args.no_newton:
+        U0 = torch.tensor(P.affine_guess(pb), device=dev, dtype=torch.float64)
+        Uw = torch.empty_like(U0)
+        for name in wanted:
+            prec, vt, _ = forms[name]
+            o = app.newton_opts(rtol=1e-10, atol=0.0, dirichlet_lift=1, tangent_prec=prec, vals_type=vt)
+            for _ in range(args.warmup):
+                Uw.copy_(U0)
+                app.newton_solve_dev(Uw.data_ptr(), o, sp)
+            torch.cuda.synchronize()
+            runs = []
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                Uw.copy_(U0)
+                runs.append(app.newton_solve_dev(Uw.data_ptr(), o, sp))
+            torch.cuda.synchronize()
+            wall = (time.perf_counter() - t0) / args.steps
+            info = runs[-1]
+            its = info["newton_its"]
+            newton[name] = {"solve_s": wall, "newton_its": its, "ms_per_newton_step": wall * 1000.0 / its,
+                            "elements_per_s": pb.n_elem * its / wall, "ksp_its": info["ksp_its"],
+                            "ksp_its_per_step": info["ksp_its_hist"], "converged": info["converged"],
+                            "rel_fnorm": info["fnorm"] / info["fnorm0"], "fnorm": info["fnorm"],
+                            "abs_1e-10_reached": bool(info["fnorm"] < 1e-10),
+                            "t_tangent_s": info["t_tangent"], "t_solve_s": info["t_solve"],
+                            "t_residual_s": info["t_residual"],
+                            "fnorm_hist": ["%.3e" % f for f in info["fnorm_hist"]]}
+            say("newton", name, json.dumps(newton[name]))
+
+    main_asm = asm.get("fp64") or asm[wanted[0]]
+    main_sp = spmv.get("fp64") or spmv[wanted[0]]
+    main_nt = newton.get("fp64") or (newton[wanted[0]] if newton else None)
+    tangent_kernel = "k_tangent_ge<double,2>" if "fp64" in asm else "k_tangent_ge<float,4>"
+    spmv_kernel = "k_spmv<double,0,0>" if "fp64" in spmv else "k_spmv<float,0,0>"
+    if main_nt:
+        value, ms_step = main_nt["elements_per_s"], main_nt["ms_per_newton_step"]
+    else:
+        value, ms_step = main_asm["elements_per_s"], main_asm["ms_per_assembly"]
     line = {
         "metric": metric_name(),
-        "value": main_form["elements_per_s"],
+        "value": value,
         "unit": "elements/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": main_form["ms_per_step"],
+        "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "phys/ge residual+tangent assembly, %d^3 quadratic C1 B-spline elements, nq=4, "
-                               "clamp/stretch a=0.01, Lame +-10%% per element" % n,
+        "config": {"workload": "phys/ge Newton solve to ||R||/||R0||<1e-10 (residual, tangent, GMRES(30) + "
+                               "multigrid, line search), %d^3 quadratic C1 B-spline elements, nq=4, clamp/stretch "
+                               "a=0.01, Lame +-10%% per element; value = elements x Newton steps / solve time; "
+                               "ms_per_step = per Newton step; steps = timed solves" % n,
                    "nelem": pb.n_elem, "ndof": app.ndof, "nnz": app.nnz, "nquad": 4, "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK["f64"], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK["f64"], "traffic": profiled_traffic("k_tangent_ge<double,2>"),
-                     "traffic_source": "profiles/k_tangent_ge_64cube_traffic.txt (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1)",
-                     "kernel": "k_tangent_ge<double,2> (27 colour launches per tangent)",
-                     "algorithmic": "11,508,480 flop per element (9 x (2*10*10*27 + 2*27*10*27) per Gauss point)"},
-        "mixed": None if "mixed" not in res else {
-            **res["mixed"], "roofline_frac_f32": res["mixed"]["tangent_tflops"] / PEAK["f32"],
-            "tangent_speedup_vs_fp64": main_form["tangent_ms"] / res["mixed"]["tangent_ms"],
-            "step_speedup_vs_fp64": main_form["ms_per_step"] / res["mixed"]["ms_per_step"]},
-        "fp64": main_form,
+        "roofline": {"bound": "hbm", "achieved": main_sp["GBps"], "peak": PEAK_HBM, "unit": "GB/s",
+                     "frac": main_sp["GBps"] / PEAK_HBM, "traffic": profiled_traffic(spmv_kernel),
+                     "kernel": spmv_kernel + " (fine-level row pass; the Chebyshev smoothing and Krylov "
+                               "passes that dominate a Newton step traverse the matrix the same way)",
+                     "algorithmic": "nnz x value bytes + x, y (FP64) + rowptr per launch"},
+        "roofline_tangent": {"bound": "mfma", "achieved": main_asm["tangent_tflops"], "peak": PEAK["f64"],
+                             "unit": "TFLOP/s", "frac": main_asm["tangent_tflops"] / PEAK["f64"],
+                             "traffic": profiled_traffic(tangent_kernel), "kernel": tangent_kernel + " (27 colour launches per tangent)",
+                             "algorithmic": "11,508,480 flop per element (9 x (2*10*10*27 + 2*27*10*27) per Gauss point)"},
+        "newton": newton,
+        "assembly": asm,
+        "spmv": spmv,
     }
+    if "mixed" in asm and "fp64" in asm:
+        line["assembly"]["mixed"]["roofline_frac_f32"] = asm["mixed"]["tangent_tflops"] / PEAK["f32"]
+        line["assembly"]["mixed"]["tangent_speedup_vs_fp64"] = asm["fp64"]["tangent_ms"] / asm["mixed"]["tangent_ms"]
+    if "mixed" in newton and "fp64" in newton:
+        line["newton"]["mixed_speedup_vs_fp64"] = newton["fp64"]["solve_s"] / newton["mixed"]["solve_s"]
     if not args.no_cpu:
-        print("bench: CPU baseline (budget %.0f s)" % args.cpu_budget, file=sys.stderr, flush=True)
+        say("CPU baseline (budget %.0f s)" % args.cpu_budget)
         line["cpu_baseline"] = cpu_baseline(pb, U, args.cpu_budget)
     else:
         line["cpu_baseline"] = None

@@ -49,7 +49,9 @@ __device__ __forceinline__ void node_box(const SpArgs &s, long long node, int a[
     }
 }
 
-// acc[i] = sum_k K[3n+i, col_k] x[col_k] over the node's rows (one wave)
+// acc[i] = sum_k K[3n+i, col_k] x[col_k] over the node's rows (one wave).
+// A row has at most 375 entries, so each lane has at most 6: all loads are
+// issued before the first product (one memory round trip per node).
 template <class VT>
 __device__ __forceinline__ void node_rows(const SpArgs &s, long long node, const VT *__restrict__ vals,
                                           const double *__restrict__ x, int lane, double acc[3])
@@ -59,19 +61,33 @@ __device__ __forceinline__ void node_rows(const SpArgs &s, long long node, const
     const long long base = s.rowptr[3 * node];
     const int L = 3 * w[0] * w[1] * w[2];
     const float r2 = 1.0f / (float)w[2], r1 = 1.0f / (float)w[1];
+    double xv[6];
+    VT v0[6], v1[6], v2[6];
+#pragma unroll
+    for (int it = 0; it < 6; it++) {
+        const int k = lane + 64 * it;
+        xv[it] = 0.0;
+        v0[it] = v1[it] = v2[it] = (VT)0;
+        if (k < L) {
+            int t = (k * 43691) >> 17;  // k / 3 for k < 131072
+            int j = k - 3 * t;
+            int t2 = (int)(((float)t + 0.5f) * r2);  // t / w2 (exact: t < 128, w2 <= 5)
+            int o2 = t - t2 * w[2];
+            int o0 = (int)(((float)t2 + 0.5f) * r1);
+            int o1 = t2 - o0 * w[1];
+            long long cn = ((long long)(lo[0] + o0) * s.nb[1] + (lo[1] + o1)) * s.nb[2] + (lo[2] + o2);
+            xv[it] = x[3 * cn + j];
+            v0[it] = vals[base + k];
+            v1[it] = vals[base + L + k];
+            v2[it] = vals[base + 2 * L + k];
+        }
+    }
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int k = lane; k < L; k += 64) {
-        int t = (k * 43691) >> 17;  // k / 3 for k < 131072
-        int j = k - 3 * t;
-        int t2 = (int)(((float)t + 0.5f) * r2);  // t / w2 (exact: t < 128, w2 <= 5)
-        int o2 = t - t2 * w[2];
-        int o0 = (int)(((float)t2 + 0.5f) * r1);
-        int o1 = t2 - o0 * w[1];
-        long long cn = ((long long)(lo[0] + o0) * s.nb[1] + (lo[1] + o1)) * s.nb[2] + (lo[2] + o2);
-        double xv = x[3 * cn + j];
-        s0 += (double)vals[base + k] * xv;
-        s1 += (double)vals[base + L + k] * xv;
-        s2 += (double)vals[base + 2 * L + k] * xv;
+#pragma unroll
+    for (int it = 0; it < 6; it++) {
+        s0 += (double)v0[it] * xv[it];
+        s1 += (double)v1[it] * xv[it];
+        s2 += (double)v2[it] * xv[it];
     }
     acc[0] = wave_sum(s0);
     acc[1] = wave_sum(s1);
@@ -382,26 +398,38 @@ __device__ __forceinline__ void decode_node(const int nb[3], long long node, int
     a[0] = (int)(node / ((long long)nb[2] * nb[1]));
 }
 
-// acc[i] = sum over the stencil of K[(n,i),(n+d,j)] x[n+d, j] (one wave)
+// acc[i] = sum over the stencil of K[(n,i),(n+d,j)] x[n+d, j] (one wave;
+// loads issued together as in node_rows)
 __device__ __forceinline__ void node_rows_sm(const SpArgs &s, long long node, const double *__restrict__ sm,
                                              const double *__restrict__ x, int lane, double acc[3])
 {
     int a[3];
     decode_node(s.nb, node, a);
     const double *v = sm + node * 1125;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int k = lane; k < 375; k += 64) {
+    double xv[6], v0[6], v1[6], v2[6];
+#pragma unroll
+    for (int it = 0; it < 6; it++) {
+        const int k = lane + 64 * it;
+        xv[it] = v0[it] = v1[it] = v2[it] = 0.0;
         int o = (k * 43691) >> 17;     // k / 3
         int j = k - 3 * o;
         int q0 = (o * 1311) >> 15;     // o / 25 (o < 125)
         int r0 = o - 25 * q0;
         int q1 = (r0 * 13108) >> 16;   // r0 / 5
         int b0 = a[0] + q0 - 2, b1 = a[1] + q1 - 2, b2 = a[2] + (r0 - 5 * q1) - 2;
-        if (b0 < 0 || b0 >= s.nb[0] || b1 < 0 || b1 >= s.nb[1] || b2 < 0 || b2 >= s.nb[2]) continue;
-        double xv = x[3 * (((long long)b0 * s.nb[1] + b1) * s.nb[2] + b2) + j];
-        s0 += v[k] * xv;
-        s1 += v[375 + k] * xv;
-        s2 += v[750 + k] * xv;
+        if (k < 375 && b0 >= 0 && b0 < s.nb[0] && b1 >= 0 && b1 < s.nb[1] && b2 >= 0 && b2 < s.nb[2]) {
+            xv[it] = x[3 * (((long long)b0 * s.nb[1] + b1) * s.nb[2] + b2) + j];
+            v0[it] = v[k];
+            v1[it] = v[375 + k];
+            v2[it] = v[750 + k];
+        }
+    }
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int it = 0; it < 6; it++) {
+        s0 += v0[it] * xv[it];
+        s1 += v1[it] * xv[it];
+        s2 += v2[it] * xv[it];
     }
     acc[0] = wave_sum(s0);
     acc[1] = wave_sum(s1);
@@ -447,6 +475,168 @@ __global__ __launch_bounds__(256) void k_gs_colour(SpArgs s, int c0, int c1, int
         for (int i = 0; i < 3; i++) q[i] = b[3 * node + i] - acc[i];
         const double *B = Dinv + 9 * node;
         for (int i = 0; i < 3; i++) x[3 * node + i] += B[3 * i] * q[0] + B[3 * i + 1] * q[1] + B[3 * i + 2] * q[2];
+    }
+}
+
+// A whole block Gauss-Seidel sweep (27 colours in order) by one work-group:
+// for the small coarse levels, where a launch per colour costs more than
+// the colour's work.  The work-group barrier orders the colours.
+template <int BWD>
+__global__ __launch_bounds__(1024) void k_gs_sweep_wg(SpArgs s, const double *__restrict__ sm, const double *__restrict__ Dinv,
+                                                      const double *__restrict__ b, double *x)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int cc = 0; cc < 27; cc++) {
+        const int col = BWD ? 26 - cc : cc;
+        const int c0 = col / 9, c1 = (col / 3) % 3, c2 = col % 3;
+        const int m0 = (s.nb[0] - c0 + 2) / 3, m1 = (s.nb[1] - c1 + 2) / 3, m2 = (s.nb[2] - c2 + 2) / 3;
+        const long long ncol = (long long)m0 * m1 * m2;
+        for (long long t = wv; t < ncol; t += nw) {
+            long long a2 = c2 + 3 * (t % m2), a1 = c1 + 3 * ((t / m2) % m1), a0 = c0 + 3 * (t / ((long long)m1 * m2));
+            long long node = (a0 * s.nb[1] + a1) * s.nb[2] + a2;
+            double acc[3];
+            node_rows_sm(s, node, sm, x, lane, acc);
+            if (lane == 0) {
+                double q[3];
+                for (int i = 0; i < 3; i++) q[i] = b[3 * node + i] - acc[i];
+                const double *B = Dinv + 9 * node;
+                for (int i = 0; i < 3; i++) x[3 * node + i] += B[3 * i] * q[0] + B[3 * i + 1] * q[1] + B[3 * i + 2] * q[2];
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
+}
+
+// ---- Chebyshev smoothing preconditioned by the nodal block diagonal
+// (Saad, "Iterative Methods for Sparse Linear Systems", 2nd ed., Alg. 12.1):
+// on [a, b] with theta = (a+b)/2, delta = (b-a)/2, sigma = theta/delta:
+//   r0 = b - K x0, d0 = Dinv r0 / theta;
+//   x_{k+1} = x_k + d_k, r_{k+1} = r_k - K d_k,
+//   d_{k+1} = rho_{k+1} rho_k d_k + (2 rho_{k+1} / delta) Dinv r_{k+1},
+//   rho_0 = 1/sigma, rho_{k+1} = 1 / (2 sigma - rho_k).
+// Each kernel reads neighbours only of a vector it does not write.
+
+// r = b - K x, d = Dinv r / theta (one wave per node)
+template <int SM, class VT>
+__global__ __launch_bounds__(256) void k_cheb_init(SpArgs s, const VT *__restrict__ vals, const double *__restrict__ Dinv,
+                                                   const double *__restrict__ b, const double *__restrict__ x,
+                                                   double *__restrict__ r, double *__restrict__ d, double inv_theta)
+{
+    const int lane = threadIdx.x & 63;
+    const long long node = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (node >= s.nnode) return;
+    double acc[3];
+    level_rows<SM, VT>(s, node, vals, x, lane, acc);
+    if (lane == 0) {
+        double q[3];
+        for (int i = 0; i < 3; i++) {
+            q[i] = b[3 * node + i] - acc[i];
+            r[3 * node + i] = q[i];
+        }
+        const double *B = Dinv + 9 * node;
+        for (int i = 0; i < 3; i++) d[3 * node + i] = inv_theta * (B[3 * i] * q[0] + B[3 * i + 1] * q[1] + B[3 * i + 2] * q[2]);
+    }
+}
+
+// start from x = 0: r = b, d = Dinv b / theta, x = 0 (thread per node)
+__global__ void k_cheb_zero(long long nnode, const double *__restrict__ Dinv, const double *__restrict__ b, double *__restrict__ r,
+                            double *__restrict__ d, double *__restrict__ x, double inv_theta)
+{
+    long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nnode) return;
+    double q[3] = {b[3 * n], b[3 * n + 1], b[3 * n + 2]};
+    const double *B = Dinv + 9 * n;
+    for (int i = 0; i < 3; i++) {
+        r[3 * n + i] = q[i];
+        x[3 * n + i] = 0.0;
+        d[3 * n + i] = inv_theta * (B[3 * i] * q[0] + B[3 * i + 1] * q[1] + B[3 * i + 2] * q[2]);
+    }
+}
+
+// x += din, r -= K din, and if newd: dout = c1 din + c2 Dinv r
+template <int SM, class VT>
+__global__ __launch_bounds__(256) void k_cheb_step(SpArgs s, const VT *__restrict__ vals, const double *__restrict__ Dinv,
+                                                   const double *__restrict__ din, double *__restrict__ dout,
+                                                   double *__restrict__ r, double *__restrict__ x, double c1, double c2,
+                                                   int newd)
+{
+    const int lane = threadIdx.x & 63;
+    const long long node = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (node >= s.nnode) return;
+    double acc[3];
+    level_rows<SM, VT>(s, node, vals, din, lane, acc);
+    if (lane == 0) {
+        double q[3];
+        for (int i = 0; i < 3; i++) {
+            q[i] = r[3 * node + i] - acc[i];
+            r[3 * node + i] = q[i];
+            x[3 * node + i] += din[3 * node + i];
+        }
+        if (newd) {
+            const double *B = Dinv + 9 * node;
+            for (int i = 0; i < 3; i++)
+                dout[3 * node + i] = c1 * din[3 * node + i] + c2 * (B[3 * i] * q[0] + B[3 * i + 1] * q[1] + B[3 * i + 2] * q[2]);
+        }
+    }
+}
+
+// fine level: the row pass is k_spmv; these apply the per-node updates
+// d = Dinv r / theta
+__global__ void k_cheb_dfromr(long long nnode, const double *__restrict__ Dinv, const double *__restrict__ r,
+                              double *__restrict__ d, double inv_theta)
+{
+    long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nnode) return;
+    const double *B = Dinv + 9 * n;
+    double q0 = r[3 * n], q1 = r[3 * n + 1], q2 = r[3 * n + 2];
+    for (int i = 0; i < 3; i++) d[3 * n + i] = inv_theta * (B[3 * i] * q0 + B[3 * i + 1] * q1 + B[3 * i + 2] * q2);
+}
+// r -= q (q = K din), x += din, and if newd: dout = c1 din + c2 Dinv r
+__global__ void k_cheb_update(long long nnode, const double *__restrict__ Dinv, const double *__restrict__ q,
+                              const double *__restrict__ din, double *__restrict__ dout, double *__restrict__ r,
+                              double *__restrict__ x, double c1, double c2, int newd)
+{
+    long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nnode) return;
+    double rv[3];
+    for (int i = 0; i < 3; i++) {
+        rv[i] = r[3 * n + i] - q[3 * n + i];
+        r[3 * n + i] = rv[i];
+        x[3 * n + i] += din[3 * n + i];
+    }
+    if (newd) {
+        const double *B = Dinv + 9 * n;
+        for (int i = 0; i < 3; i++)
+            dout[3 * n + i] = c1 * din[3 * n + i] + c2 * (B[3 * i] * rv[0] + B[3 * i + 1] * rv[1] + B[3 * i + 2] * rv[2]);
+    }
+}
+
+// y = Dinv K x (power iteration for the largest eigenvalue of Dinv K)
+template <int SM, class VT>
+__global__ __launch_bounds__(256) void k_dinv_k(SpArgs s, const VT *__restrict__ vals, const double *__restrict__ Dinv,
+                                                const double *__restrict__ x, double *__restrict__ y)
+{
+    const int lane = threadIdx.x & 63;
+    const long long node = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (node >= s.nnode) return;
+    double acc[3];
+    level_rows<SM, VT>(s, node, vals, x, lane, acc);
+    if (lane == 0) {
+        const double *B = Dinv + 9 * node;
+        for (int i = 0; i < 3; i++) y[3 * node + i] = B[3 * i] * acc[0] + B[3 * i + 1] * acc[1] + B[3 * i + 2] * acc[2];
+    }
+}
+
+// deterministic pseudo-random start vector in [-1, 1]
+__global__ void k_hash_vec(long long n, double *v)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        unsigned long long z = (unsigned long long)i * 0x9E3779B97F4A7C15ULL + 0x5EEDULL;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z ^= z >> 31;
+        v[i] = 2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
     }
 }
 
@@ -594,15 +784,17 @@ __global__ void k_prolong_add(Xfer3 x, const double *__restrict__ xc, double *__
     xf[3 * F + 2] += s2;
 }
 
-// x = Ainv b (dense n x n, one block)
+// x = Ainv b (dense n x n), one wave per row
 __global__ __launch_bounds__(256) void k_dense_mv(int n, const double *__restrict__ Ainv, const double *__restrict__ b,
                                                   double *__restrict__ x)
 {
-    for (int r = threadIdx.x; r < n; r += blockDim.x) {
-        double s = 0.0;
-        for (int c = 0; c < n; c++) s += Ainv[(long long)r * n + c] * b[c];
-        x[r] = s;
-    }
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n) return;
+    double s = 0.0;
+    for (int c = lane; c < n; c += 64) s += Ainv[(long long)r * n + c] * b[c];
+    s = wave_sum(s);
+    if (lane == 0) x[r] = s;
 }
 
 }  // namespace
@@ -621,11 +813,19 @@ struct MGLevel {
     Xfer3 T;                              // views into xfer_i / xfer_w
     double *Ainv = nullptr;               // coarsest level: dense inverse
     bool dense = false;
+    double *d[2] = {nullptr, nullptr};    // Chebyshev directions
+    double lmax = 0.0;                    // largest eigenvalue estimate of Dinv K
 };
 }  // namespace
 
 struct igap4h::Solver {
     std::vector<MGLevel> mg;  // mg[0] is the fine level (its matrix is the assembled AIJ values)
+    // the V-cycle captured as a graph (in: mg_in, out: z); rebuilt when the key changes
+    hipStream_t cap = nullptr;
+    hipGraphExec_t mg_exec = nullptr;
+    const void *key_vals = nullptr;
+    int key_f32 = -1, key_pre = -1, key_post = -1, key_smoother = -1;
+    double *mg_in = nullptr;
     long long ndof = 0, nnode = 0;
     int restart = 0;
     size_t vals_bytes = 0;
@@ -652,8 +852,11 @@ void igap4h::solver_free(igap4h_ctx *c)
         (void)hipFree(p);
     if (s->hp) (void)hipHostFree(s->hp);
     if (s->hh) (void)hipHostFree(s->hh);
+    if (s->mg_exec) (void)hipGraphExecDestroy(s->mg_exec);
+    if (s->cap) (void)hipStreamDestroy(s->cap);
+    (void)hipFree(s->mg_in);
     for (MGLevel &L : s->mg) {
-        for (double *p : {L.sm, L.Dblk, L.Dinv, L.x, L.b, L.r, L.stage[0], L.stage[1], L.xfer_w, L.Ainv})
+        for (double *p : {L.sm, L.Dblk, L.Dinv, L.x, L.b, L.r, L.stage[0], L.stage[1], L.xfer_w, L.Ainv, L.d[0], L.d[1]})
             (void)hipFree(p);
         (void)hipFree(L.xfer_i);
     }
@@ -739,7 +942,9 @@ static int mg_build(igap4h_ctx *c, Solver *s)
     L0.ndof = 3 * L0.nnode;
     s->mg.push_back(L0);
     int rc;
-    if ((rc = dev_alloc(&s->mg[0].r, (size_t)L0.ndof))) return rc;
+    if ((rc = dev_alloc(&s->mg[0].r, (size_t)L0.ndof)) || (rc = dev_alloc(&s->mg[0].d[0], (size_t)L0.ndof)) ||
+        (rc = dev_alloc(&s->mg[0].d[1], (size_t)L0.ndof)) || (rc = dev_alloc(&s->mg[0].x, (size_t)L0.ndof)))
+        return rc;  // mg[0].x: fine-level product buffer (the level's solution is the caller's z)
     while (s->mg.size() < 12) {
         MGLevel &F = s->mg.back();
         bool can = F.ndof > MG_DENSE;
@@ -802,7 +1007,8 @@ static int mg_build(igap4h_ctx *c, Solver *s)
         if ((rc = dev_alloc(&F.stage[0], st0)) || (rc = dev_alloc(&F.stage[1], st1))) return rc;
         if ((rc = dev_alloc(&C.sm, (size_t)C.nnode * 1125)) || (rc = dev_alloc(&C.Dblk, 9 * (size_t)C.nnode)) ||
             (rc = dev_alloc(&C.Dinv, 9 * (size_t)C.nnode)) || (rc = dev_alloc(&C.x, (size_t)C.ndof)) ||
-            (rc = dev_alloc(&C.b, (size_t)C.ndof)) || (rc = dev_alloc(&C.r, (size_t)C.ndof)))
+            (rc = dev_alloc(&C.b, (size_t)C.ndof)) || (rc = dev_alloc(&C.r, (size_t)C.ndof)) ||
+            (rc = dev_alloc(&C.d[0], (size_t)C.ndof)) || (rc = dev_alloc(&C.d[1], (size_t)C.ndof)))
             return rc;
         for (int d = 0; d < 3; d++) kf[d] = kc[d];
         s->mg.push_back(C);
@@ -861,6 +1067,22 @@ static bool dense_inverse(std::vector<double> &A, int n, std::vector<double> &in
 // ====================================================================== operators
 namespace {
 
+constexpr long long MG_WG_NODES = 512;  // levels up to this size smooth in one work-group
+constexpr double CHEB_RATIO = 10.0;     // Chebyshev interval [lmax' / CHEB_RATIO, lmax'], lmax' = 1.1 lmax
+constexpr int CHEB_POWER_ITS = 12;      // power iterations for lmax
+
+// launch a one-wave-per-node level kernel KERNEL<SM, VT> on level l of Op *this
+#define LEVEL_LAUNCH(l, KERNEL, ...)                                                                                   \
+    do {                                                                                                               \
+        const MGLevel &L_ = s->mg[l];                                                                                  \
+        SpArgs a_ = (l) ? level_args(L_) : sp;                                                                         \
+        dim3 g_(grid_for(a_.nnode, 4));                                                                                \
+        if (l) hipLaunchKernelGGL((KERNEL<1, double>), g_, dim3(256), 0, st, a_, (const double *)L_.sm, __VA_ARGS__);  \
+        else if (f32) hipLaunchKernelGGL((KERNEL<0, float>), g_, dim3(256), 0, st, a_, (const float *)vals, __VA_ARGS__); \
+        else hipLaunchKernelGGL((KERNEL<0, double>), g_, dim3(256), 0, st, a_, (const double *)vals, __VA_ARGS__);    \
+        HIPCHK(hipGetLastError());                                                                                     \
+    } while (0)
+
 struct Op {
     igap4h_ctx *c;
     Solver *s;
@@ -869,6 +1091,9 @@ struct Op {
     int f32;
     int pc;
     hipStream_t st;
+    int nu_pre = 1, nu_post = 1;  // Gauss-Seidel sweeps before / after the coarse correction
+    int mg_graph = 1;             // replay the V-cycle from a captured graph
+    int smoother = 0;             // 0 block Gauss-Seidel, 1 Chebyshev (degree nu_pre / nu_post)
 
     int spmv(const double *x, double *y, const double *b = nullptr) const
     {
@@ -897,6 +1122,12 @@ struct Op {
     {
         const MGLevel &L = s->mg[l];
         SpArgs a = l ? level_args(L) : sp;
+        if (l && L.nnode <= MG_WG_NODES) {
+            if (bwd) hipLaunchKernelGGL(k_gs_sweep_wg<1>, dim3(1), dim3(1024), 0, st, a, (const double *)L.sm, (const double *)L.Dinv, b, x);
+            else hipLaunchKernelGGL(k_gs_sweep_wg<0>, dim3(1), dim3(1024), 0, st, a, (const double *)L.sm, (const double *)L.Dinv, b, x);
+            HIPCHK(hipGetLastError());
+            return 0;
+        }
         const double *Dinv = l ? L.Dinv : s->Dinv;
         for (int cc = 0; cc < 27; cc++) {
             int col = bwd ? 26 - cc : cc;
@@ -914,7 +1145,7 @@ struct Op {
     }
     int mg_setup() const
     {
-        int rc;
+        int rc = 0;
         for (size_t l = 0; l + 1 < s->mg.size(); l++) {
             const MGLevel &F = s->mg[l];
             const MGLevel &C = s->mg[l + 1];
@@ -936,6 +1167,9 @@ struct Op {
             hipLaunchKernelGGL(k_block_diag_sm, dim3(grid_for(C.nnode, 256)), dim3(256), 0, st, C.nnode, (const double *)C.sm, C.Dblk, C.Dinv);
             HIPCHK(hipGetLastError());
         }
+        if (smoother == 1)
+            for (size_t l = 0; l + 1 < s->mg.size(); l++)
+                if ((rc = estimate_lmax(l))) return rc;
         MGLevel &B = s->mg.back();
         if (B.dense) {
             std::vector<double> h((size_t)B.nnode * 1125), A((size_t)B.ndof * B.ndof, 0.0), inv;
@@ -959,6 +1193,79 @@ struct Op {
         (void)rc;
         return 0;
     }
+    int norm_n(const double *x, long long n, double *out_dev) const
+    {
+        hipLaunchKernelGGL(k_dot2, dim3(NRB), dim3(256), 0, st, n, x, x, (const double *)nullptr, (const double *)nullptr, s->partial);
+        HIPCHK(hipGetLastError());
+        return finish(s->partial, 1, NRB, out_dev, true);
+    }
+    // largest eigenvalue of Dinv K on level l by power iteration
+    int estimate_lmax(size_t l) const
+    {
+        MGLevel &L = s->mg[l];
+        const double *Dinv = l ? L.Dinv : s->Dinv;
+        double *v = L.d[0], *w = L.d[1];
+        const long long n = L.ndof;
+        int rc;
+        hipLaunchKernelGGL(k_hash_vec, dim3(vgrid(n)), dim3(256), 0, st, n, v);
+        HIPCHK(hipGetLastError());
+        if ((rc = norm_n(v, n, s->sc + 12))) return rc;
+        hipLaunchKernelGGL(k_scale_inv, dim3(vgrid(n)), dim3(256), 0, st, n, (const double *)(s->sc + 12), v);
+        for (int it = 0; it < CHEB_POWER_ITS; it++) {
+            LEVEL_LAUNCH(l, k_dinv_k, Dinv, (const double *)v, w);
+            if ((rc = norm_n(w, n, s->sc + 12))) return rc;
+            hipLaunchKernelGGL(k_scale_inv, dim3(vgrid(n)), dim3(256), 0, st, n, (const double *)(s->sc + 12), w);
+            HIPCHK(hipGetLastError());
+            std::swap(v, w);
+        }
+        if ((rc = read_scalars(12, 1))) return rc;
+        L.lmax = s->hp[12];
+        if (!(L.lmax > 0.0) || !std::isfinite(L.lmax)) return fail(IGAP4H_EINVAL, "multigrid: eigenvalue estimate %g on level %zu", L.lmax, l);
+        return 0;
+    }
+    // Chebyshev smoothing of degree m on level l for K x = b (from x = 0 if
+    // from_zero); with want_res, L.r = b - K x on return
+    int cheb(size_t l, const double *b, double *x, int m, bool from_zero, bool want_res) const
+    {
+        const MGLevel &L = s->mg[l];
+        const double *Dinv = l ? L.Dinv : s->Dinv;
+        const double bmax = 1.1 * L.lmax, amin = bmax / CHEB_RATIO;
+        const double theta = 0.5 * (bmax + amin), delta = 0.5 * (bmax - amin), sigma = theta / delta;
+        double rho = 1.0 / sigma;
+        double *r = L.r, *d0 = L.d[0], *d1 = L.d[1];
+        const dim3 gn(grid_for(L.nnode, 256));
+        int rc;
+        if (from_zero) {
+            hipLaunchKernelGGL(k_cheb_zero, gn, dim3(256), 0, st, L.nnode, Dinv, b, r, d0, x, 1.0 / theta);
+            HIPCHK(hipGetLastError());
+        } else if (l == 0) {  // fine level: the row pass is the SpMV kernel
+            if ((rc = spmv(x, r, b))) return rc;
+            hipLaunchKernelGGL(k_cheb_dfromr, gn, dim3(256), 0, st, L.nnode, Dinv, (const double *)r, d0, 1.0 / theta);
+            HIPCHK(hipGetLastError());
+        } else {
+            LEVEL_LAUNCH(l, k_cheb_init, Dinv, b, (const double *)x, r, d0, 1.0 / theta);
+        }
+        for (int k = 1; k <= m; k++) {
+            const bool last = k == m;
+            if (last && !want_res) {
+                hipLaunchKernelGGL(k_waxpy, dim3(vgrid(L.ndof)), dim3(256), 0, st, L.ndof, 1.0, (const double *)x, (const double *)d0, x);
+                HIPCHK(hipGetLastError());
+                break;
+            }
+            const double rn = 1.0 / (2.0 * sigma - rho);
+            if (l == 0) {
+                if ((rc = spmv(d0, L.x))) return rc;
+                hipLaunchKernelGGL(k_cheb_update, gn, dim3(256), 0, st, L.nnode, Dinv, (const double *)L.x, (const double *)d0, d1, r, x,
+                                   rn * rho, 2.0 * rn / delta, last ? 0 : 1);
+                HIPCHK(hipGetLastError());
+            } else {
+                LEVEL_LAUNCH(l, k_cheb_step, Dinv, (const double *)d0, d1, r, x, rn * rho, 2.0 * rn / delta, last ? 0 : 1);
+            }
+            rho = rn;
+            std::swap(d0, d1);
+        }
+        return 0;
+    }
     // x = V-cycle(b) on level l (x overwritten)
     int vcycle(size_t l, const double *b, double *x) const
     {
@@ -967,7 +1274,7 @@ struct Op {
         int rc;
         if (l + 1 == s->mg.size()) {
             if (L.dense) {
-                hipLaunchKernelGGL(k_dense_mv, dim3(1), dim3(256), 0, st, (int)n, (const double *)L.Ainv, b, x);
+                hipLaunchKernelGGL(k_dense_mv, dim3(grid_for(n, 4)), dim3(256), 0, st, (int)n, (const double *)L.Ainv, b, x);
                 HIPCHK(hipGetLastError());
                 return 0;
             }
@@ -978,21 +1285,86 @@ struct Op {
             return 0;
         }
         const MGLevel &C = s->mg[l + 1];
+        if (smoother == 1) {
+            const double *res = b;
+            if (nu_pre > 0) {
+                if ((rc = cheb(l, b, x, nu_pre, true, true))) return rc;
+                res = L.r;
+            } else {
+                hipLaunchKernelGGL(k_zero, dim3(vgrid(n)), dim3(256), 0, st, x, n);
+                HIPCHK(hipGetLastError());
+            }
+            hipLaunchKernelGGL(k_restrict, dim3(grid_for(C.nnode, 256)), dim3(256), 0, st, L.T, res, C.b);
+            HIPCHK(hipGetLastError());
+            if ((rc = vcycle(l + 1, C.b, C.x))) return rc;
+            hipLaunchKernelGGL(k_prolong_add, dim3(grid_for(L.nnode, 256)), dim3(256), 0, st, L.T, (const double *)C.x, x);
+            HIPCHK(hipGetLastError());
+            if (nu_post > 0) return cheb(l, b, x, nu_post, false, false);
+            return 0;
+        }
         hipLaunchKernelGGL(k_zero, dim3(vgrid(n)), dim3(256), 0, st, x, n);
         HIPCHK(hipGetLastError());
-        if ((rc = gs_sweep(l, false, b, x))) return rc;
-        if (l) {
-            hipLaunchKernelGGL(k_resid_sm, dim3(grid_for(L.nnode, 4)), dim3(256), 0, st, level_args(L), (const double *)L.sm, (const double *)x, b, L.r);
-            HIPCHK(hipGetLastError());
-        } else if ((rc = spmv(x, L.r, b))) {
-            return rc;
+        for (int k = 0; k < nu_pre; k++)
+            if ((rc = gs_sweep(l, false, b, x))) return rc;
+        const double *res = b;  // with no pre-smoothing x = 0 and the residual is b
+        if (nu_pre > 0) {
+            if (l) {
+                hipLaunchKernelGGL(k_resid_sm, dim3(grid_for(L.nnode, 4)), dim3(256), 0, st, level_args(L), (const double *)L.sm, (const double *)x, b, L.r);
+                HIPCHK(hipGetLastError());
+            } else if ((rc = spmv(x, L.r, b))) {
+                return rc;
+            }
+            res = L.r;
         }
-        hipLaunchKernelGGL(k_restrict, dim3(grid_for(C.nnode, 256)), dim3(256), 0, st, L.T, (const double *)L.r, C.b);
+        hipLaunchKernelGGL(k_restrict, dim3(grid_for(C.nnode, 256)), dim3(256), 0, st, L.T, res, C.b);
         HIPCHK(hipGetLastError());
         if ((rc = vcycle(l + 1, C.b, C.x))) return rc;
         hipLaunchKernelGGL(k_prolong_add, dim3(grid_for(L.nnode, 256)), dim3(256), 0, st, L.T, (const double *)C.x, x);
         HIPCHK(hipGetLastError());
-        return gs_sweep(l, true, b, x);
+        for (int k = 0; k < nu_post; k++)
+            if ((rc = gs_sweep(l, true, b, x))) return rc;
+        return 0;
+    }
+    // z = V-cycle(r), replayed from a captured graph (z must be s->z)
+    int mg_apply(const double *r, double *z) const
+    {
+        if (z != s->z) return vcycle(0, r, z);
+        const long long n = 3 * sp.nnode;
+        if (!s->mg_exec || s->key_vals != vals || s->key_f32 != f32 || s->key_pre != nu_pre || s->key_post != nu_post ||
+            s->key_smoother != smoother) {
+            if (s->mg_exec) {
+                HIPCHK(hipGraphExecDestroy(s->mg_exec));
+                s->mg_exec = nullptr;
+            }
+            int rc;
+            if (!s->mg_in && (rc = dev_alloc(&s->mg_in, (size_t)n))) return rc;
+            if (!s->cap) HIPCHK(hipStreamCreateWithFlags(&s->cap, hipStreamNonBlocking));
+            Op cop = *this;
+            cop.st = s->cap;
+            HIPCHK(hipStreamBeginCapture(s->cap, hipStreamCaptureModeRelaxed));
+            rc = cop.vcycle(0, s->mg_in, s->z);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(s->cap, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(IGAP4H_EHIP, "V-cycle capture: %s", hipGetErrorString(e));
+            e = hipGraphInstantiate(&s->mg_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                s->mg_exec = nullptr;
+                return fail(IGAP4H_EHIP, "V-cycle graph instantiation: %s", hipGetErrorString(e));
+            }
+            s->key_vals = vals;
+            s->key_f32 = f32;
+            s->key_pre = nu_pre;
+            s->key_post = nu_post;
+            s->key_smoother = smoother;
+        }
+        HIPCHK(hipMemcpyAsync(s->mg_in, r, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipGraphLaunch(s->mg_exec, st));
+        return 0;
     }
     int setup_pc() const
     {
@@ -1031,7 +1403,7 @@ struct Op {
             HIPCHK(hipMemcpyAsync(z, r, n * sizeof(double), hipMemcpyDeviceToDevice, st));
             return 0;
         }
-        if (pc == IGAP4H_PC_MG) return vcycle(0, r, z);
+        if (pc == IGAP4H_PC_MG) return mg_graph ? mg_apply(r, z) : vcycle(0, r, z);
         if (pc == IGAP4H_PC_BJACOBI3) {
             hipLaunchKernelGGL(k_bjacobi, dim3(grid_for(sp.nnode, 256)), dim3(256), 0, st, sp.nnode, (const double *)s->Dinv, r, z);
             HIPCHK(hipGetLastError());
@@ -1250,6 +1622,9 @@ int check_opts(const igap4h_newton_opts *o)
     if (o->ksp_type == IGAP4H_KSP_GMRES && (o->restart < 1 || o->restart > 4096))
         return fail(IGAP4H_EINVAL, "GMRES restart %d outside 1..4096", o->restart);
     if (o->ksp_max_it < 1) return fail(IGAP4H_EINVAL, "ksp_max_it must be >= 1");
+    if (o->pc == IGAP4H_PC_MG && (o->mg_pre < 0 || o->mg_pre > 4 || o->mg_post < 0 || o->mg_post > 4 || o->mg_pre + o->mg_post < 1))
+        return fail(IGAP4H_EINVAL, "multigrid sweeps pre %d / post %d outside 0..4 (sum >= 1)", o->mg_pre, o->mg_post);
+    if (o->mg_smoother != IGAP4H_MG_GS && o->mg_smoother != IGAP4H_MG_CHEBYSHEV) return fail(IGAP4H_EINVAL, "bad mg_smoother %d", o->mg_smoother);
     if (o->vals_type != IGAP4H_VALS_F64 && o->vals_type != IGAP4H_VALS_F32) return fail(IGAP4H_EINVAL, "bad vals_type %d", o->vals_type);
     if (o->tangent_prec != IGAP4H_FP64 && o->tangent_prec != IGAP4H_FP32) return fail(IGAP4H_EINVAL, "bad tangent_prec %d", o->tangent_prec);
     return 0;
@@ -1277,13 +1652,17 @@ extern "C" int igap4h_newton_default_opts(igap4h_newton_opts *o)
     o->tangent_prec = IGAP4H_FP64;
     o->vals_type = IGAP4H_VALS_F64;
     o->ksp_type = IGAP4H_KSP_GMRES;  // snes_init.c:32
-    o->restart = 200;
+    o->restart = 30;
     o->ksp_rtol = 1e-5;               // PETSc default (snes_init.c:26 keeps it)
     o->ksp_max_it = 20000;
     o->ksp_guess = 1;                 // snes_init.c:25
-    o->pc = IGAP4H_PC_BJACOBI3;
+    o->pc = IGAP4H_PC_MG;
     o->line_search = 1;               // snes_init.c:21-22
     o->stagnation = 1;                // snes_solve.c:30
+    o->mg_pre = 2;
+    o->mg_post = 2;
+    o->mg_graph = 1;
+    o->mg_smoother = IGAP4H_MG_CHEBYSHEV;
     return 0;
 }
 
@@ -1305,6 +1684,10 @@ extern "C" int igap4h_linear_solve_dev(igap4h_ctx *c, const void *d_vals, int va
     HIPCHK(hipSetDevice(c->device));
     if ((rc = solver_ws(c, o->ksp_type == IGAP4H_KSP_GMRES ? o->restart : 1, 0))) return rc;
     Op A{c, c->solver, sp_args(c), d_vals, vals_type == IGAP4H_VALS_F32, o->pc, (hipStream_t)stream};
+    A.nu_pre = o->mg_pre;
+    A.nu_post = o->mg_post;
+    A.mg_graph = o->mg_graph;
+    A.smoother = o->mg_smoother;
     if ((rc = A.setup_pc())) return rc;
     return linear_solve(A, d_b, d_x, o, its, relres);
 }
@@ -1323,6 +1706,10 @@ extern "C" int igap4h_newton_solve_dev(igap4h_ctx *c, double *d_Uh, const igap4h
     hipStream_t st = (hipStream_t)stream;
     const long long n = s->ndof;
     Op A{c, s, sp_args(c), s->vals, f32, o->pc, st};
+    A.nu_pre = o->mg_pre;
+    A.nu_post = o->mg_post;
+    A.mg_graph = o->mg_graph;
+    A.smoother = o->mg_smoother;
     double *U = d_Uh;  // slice 0: current iterate
 
     auto t_all = clk::now();

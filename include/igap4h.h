@@ -214,6 +214,7 @@ int igap4h_energy_dev(igap4h_ctx *ctx, const double *d_U, double *energy, void *
  * solve is bitwise reproducible on one device.                            */
 enum { IGAP4H_PC_NONE = 0, IGAP4H_PC_BJACOBI3 = 1, IGAP4H_PC_SGS = 2, IGAP4H_PC_MG = 3 };
 enum { IGAP4H_KSP_GMRES = 0, IGAP4H_KSP_CG = 1 };
+enum { IGAP4H_MG_GS = 0, IGAP4H_MG_CHEBYSHEV = 1 };
 
 typedef struct {
     double atol;       /* stop when ||R||_2 < atol (reference 1e-10); <= 0 off  */
@@ -240,6 +241,12 @@ typedef struct {
                           every Newton increment vanishes there and the zeroed columns
                           of the tangent (snesiga_Tangent.c:270) lose nothing; 0: start
                           from U as given (reference)                                */
+    int mg_pre, mg_post;/* IGAP4H_PC_MG: Gauss-Seidel sweeps before / after the coarse
+                          correction on each level (0..4 each, sum >= 1)           */
+    int mg_graph;      /* IGAP4H_PC_MG: 1 replays the V-cycle from a captured HIP graph */
+    int mg_smoother;   /* IGAP4H_PC_MG: IGAP4H_MG_GS (block Gauss-Seidel, mg_pre/mg_post
+                          sweeps) or IGAP4H_MG_CHEBYSHEV (block-Jacobi Chebyshev of
+                          degree mg_pre/mg_post, largest eigenvalue by power iteration) */
 } igap4h_newton_opts;
 
 typedef struct {

@@ -96,7 +96,8 @@ def test_linear_solve(torch_dev, ksp, pc, restart):
 
 @pytest.mark.parametrize("n", [8, 16, (16, 8, 12)])
 def test_multigrid_mesh_independent(torch_dev, n):
-    """GMRES + one V-cycle: iteration count bounded independently of h
+    """GMRES + one V-cycle (Gauss-Seidel or Chebyshev smoothing): iteration
+    count bounded independently of h
     (block Jacobi needs thousands at 16^3)."""
     torch, dev = torch_dev
     say("multigrid", n)
@@ -108,14 +109,17 @@ def test_multigrid_mesh_independent(torch_dev, n):
     dv = torch.tensor(vals, device=dev)
     db = torch.tensor(b, device=dev)
     dx = torch.zeros_like(db)
-    its, relres = app.linear_solve_dev(dv.data_ptr(), L.IGAP4H_VALS_F64, db.data_ptr(), dx.data_ptr(),
-                                       pc=L.IGAP4H_PC_MG, restart=60, ksp_rtol=1e-8)
-    torch.cuda.synchronize()
-    x = dx.cpu().numpy()
-    true_rel = np.linalg.norm(b - K @ x) / np.linalg.norm(b)
-    say("  its", its, "true", true_rel)
-    assert true_rel < 2e-8
-    assert its <= 60, its
+    for smoother, deg in ((L.IGAP4H_MG_GS, 1), (L.IGAP4H_MG_CHEBYSHEV, 2)):
+        dx.zero_()
+        its, relres = app.linear_solve_dev(dv.data_ptr(), L.IGAP4H_VALS_F64, db.data_ptr(), dx.data_ptr(),
+                                           pc=L.IGAP4H_PC_MG, restart=60, ksp_rtol=1e-8, mg_smoother=smoother,
+                                           mg_pre=deg, mg_post=deg)
+        torch.cuda.synchronize()
+        x = dx.cpu().numpy()
+        true_rel = np.linalg.norm(b - K @ x) / np.linalg.norm(b)
+        say("  smoother", smoother, "its", its, "true", true_rel)
+        assert true_rel < 2e-8
+        assert its <= 80, its
 
 
 def test_linear_solve_deterministic(torch_dev):
