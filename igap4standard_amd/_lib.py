@@ -92,7 +92,8 @@ class NewtonInfo(ctypes.Structure):
 SYMBOLS = ("igap4h_create", "igap4h_destroy", "igap4h_last_error", "igap4h_sizes", "igap4h_pattern",
            "igap4h_pattern_dev", "igap4h_dirichlet_rows", "igap4h_residual", "igap4h_tangent",
            "igap4h_residual_dev", "igap4h_tangent_dev", "igap4h_set_coef", "igap4h_energy_dev",
-           "igap4h_newton_default_opts", "igap4h_newton_solve_dev", "igap4h_spmv_dev", "igap4h_linear_solve_dev")
+           "igap4h_newton_default_opts", "igap4h_newton_solve_dev", "igap4h_spmv_dev", "igap4h_linear_solve_dev",
+           "igap4h_residual_vjp_coef_dev")
 
 _lib = None
 
@@ -128,6 +129,7 @@ def lib():
         L.igap4h_tangent_dev.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp, vp]
         L.igap4h_set_coef.argtypes = [vp, c_double_p]
         L.igap4h_energy_dev.argtypes = [vp, vp, c_double_p, vp]
+        L.igap4h_residual_vjp_coef_dev.argtypes = [vp, vp, vp, vp, vp]
         L.igap4h_newton_default_opts.argtypes = [ctypes.POINTER(NewtonOpts)]
         L.igap4h_newton_solve_dev.argtypes = [vp, vp, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonInfo), vp]
         L.igap4h_spmv_dev.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp]

@@ -1,0 +1,160 @@
+"""Adjoint gradient for material calibration with the GPU assembly (phys/ge).
+
+The reference has one global par_mat and no derivatives; this module adds a
+calibration loop on top of the C-ABI:
+
+    forward   U(E): Newton solve of R(U; coef(E)) = 0     (igap4h_newton_solve_dev)
+    loss      L = 1/2 sum over control points |U - U*|^2
+    adjoint   K(U) lam = dL/dU = U - U*                   (igap4h_tangent_dev + igap4h_linear_solve_dev)
+    backward  dL/dcoef[e][k] = -lam^T dR/dcoef[e][k]      (igap4h_residual_vjp_coef_dev)
+              dL/dE_e = sum_k dL/dcoef[e][k] dcoef_k/dE
+
+K is the tangent with Dirichlet rows and columns replaced by the identity
+(src/snesiga_Tangent.c:270); the residual's Dirichlet rows U - g do not depend
+on the coefficients, so lam on those rows drops out and K^T = K gives the
+interior adjoint exactly.
+
+    python -m igap4standard_amd.adjoint --n 32     (one JSON line: loss, gradient checks)
+"""
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import _lib as L
+from . import problem as P
+from .snesiga import App
+
+
+class Calibration:
+    """Young's-modulus calibration on a fixed mesh and loading (problem.adjoint_problem)."""
+
+    def __init__(self, pb, nu=0.3, ell=0.1, device="cuda:0", newton_rtol=1e-12, adjoint_rtol=1e-12):
+        import torch
+        self.torch = torch
+        self.dev = torch.device(device)
+        self.pb, self.nu, self.ell = pb, nu, ell
+        self.app = App(pb)
+        self.dcdE = P.dcoef_dyoung(nu)
+        self.U0 = torch.tensor(P.affine_guess(pb), device=self.dev, dtype=torch.float64)
+        self.U = torch.empty_like(self.U0)
+        self.vals = torch.empty(self.app.nnz, device=self.dev, dtype=torch.float64)
+        self.lam = torch.zeros(self.app.ndof, device=self.dev, dtype=torch.float64)
+        self.g = torch.empty(pb.n_elem * 6, device=self.dev, dtype=torch.float64)
+        self.R = torch.empty(self.app.ndof, device=self.dev, dtype=torch.float64)
+        # stop at rtol relative to the first residual or, for warm starts, at an
+        # absolute floor near round-off
+        self.newton_opts = self.app.newton_opts(rtol=newton_rtol, atol=1e-15, dirichlet_lift=1, max_newton=30)
+        self.adjoint_opts = self.app.newton_opts(ksp_rtol=adjoint_rtol, ksp_max_it=5000)
+
+    def set_young(self, E):
+        self.app.set_coef(P.young_to_coef(E, self.nu, self.ell))
+
+    def solve(self, E, warm=None):
+        """U(E) on the device (from the affine guess, or from warm)."""
+        self.set_young(E)
+        self.U.copy_(self.U0 if warm is None else warm)
+        info = self.app.newton_solve_dev(self.U.data_ptr(), self.newton_opts)
+        self.torch.cuda.synchronize()
+        if info["converged"] != 1 and not (info["converged"] in (-1, -3) and info["fnorm"] < 1e-13):
+            raise RuntimeError("forward Newton solve failed: %s" % info)
+        return self.U.clone(), info
+
+    def loss(self, U, Ut):
+        d = (U - Ut).double()
+        return 0.5 * float((d * d).sum())
+
+    def gradient(self, E, U, Ut):
+        """(loss, dL/dE (n_elem,), dL/dcoef (n_elem, 6), adjoint iterations) at the solution U of E."""
+        self.set_young(E)
+        self.app.tangent_dev(U.data_ptr(), self.vals.data_ptr(), L.IGAP4H_FP64, L.IGAP4H_VALS_F64)
+        b = (U - Ut).contiguous()
+        self.lam.zero_()
+        its, rel = self.app.linear_solve_dev(self.vals.data_ptr(), L.IGAP4H_VALS_F64, b.data_ptr(),
+                                             self.lam.data_ptr(), self.adjoint_opts)
+        self.app.residual_vjp_coef_dev(U.data_ptr(), self.lam.data_ptr(), self.g.data_ptr())
+        self.torch.cuda.synchronize()
+        gc = -self.g.cpu().numpy().reshape(-1, 6)
+        return self.loss(U, Ut), gc @ self.dcdE, gc, its
+
+    def vjp_E(self, U, lam, E):
+        """lam^T dR/dE_e for every element (the backward kernel, chained to E)."""
+        self.set_young(E)
+        self.app.residual_vjp_coef_dev(U.data_ptr(), lam.data_ptr(), self.g.data_ptr())
+        self.torch.cuda.synchronize()
+        return self.g.cpu().numpy().reshape(-1, 6) @ self.dcdE
+
+    def vjp_E_fd(self, U, lam, E, elems, h=1e-6):
+        """Central finite differences of lam^T R(U; E) in E_e for e in elems."""
+        out = []
+        for e in elems:
+            vals = []
+            for sgn in (1.0, -1.0):
+                Ep = E.copy()
+                Ep[e] += sgn * h
+                self.set_young(Ep)
+                self.app.residual_dev(U.data_ptr(), self.R.data_ptr())
+                self.torch.cuda.synchronize()
+                vals.append(float((lam * self.R).sum()))
+            out.append((vals[0] - vals[1]) / (2.0 * h))
+        self.set_young(E)
+        return np.array(out)
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=32)
+    ap.add_argument("--nfd", type=int, default=64, help="elements checked by finite differences")
+    ap.add_argument("--h", type=float, default=1e-6, help="central FD step in E_e")
+    ap.add_argument("--loss-fd", type=int, default=0, help="elements also checked by FD of the full loss")
+    ap.add_argument("--loss-h", type=float, default=1e-3)
+    args = ap.parse_args(argv)
+
+    def say(*a):
+        print("adjoint:", *a, file=sys.stderr, flush=True)
+
+    pb, E, Et = P.adjoint_problem(args.n)
+    cal = Calibration(pb)
+    t0 = time.perf_counter()
+    Ut, it_t = cal.solve(Et)
+    say("target solve", it_t["newton_its"], "steps, |R|/|R0| = %.2e" % (it_t["fnorm"] / it_t["fnorm0"]))
+    U, it_f = cal.solve(E)
+    t_fwd = time.perf_counter() - t0
+    say("forward solve", it_f["newton_its"], "steps, |R|/|R0| = %.2e" % (it_f["fnorm"] / it_f["fnorm0"]))
+    t0 = time.perf_counter()
+    loss, gE, gc, its = cal.gradient(E, U, Ut)
+    t_adj = time.perf_counter() - t0
+    say("loss %.6e, adjoint GMRES its %d, %.3f s" % (loss, its, t_adj))
+    rng = np.random.default_rng(0x5EED0007)
+    elems = rng.choice(pb.n_elem, size=min(args.nfd, pb.n_elem), replace=False)
+    # backward kernel vs central FD of lam^T R in E_e (step h)
+    v_adj = cal.vjp_E(U, cal.lam, E)[elems]
+    v_fd = cal.vjp_E_fd(U, cal.lam, E, elems, args.h)
+    rel_vjp = float(np.linalg.norm(v_adj - v_fd) / np.linalg.norm(v_fd))
+    say("VJP vs FD on %d elements: rel err %.2e (max elementwise %.2e)" % (
+        len(elems), rel_vjp, float(np.max(np.abs(v_adj - v_fd) / np.abs(v_fd)))))
+    out = {"n": args.n, "nelem": pb.n_elem, "ndof": pb.ndof, "loss": loss, "forward_s": t_fwd, "adjoint_s": t_adj,
+           "adjoint_its": its, "vjp_fd": {"elements": len(elems), "step": args.h, "rel_err": rel_vjp},
+           "grad_norm": float(np.linalg.norm(gE))}
+    if args.loss_fd:
+        chk = elems[:args.loss_fd]
+        fd = []
+        for e in chk:
+            lv = []
+            for sgn in (1.0, -1.0):
+                Ep = E.copy()
+                Ep[e] += sgn * args.loss_h
+                Up, _ = cal.solve(Ep, warm=U)
+                lv.append(cal.loss(Up, Ut))
+            fd.append((lv[0] - lv[1]) / (2.0 * args.loss_h))
+        fd = np.array(fd)
+        rel = float(np.linalg.norm(gE[chk] - fd) / np.linalg.norm(fd))
+        say("dL/dE vs FD of the loss on %d elements (step %g): rel err %.2e" % (len(chk), args.loss_h, rel))
+        out["loss_fd"] = {"elements": len(chk), "step": args.loss_h, "rel_err": rel}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

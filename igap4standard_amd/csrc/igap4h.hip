@@ -833,6 +833,66 @@ __global__ __launch_bounds__(64) void k_energy_elem(MeshArgs m, const double *__
     }
 }
 
+// ============================================================== adjoint (ge)
+// g[e][k] = lam^T dR/dcoef[e][k] = sum_q w_q d/de phi_k(u + e lam_q) at e = 0,
+// lam interpolated like u (no F0 term) and zero on the Dirichlet rows, whose
+// residual U - g does not depend on the coefficients.  W is linear in the
+// coefficients, so this is exact.  Basis values are recomputed per element.
+__global__ __launch_bounds__(64) void k_vjp_coef(MeshArgs m, const double *__restrict__ U, const double *__restrict__ lam,
+                                                 const unsigned char *__restrict__ dmask, double *__restrict__ g)
+{
+    __shared__ double sU[81], sL[81];
+    __shared__ double sphi[3 * 4 * 9];
+    __shared__ double sred[6][64];
+    const int tid = threadIdx.x;
+    const long long e = blockIdx.x;
+    int ev[3];
+    ev[2] = (int)(e % m.ne[2]);
+    ev[1] = (int)((e / m.ne[2]) % m.ne[1]);
+    ev[0] = (int)(e / ((long long)m.ne[2] * m.ne[1]));
+    for (int idx = tid; idx < 81; idx += 64) {
+        int A = idx / 3, i = idx % 3;
+        long long r = 3 * node_of(m, ev[0] + A / 9, ev[1] + (A / 3) % 3, ev[2] + A % 3) + i;
+        sU[idx] = U[r];
+        sL[idx] = dmask[r] ? 0.0 : lam[r];
+    }
+    fill_phi(m, ev, sphi, tid, 64);
+    __syncthreads();
+    const int nqp = m.nq * m.nq * m.nq;
+    double val[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (tid < nqp) {
+        int qx = tid / (m.nq * m.nq), qy = (tid / m.nq) % m.nq, qz = tid % m.nq;
+        double u[1][30], w;
+        interp_slots<1>(m, sU, sphi, ev, qx, qy, qz, u, &w);
+        double l[30];
+        for (int s = 0; s < 30; s++) l[s] = 0.0;
+        for (int A = 0; A < 27; A++) {
+            double n[10];
+            basis_slots(sphi, qx, qy, qz, A, n);
+            for (int i = 0; i < 3; i++) {
+                const double Lv = sL[3 * A + i];
+#pragma unroll
+                for (int s = 0; s < 10; s++) l[10 * i + s] += n[s] * Lv;
+            }
+        }
+        Jet<2> uj[30];
+        for (int s = 0; s < 30; s++) {
+            uj[s].c[0] = u[0][s];
+            uj[s].c[1] = l[s];
+        }
+        Jet<2> phi[6];
+        energy_basis<Jet<2>>(uj, phi);
+        for (int k = 0; k < 6; k++) val[k] = w * phi[k].c[1];
+    }
+    for (int k = 0; k < 6; k++) sred[k][tid] = val[k];
+    __syncthreads();
+    if (tid < 6) {
+        double s = 0.0;
+        for (int l = 0; l < nqp; l++) s += sred[tid][l];
+        g[6 * e + tid] = s;
+    }
+}
+
 // ============================================================== context
 
 static MeshArgs mesh_args(const igap4h_ctx *c)
@@ -1238,6 +1298,19 @@ extern "C" int igap4h_tangent(igap4h_ctx *c, const double *U, double *U_hist, in
     rc = igap4h_tangent_dev(c, c->d_Uh, prec, vals_type, c->d_vals, nullptr);
     if (rc) return rc;
     HIPCHK(hipMemcpy(vals, c->d_vals, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int igap4h_residual_vjp_coef_dev(igap4h_ctx *c, const double *d_U, const double *d_lambda, double *d_g,
+                                            void *stream)
+{
+    if (!c || !d_U || !d_lambda || !d_g) return fail(IGAP4H_EINVAL, "NULL argument");
+    if (c->physics != IGAP4H_PHYS_GE) return fail(IGAP4H_EINVAL, "the coefficient VJP is implemented for phys/ge only");
+    long long ne = c->mesh.nelem();
+    MeshArgs m = mesh_args(c);
+    hipLaunchKernelGGL(k_vjp_coef, dim3((unsigned)ne), dim3(64), 0, (hipStream_t)stream, m, d_U, d_lambda,
+                       (const unsigned char *)c->d_mask, d_g);
+    HIPCHK(hipGetLastError());
     return 0;
 }
 

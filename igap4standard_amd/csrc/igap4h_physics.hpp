@@ -263,6 +263,26 @@ IGAP4H_HD T energy_of(const T *u, const Coef &c)
     return W;
 }
 
+// The six energies W is linear in: W = sum_k coef_k phi_k with coef =
+// (A36, A37, A38, e0, e345, le^2) and
+// phi = (h3 (h3^2 - 3 h2^2), q, q^2, h1^2, E01^2 + E02^2 + E12^2, |dev g|^2).
+template <class T>
+IGAP4H_HD void energy_basis(const T *u, T phi[6])
+{
+    Coef one{0.0, 0.0, 0.0, 0.0, 0.0, 1.0};  // lsq = 1: tau = 2 dev g
+    Strain<T> st;
+    strain_of(u, one, st);
+    phi[0] = st.h3 * (st.h3 * st.h3 - st.h2 * st.h2 * 3.0);
+    phi[1] = st.q;
+    phi[2] = st.q * st.q;
+    phi[3] = st.h1 * st.h1;
+    phi[4] = st.eps[3] * st.eps[3] + st.eps[4] * st.eps[4] + st.eps[5] * st.eps[5];
+    T g2 = Num<T>::cst(0.0);
+    for (int I = 0; I < 3; I++)
+        for (int K = 0; K < 3; K++) g2 = g2 + st.tau[I][K] * st.tau[I][K];
+    phi[5] = g2 * 0.25;
+}
+
 template <class T>
 IGAP4H_HD void compact_of(const T *u, const Coef &c, Compact<T> &cp)
 {

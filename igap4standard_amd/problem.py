@@ -181,6 +181,103 @@ def stretch_problem(n, a=0.01, lam=1.0, mu=0.5, ell=0.1, noise=0.0, seed=0x5EED0
     return pb
 
 
+def gaussian01(seed, n):
+    """n standard normal draws (Box-Muller on splitmix64 uniforms)."""
+    u = uniform01(seed, 2 * n).reshape(n, 2)
+    u1 = np.maximum(u[:, 0], 1.0 / (1 << 53))
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u[:, 1])
+
+
+def gaussian_random_field(coords, corr_len, seed, std=1.0):
+    """Zero-mean Gaussian random field on the tensor grid coords (3 1D
+    arrays): white noise (gaussian01) smoothed by a separable Gaussian kernel
+    exp(-d^2 / corr_len^2) per direction, so the covariance is proportional
+    to exp(-d^2 / (2 corr_len^2)); then shifted and scaled to sample mean 0
+    and sample standard deviation std."""
+    shape = tuple(len(c) for c in coords)
+    f = gaussian01(seed, int(np.prod(shape))).reshape(shape)
+    for d, c in enumerate(coords):
+        c = np.asarray(c, float)
+        Kd = np.exp(-((c[:, None] - c[None, :]) / corr_len) ** 2)
+        f = np.moveaxis(np.tensordot(Kd, np.moveaxis(f, d, 0), axes=(1, 0)), 0, d)
+    f = f - f.mean()
+    return f * (std / f.std())
+
+
+def element_centres(pb):
+    return [0.5 * (k[2:-3] + k[3:-2]) for k in pb.knots]
+
+
+def young_to_coef(E, nu, ell):
+    """Per-element coefficients of the Lame mapping for Young's modulus E and
+    Poisson ratio nu: lam = E nu / ((1 + nu)(1 - 2 nu)), mu = E / (2 (1 + nu))."""
+    E = np.asarray(E, float)
+    lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    mu = E / (2.0 * (1.0 + nu))
+    return coef_from_lame(lam, mu, np.full(E.shape, ell))
+
+
+def dcoef_dyoung(nu):
+    """d(coef)/dE of young_to_coef (constant: the mapping is linear in E)."""
+    dl = nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    dm = 1.0 / (2.0 * (1.0 + nu))
+    return np.array([0.0, dm, 0.0, 1.5 * dl + dm, 2.0 * dm, 0.0])
+
+
+def lognormal_young(pb, sigma=0.3, corr_len=0.25, seed=0x5EED0005):
+    """E_e = exp(-sigma^2/2 + sigma Z_e) (mean 1), Z a unit Gaussian random
+    field at the element centres (elements in reference order)."""
+    Z = gaussian_random_field(element_centres(pb), corr_len, seed).reshape(-1)
+    return np.exp(-0.5 * sigma * sigma + sigma * Z)
+
+
+def adjoint_problem(n=32, sigma=0.3, corr_len=0.25, nu=0.3, ell=0.1, a=0.01, seed=0x5EED0005,
+                    target_seed=0x5EED0006, nquad=4):
+    """Material calibration problem on n^3 elements: clamp/stretch loading,
+    per-element Young's modulus from a log-normal field (seed), Poisson ratio
+    nu, length scale ell.  Returns (pb, E, E_target); the target displacement
+    is the solution for E_target (second field, target_seed)."""
+    pb = stretch_problem(n, a=a, nquad=nquad)
+    E = lognormal_young(pb, sigma, corr_len, seed)
+    Et = lognormal_young(pb, sigma, corr_len, target_seed)
+    pb.elem_coef = young_to_coef(E, nu, ell)
+    return pb, E, Et
+
+
+def transient_problem(n=48, rho=1.0, damping=0.0, dt=1e-3, lam=1.0, mu=0.5, ell=0.05, nquad=4):
+    """phys/getime on n^3 elements, all faces clamped (standard and high-order
+    Dirichlet zero), uniform Lame coefficients; par_mat[5..7] = (rho, c, dt)
+    as the getime kernels read them (phys/getime/residual.c:4127)."""
+    pb = reference_example(n, physics=GETIME, nquad=nquad)
+    pb.bc_type = bc_all_clamped()
+    pb.par_mat = np.array(pb.par_mat, dtype=np.float64)
+    pb.par_mat[5], pb.par_mat[6], pb.par_mat[7] = rho, damping, dt
+    pb.elem_coef = coef_from_lame(np.full(pb.n_elem, lam), np.full(pb.n_elem, mu), np.full(pb.n_elem, ell))
+    return pb
+
+
+def clamped_mask(pb, layers=2):
+    """True on the control points of the first/last `layers` layers of every
+    face (the rows an all-faces clamp fixes), per dof."""
+    nb = pb.nbasis
+    m = np.zeros(tuple(nb), bool)
+    m[:layers], m[-layers:] = True, True
+    m[:, :layers], m[:, -layers:] = True, True
+    m[:, :, :layers], m[:, :, -layers:] = True, True
+    return np.repeat(m.reshape(-1), 3)
+
+
+def initial_velocity(pb, std=1e-3, corr_len=0.2, seed=0x5EED0003):
+    """Velocity at the control points: one Gaussian random field per
+    component (seeds seed, seed+1, seed+2; zero mean, standard deviation std,
+    correlation length corr_len), zero on the clamped layers."""
+    g = pb.greville()
+    comps = [gaussian_random_field(g, corr_len, seed + i, std=std).reshape(-1) for i in range(3)]
+    v = np.stack(comps, axis=-1).reshape(-1)
+    v[clamped_mask(pb)] = 0.0
+    return v
+
+
 def affine_guess(pb, nhist=None):
     """Control values of the affine field u_i = sum_J a_iJ X_J with
     a_iJ = par_dirichlet[3 i + J] (the field src/iga_dirichlet.c:42-44

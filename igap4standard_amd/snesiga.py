@@ -202,6 +202,11 @@ class App:
                                                 ctypes.byref(rel), ctypes.c_void_p(stream or 0)))
         return its.value, rel.value
 
+    def residual_vjp_coef_dev(self, d_U, d_lam, d_g, stream=None):
+        """d_g (n_elem x 6) = lambda^T dR/dcoef at U (phys/ge)."""
+        L.check(L.lib().igap4h_residual_vjp_coef_dev(self.ctx, ctypes.c_void_p(d_U), ctypes.c_void_p(d_lam),
+                                                     ctypes.c_void_p(d_g), ctypes.c_void_p(stream or 0)))
+
     def energy_dev(self, d_U, stream=None):
         e = ctypes.c_double()
         L.check(L.lib().igap4h_energy_dev(self.ctx, ctypes.c_void_p(d_U), ctypes.byref(e),

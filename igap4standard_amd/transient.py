@@ -1,0 +1,103 @@
+"""Implicit time stepping of phys/getime on the GPU.
+
+The reference's driver (example/ex1/main.c:46-60) solves one SNES problem per
+step and then shifts the history, U_hist[h] = U_hist[h-1] for h = torder..1
+(:55), starting each solve from the previous step (initguess, :116-124).
+Here every step is one igap4h_newton_solve_dev on the device history
+U_hist = [U_{n+1}, U_n, U_{n-1}] and the shift is two device copies.
+
+    python -m igap4standard_amd.transient --n 48 --steps 100   (one JSON line)
+"""
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import _lib as L
+from . import problem as P
+from .snesiga import App
+
+
+class TimeStepper:
+    def __init__(self, pb, mixed=True, device="cuda:0", **newton_kw):
+        import torch
+        if pb.physics != P.GETIME:
+            raise ValueError("TimeStepper needs a phys/getime problem")
+        self.torch = torch
+        self.dev = torch.device(device)
+        self.pb = pb
+        self.app = App(pb)
+        n = pb.ndof
+        self.n = n
+        self.Uh = torch.zeros(3 * n, device=self.dev, dtype=torch.float64)
+        kw = dict(atol=1e-10, rtol=0.0)   # the reference's absolute tolerance (src/snes_init.c:18)
+        if mixed:
+            kw.update(tangent_prec=L.IGAP4H_FP32, vals_type=L.IGAP4H_VALS_F32)
+        kw.update(newton_kw)
+        self.opts = self.app.newton_opts(**kw)
+
+    def set_state(self, U_n, U_nm1):
+        """History: slice 1 = U_n, slice 2 = U_{n-1}; slice 0 (the guess) = U_n."""
+        t = self.torch
+        self.Uh[self.n:2 * self.n] = t.as_tensor(U_n, device=self.dev)
+        self.Uh[2 * self.n:] = t.as_tensor(U_nm1, device=self.dev)
+        self.Uh[:self.n] = self.Uh[self.n:2 * self.n]
+
+    def step(self):
+        info = self.app.newton_solve_dev(self.Uh.data_ptr(), self.opts)
+        n = self.n
+        self.Uh[2 * n:].copy_(self.Uh[n:2 * n])   # U_{n-1} <- U_n
+        self.Uh[n:2 * n].copy_(self.Uh[:n])       # U_n <- U_{n+1}; slice 0 stays as the next guess
+        return info
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=48)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--dt", type=float, default=1e-3)
+    ap.add_argument("--fp64", action="store_true", help="FP64 tangent (default: FP32-MFMA tangent, FP32 values)")
+    args = ap.parse_args(argv)
+    import torch
+
+    def say(*a):
+        print("transient:", *a, file=sys.stderr, flush=True)
+
+    pb = P.transient_problem(args.n, dt=args.dt)
+    v0 = P.initial_velocity(pb)
+    ts = TimeStepper(pb, mixed=not args.fp64)
+    ts.set_state(np.zeros(pb.ndof), -args.dt * v0)
+    say("%d^3 getime, %d dofs, dt %g, %s tangent" % (args.n, pb.ndof, args.dt, "FP64" if args.fp64 else "FP32"))
+    infos, norms = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        info = ts.step()
+        infos.append(info)
+        if info["converged"] != 1:
+            say("step %d did not converge: %s" % (k, info))
+            break
+        if k % 10 == 0 or k == args.steps - 1:
+            u = float(torch.linalg.vector_norm(ts.Uh[:ts.n]))
+            norms.append((k + 1, u))
+            say("step %d: %d Newton, %d Krylov, |R| %.2e, |U| %.4e, %.1f s" % (
+                k + 1, info["newton_its"], info["ksp_its"], info["fnorm"], u, time.perf_counter() - t0))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    done = len(infos)
+    out = {"n": args.n, "ndof": pb.ndof, "nelem": pb.n_elem, "dt": args.dt, "steps": done,
+           "tangent": "fp64" if args.fp64 else "fp32-mfma (fp32 values)", "total_s": wall,
+           "ms_per_step": 1000.0 * wall / max(done, 1),
+           "newton_its_mean": float(np.mean([i["newton_its"] for i in infos])),
+           "ksp_its_mean": float(np.mean([i["ksp_its"] for i in infos])),
+           "max_final_fnorm": float(max(i["fnorm"] for i in infos)),
+           "all_converged": all(i["converged"] == 1 for i in infos),
+           "elements_per_s_per_newton_step": pb.n_elem * sum(i["newton_its"] for i in infos) / wall,
+           "u_norm_history": norms}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

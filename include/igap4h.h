@@ -201,6 +201,16 @@ int igap4h_set_coef(igap4h_ctx *ctx, const double *elem_coef);
  * volume part, phys/ge density hh[0]); ge only.  d_U: device ndof.        */
 int igap4h_energy_dev(igap4h_ctx *ctx, const double *d_U, double *energy, void *stream);
 
+/* ---- adjoint: forward/backward pair for the material coefficients ------ */
+/* Forward: igap4h_residual_dev with the coefficients of igap4h_set_coef.
+ * Backward (phys/ge): d_g[6e + k] = lambda^T dR/dcoef[e][k] for every element
+ * e and coefficient k of (A36, A37, A38, e0, e345, le^2), at U (device, ndof,
+ * slice 0 of U_hist).  Exact (W is linear in the coefficients); Dirichlet
+ * rows of R do not depend on them, so lambda is ignored there.  No reference
+ * counterpart: the reference has one par_mat and no derivatives.           */
+int igap4h_residual_vjp_coef_dev(igap4h_ctx *ctx, const double *d_U, const double *d_lambda, double *d_g,
+                                 void *stream);
+
 /* ---- Newton-Krylov driver ------------------------------------------- */
 /* The reference leaves the solve to PETSc (src/snes_init.c:8-47,
  * src/snes_solve.c:10-36: NEWTONLS + BT line search, GMRES with right

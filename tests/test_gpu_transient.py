@@ -1,0 +1,46 @@
+"""Implicit getime time stepping on the GPU (igap4standard_amd/transient.py):
+every step's solution satisfies the oracle's getime residual with the same
+history to the reference's absolute tolerance 1e-10 (src/snes_init.c:18), the
+history shift follows example/ex1/main.c:55, and the FP32-MFMA-tangent form
+follows the FP64 trajectory (only the Jacobian differs)."""
+import sys
+
+import numpy as np
+import pytest
+
+from igap4standard_amd import problem as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_time_steps_match_oracle_residual(oracle):
+    from igap4standard_amd.transient import TimeStepper
+    say("transient 6^3")
+    pb = P.transient_problem(6, dt=1e-3)
+    v0 = P.initial_velocity(pb, std=1e-3, corr_len=0.3)
+    n = pb.ndof
+    traj = {}
+    for mixed in (False, True):
+        ts = TimeStepper(pb, mixed=mixed)
+        ts.set_state(np.zeros(n), -1e-3 * v0)
+        us = []
+        for k in range(3):
+            before = ts.Uh.cpu().numpy()
+            info = ts.step()
+            after = ts.Uh.cpu().numpy()
+            assert info["converged"] == 1, info
+            U_new = after[:n]
+            assert np.array_equal(after[n:2 * n], U_new) and np.array_equal(after[2 * n:], before[n:2 * n])
+            R = oracle.residual(pb, np.concatenate([U_new, before[n:2 * n], before[2 * n:]]))
+            say("  mixed", mixed, "step", k, "newton", info["newton_its"], "|R| oracle %.2e" % np.linalg.norm(R))
+            assert np.linalg.norm(R) < 1e-10
+            us.append(U_new)
+        traj[mixed] = np.array(us)
+    rel = np.linalg.norm(traj[True] - traj[False]) / np.linalg.norm(traj[False])
+    assert rel < 1e-8, rel
+    assert np.linalg.norm(traj[False][-1]) > 0.0
