@@ -93,7 +93,8 @@ SYMBOLS = ("igap4h_create", "igap4h_destroy", "igap4h_last_error", "igap4h_sizes
            "igap4h_pattern_dev", "igap4h_dirichlet_rows", "igap4h_residual", "igap4h_tangent",
            "igap4h_residual_dev", "igap4h_tangent_dev", "igap4h_set_coef", "igap4h_energy_dev",
            "igap4h_newton_default_opts", "igap4h_newton_solve_dev", "igap4h_spmv_dev", "igap4h_linear_solve_dev",
-           "igap4h_residual_vjp_coef_dev")
+           "igap4h_residual_vjp_coef_dev", "igap4h_batch_create", "igap4h_batch_destroy", "igap4h_batch_sizes",
+           "igap4h_batch_tune", "igap4h_batch_set_variant", "igap4h_batch_assemble")
 
 _lib = None
 
@@ -130,13 +131,21 @@ def lib():
         L.igap4h_set_coef.argtypes = [vp, c_double_p]
         L.igap4h_energy_dev.argtypes = [vp, vp, c_double_p, vp]
         L.igap4h_residual_vjp_coef_dev.argtypes = [vp, vp, vp, vp, vp]
+        L.igap4h_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                          ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+        L.igap4h_batch_destroy.argtypes = [vp]
+        L.igap4h_batch_destroy.restype = None
+        L.igap4h_batch_sizes.argtypes = [vp, c_i64_p, c_int_p]
+        L.igap4h_batch_tune.argtypes = [vp, ctypes.c_int, c_int_p, c_double_p, c_int_p]
+        L.igap4h_batch_set_variant.argtypes = [vp, ctypes.c_int]
+        L.igap4h_batch_assemble.argtypes = [vp, vp]
         L.igap4h_newton_default_opts.argtypes = [ctypes.POINTER(NewtonOpts)]
         L.igap4h_newton_solve_dev.argtypes = [vp, vp, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonInfo), vp]
         L.igap4h_spmv_dev.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp]
         L.igap4h_linear_solve_dev.argtypes = [vp, vp, ctypes.c_int, vp, vp, ctypes.POINTER(NewtonOpts), c_int_p,
                                               c_double_p, vp]
         for name in SYMBOLS:
-            if name not in ("igap4h_destroy", "igap4h_last_error"):
+            if name not in ("igap4h_destroy", "igap4h_last_error", "igap4h_batch_destroy"):
                 getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib

@@ -180,15 +180,14 @@ __device__ __forceinline__ Coef load_coef(const double *coef, long long e)
 // One wave per element, lane = Gauss point.  Writes the element vector
 // R_e[81] (snesiga_Residual.c:104-160) to Re; k_gather sums it per dof.
 template <int PK>
-__global__ __launch_bounds__(64) void k_residual_elem(MeshArgs m, const double *__restrict__ Uh,
-                                                      const double *__restrict__ coef, double *__restrict__ Re)
+__device__ __forceinline__ void residual_elem_body(const MeshArgs &m, const double *__restrict__ Uh,
+                                                   const double *__restrict__ coef, double *__restrict__ Re, long long e)
 {
     constexpr int NH = PhysT<PK>::NH;
     __shared__ double sU[NH * 81];
     __shared__ double sphi[3 * 4 * 9];
     __shared__ double sred[64 * 81];
     const int tid = threadIdx.x;
-    const long long e = blockIdx.x;
     int ev[3];
     ev[2] = (int)(e % m.ne[2]);
     ev[1] = (int)((e / m.ne[2]) % m.ne[1]);
@@ -247,15 +246,20 @@ __global__ __launch_bounds__(64) void k_residual_elem(MeshArgs m, const double *
     }
 }
 
+template <int PK>
+__global__ __launch_bounds__(64) void k_residual_elem(MeshArgs m, const double *__restrict__ Uh,
+                                                      const double *__restrict__ coef, double *__restrict__ Re)
+{
+    residual_elem_body<PK>(m, Uh, coef, Re, (long long)blockIdx.x);
+}
+
 // Sum of element vectors per dof in element order (the reference's ADD order),
 // Neumann load, then Dirichlet rows R = U - g (INSERT, snesiga_Residual.c:382).
-__global__ __launch_bounds__(256) void k_gather(MeshArgs m, const double *__restrict__ Re,
-                                                const double *__restrict__ fneu, const unsigned char *__restrict__ dmask,
-                                                const double *__restrict__ gval, const double *__restrict__ U0,
-                                                double *__restrict__ R)
+__device__ __forceinline__ void gather_body(const MeshArgs &m, const double *__restrict__ Re,
+                                            const double *__restrict__ fneu, const unsigned char *__restrict__ dmask,
+                                            const double *__restrict__ gval, const double *__restrict__ U0,
+                                            double *__restrict__ R, long long dof)
 {
-    long long dof = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (dof >= m.ndof) return;
     long long node = dof / 3;
     int i = (int)(dof % 3);
     int a2 = (int)(node % m.nb[2]);
@@ -272,6 +276,16 @@ __global__ __launch_bounds__(256) void k_gather(MeshArgs m, const double *__rest
     if (fneu) s += fneu[dof];
     if (dmask[dof]) s = U0[dof] - gval[dof];
     R[dof] = s;
+}
+
+__global__ __launch_bounds__(256) void k_gather(MeshArgs m, const double *__restrict__ Re,
+                                                const double *__restrict__ fneu, const unsigned char *__restrict__ dmask,
+                                                const double *__restrict__ gval, const double *__restrict__ U0,
+                                                double *__restrict__ R)
+{
+    long long dof = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (dof >= m.ndof) return;
+    gather_body(m, Re, fneu, dmask, gval, U0, R, dof);
 }
 
 // ============================================================== tangent
@@ -554,11 +568,9 @@ __device__ __forceinline__ void ge_task(const double *sCmp, const double *sphi, 
 }
 
 template <class ACC, int QC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_tangent_ge(MeshArgs m, const double *__restrict__ Uh,
-                                                    const double *__restrict__ coef,
-                                                    const long long *__restrict__ rowptr,
-                                                    const unsigned char *__restrict__ dmask, void *__restrict__ vals,
-                                                    int vals_f32, int cx, int cy, int cz, int my, int mz)
+__device__ __forceinline__ void tangent_ge_body(const MeshArgs &m, const double *__restrict__ Uh, const double *__restrict__ coef,
+                                                const long long *__restrict__ rowptr, const unsigned char *__restrict__ dmask,
+                                                void *__restrict__ vals, int vals_f32, const int ev[3])
 {
     using G = GeTile<QC>;
     using V = typename V4<ACC>::t;
@@ -580,11 +592,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     ACC *sT = sNk + G::NKA;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int b = blockIdx.x;
-    int ev[3];
-    ev[2] = cz + 3 * (b % mz);
-    ev[1] = cy + 3 * ((b / mz) % my);
-    ev[0] = cx + 3 * (b / (mz * my));
     const long long e = ((long long)ev[0] * m.ne[1] + ev[1]) * m.ne[2] + ev[2];
     const int nq = m.nq, nqp = nq * nq * nq;
 
@@ -715,6 +722,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         scatter_rows<float, ACC, 256>(sKe, sRowBase, sRowSx, sRowSy, sMask, (float *)vals, tid);
     else
         scatter_rows<double, ACC, 256>(sKe, sRowBase, sRowSx, sRowSy, sMask, (double *)vals, tid);
+}
+
+template <class ACC, int QC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_tangent_ge(MeshArgs m, const double *__restrict__ Uh,
+                                                    const double *__restrict__ coef,
+                                                    const long long *__restrict__ rowptr,
+                                                    const unsigned char *__restrict__ dmask, void *__restrict__ vals,
+                                                    int vals_f32, int cx, int cy, int cz, int my, int mz)
+{
+    const int b = blockIdx.x;
+    const int ev[3] = {cx + 3 * (b / (mz * my)), cy + 3 * ((b / mz) % my), cz + 3 * (b % mz)};
+    tangent_ge_body<ACC, QC>(m, Uh, coef, rowptr, dmask, vals, vals_f32, ev);
 }
 
 // diagonal of Dirichlet rows = 1 (MatZeroRowsColumns(..., 1.0, ...))
@@ -1333,5 +1352,347 @@ extern "C" int igap4h_energy_dev(igap4h_ctx *c, const double *d_U, double *energ
     double s = 0.0;
     for (double v : h) s += v;
     *energy = s;
+    return 0;
+}
+
+// ============================================================== batched assembly
+// Many independent phys/ge instances (own mesh sizes, coefficients, U, R and
+// CSR values) assembled together: one residual launch over all elements, one
+// gather launch, then per colour ONE tangent launch over that colour's
+// elements of every instance (instances never share a basis function), the
+// whole sequence captured once into a HIP graph and replayed per assembly.
+struct BatchInst {
+    MeshArgs m;
+    const double *Uh;
+    const double *coef;
+    const long long *rowptr;
+    const unsigned char *mask;
+    const double *g;
+    const double *fneu;
+    const int *drows;
+    int ndrows;
+    double *Re;
+    double *R;
+    void *vals;
+    long long nnz;
+};
+
+// largest i in [0, n) with off[i] <= x (off ascending, off[0] = 0)
+__device__ __forceinline__ int batch_find(const long long *off, int n, long long x)
+{
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (off[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(64) void k_residual_batch(const BatchInst *__restrict__ I, const long long *__restrict__ eoff, int n)
+{
+    const int i = batch_find(eoff, n, blockIdx.x);
+    const BatchInst &b = I[i];
+    const MeshArgs &m = b.m;
+    residual_elem_body<0>(m, b.Uh, b.coef, b.Re, (long long)blockIdx.x - eoff[i]);
+}
+
+// blockIdx.y = instance; grid-stride over its dofs
+__global__ __launch_bounds__(256) void k_gather_batch(const BatchInst *__restrict__ I)
+{
+    const BatchInst &b = I[blockIdx.y];
+    const MeshArgs &m = b.m;
+    for (long long dof = (long long)blockIdx.x * blockDim.x + threadIdx.x; dof < m.ndof; dof += (long long)gridDim.x * blockDim.x)
+        gather_body(m, b.Re, b.fneu, b.mask, b.g, b.Uh, b.R, dof);
+}
+
+__global__ void k_fill_zero_batch(const BatchInst *__restrict__ I, int vals_f32)
+{
+    const BatchInst &b = I[blockIdx.y];
+    const long long n32 = vals_f32 ? b.nnz : 2 * b.nnz;
+    unsigned int *p = (unsigned int *)b.vals;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n32; k += (long long)gridDim.x * blockDim.x) p[k] = 0u;
+}
+
+__global__ void k_dirichlet_diag_batch(const BatchInst *__restrict__ I, int vals_f32)
+{
+    const BatchInst &b = I[blockIdx.y];
+    const MeshArgs &m = b.m;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < b.ndrows; t += gridDim.x * blockDim.x) {
+        long long r = b.drows[t];
+        long long node = r / 3;
+        int i = (int)(r % 3);
+        int a2 = (int)(node % m.nb[2]);
+        int a1 = (int)((node / m.nb[2]) % m.nb[1]);
+        int a0 = (int)(node / ((long long)m.nb[2] * m.nb[1]));
+        int lo0 = max(0, a0 - 2), lo1 = max(0, a1 - 2), lo2 = max(0, a2 - 2);
+        int w1 = min(m.nb[1] - 1, a1 + 2) - lo1 + 1, w2 = min(m.nb[2] - 1, a2 + 2) - lo2 + 1;
+        long long pos = b.rowptr[r] + 3LL * (((a0 - lo0) * w1 + (a1 - lo1)) * w2 + (a2 - lo2)) + i;
+        if (vals_f32) ((float *)b.vals)[pos] = 1.0f;
+        else ((double *)b.vals)[pos] = 1.0;
+    }
+}
+
+// colour (cx, cy, cz) of every instance; coff = this colour's element offsets
+template <class ACC, int QC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_tangent_ge_batch(
+    const BatchInst *__restrict__ I, const long long *__restrict__ coff, int n, int vals_f32, int cx, int cy, int cz)
+{
+    const int i = batch_find(coff, n, blockIdx.x);
+    const BatchInst &b = I[i];
+    const MeshArgs &m = b.m;
+    const long long t = (long long)blockIdx.x - coff[i];
+    const int my = (m.ne[1] - cy + 2) / 3, mz = (m.ne[2] - cz + 2) / 3;
+    const int ev[3] = {cx + 3 * (int)(t / ((long long)mz * my)), cy + 3 * (int)((t / mz) % my), cz + 3 * (int)(t % mz)};
+    tangent_ge_body<ACC, QC>(m, b.Uh, b.coef, b.rowptr, b.mask, b.vals, vals_f32, ev);
+}
+
+// the tuning space: Gauss points per MFMA chunk (contraction depth 9 QC per pass)
+static const int kBatchQC[2][3] = {{1, 2, 3}, {2, 3, 4}};  // [prec FP64, FP32]
+
+struct igap4h_batch {
+    int n = 0, prec = 0, vals_f32 = 0, device = 0;
+    std::vector<BatchInst> h_inst;
+    BatchInst *d_inst = nullptr;
+    long long *d_off = nullptr;             // [eoff (n+1) | coff 27 x (n+1)]
+    std::vector<long long> eoff, coff;      // host copies
+    long long maxdof = 0, maxnnz = 0;
+    int maxnd = 0;
+    int qc = 0;                             // chosen variant
+    hipStream_t cap = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+template <class ACC, int QC>
+static int batch_tangent_launches(igap4h_batch *b, hipStream_t st)
+{
+    for (int c = 0; c < 27; c++) {
+        const long long *off = b->coff.data() + (size_t)c * (b->n + 1);
+        long long nb = off[b->n];
+        if (nb <= 0) continue;
+        hipLaunchKernelGGL((k_tangent_ge_batch<ACC, QC>), dim3((unsigned)nb), dim3(256), 0, st, (const BatchInst *)b->d_inst,
+                           (const long long *)(b->d_off + (b->n + 1) * (1 + c)), b->n, b->vals_f32, c / 9, (c / 3) % 3, c % 3);
+        HIPCHK(hipGetLastError());
+    }
+    return 0;
+}
+
+static int batch_tangent(igap4h_batch *b, int qc, hipStream_t st)
+{
+    if (b->prec == IGAP4H_FP32) {
+        if (qc == 2) return batch_tangent_launches<float, 2>(b, st);
+        if (qc == 3) return batch_tangent_launches<float, 3>(b, st);
+        if (qc == 4) return batch_tangent_launches<float, 4>(b, st);
+    } else {
+        if (qc == 1) return batch_tangent_launches<double, 1>(b, st);
+        if (qc == 2) return batch_tangent_launches<double, 2>(b, st);
+        if (qc == 3) return batch_tangent_launches<double, 3>(b, st);
+    }
+    return fail(IGAP4H_EINVAL, "no batched tangent variant with %d Gauss points per chunk for this precision", qc);
+}
+
+static int batch_sequence(igap4h_batch *b, hipStream_t st)
+{
+    const unsigned n = (unsigned)b->n;
+    hipLaunchKernelGGL(k_fill_zero_batch, dim3((unsigned)std::min<long long>(1024, (b->maxnnz + 255) / 256), n), dim3(256), 0, st,
+                       (const BatchInst *)b->d_inst, b->vals_f32);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_residual_batch, dim3((unsigned)b->eoff[b->n]), dim3(64), 0, st, (const BatchInst *)b->d_inst,
+                       (const long long *)b->d_off, b->n);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_gather_batch, dim3((unsigned)((b->maxdof + 255) / 256), n), dim3(256), 0, st, (const BatchInst *)b->d_inst);
+    HIPCHK(hipGetLastError());
+    int rc = batch_tangent(b, b->qc, st);
+    if (rc) return rc;
+    if (b->maxnd > 0) {
+        hipLaunchKernelGGL(k_dirichlet_diag_batch, dim3((unsigned)((b->maxnd + 255) / 256), n), dim3(256), 0, st,
+                           (const BatchInst *)b->d_inst, b->vals_f32);
+        HIPCHK(hipGetLastError());
+    }
+    return 0;
+}
+
+static void batch_drop_graph(igap4h_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+}
+
+extern "C" void igap4h_batch_destroy(igap4h_batch *b)
+{
+    if (!b) return;
+    batch_drop_graph(b);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
+    (void)hipFree(b->d_inst);
+    (void)hipFree(b->d_off);
+    delete b;
+}
+
+extern "C" int igap4h_batch_create(igap4h_ctx *const *ctxs, int n, const double *const *d_U_hist, double *const *d_R,
+                                   void *const *d_vals, int prec, int vals_type, igap4h_batch **out)
+{
+    if (!out) return fail(IGAP4H_EINVAL, "batch output pointer is NULL");
+    *out = nullptr;
+    if (!ctxs || !d_U_hist || !d_R || !d_vals || n < 1) return fail(IGAP4H_EINVAL, "batch needs n >= 1 contexts and arrays");
+    if (prec != IGAP4H_FP64 && prec != IGAP4H_FP32) return fail(IGAP4H_EINVAL, "bad precision %d", prec);
+    if (vals_type != IGAP4H_VALS_F64 && vals_type != IGAP4H_VALS_F32) return fail(IGAP4H_EINVAL, "bad vals type %d", vals_type);
+    igap4h_batch *b = new igap4h_batch();
+    b->n = n;
+    b->prec = prec;
+    b->vals_f32 = vals_type == IGAP4H_VALS_F32;
+    b->qc = prec == IGAP4H_FP32 ? GeQC<float>::v : GeQC<double>::v;
+    b->eoff.assign(n + 1, 0);
+    b->coff.assign((size_t)27 * (n + 1), 0);
+    for (int i = 0; i < n; i++) {
+        igap4h_ctx *c = ctxs[i];
+        if (!c || !d_U_hist[i] || !d_R[i] || !d_vals[i]) {
+            igap4h_batch_destroy(b);
+            return fail(IGAP4H_EINVAL, "batch instance %d: NULL context or array", i);
+        }
+        if (c->physics != IGAP4H_PHYS_GE) {
+            igap4h_batch_destroy(b);
+            return fail(IGAP4H_EINVAL, "batch instance %d: batched assembly is implemented for phys/ge", i);
+        }
+        if (i == 0) b->device = c->device;
+        else if (c->device != b->device) {
+            igap4h_batch_destroy(b);
+            return fail(IGAP4H_EINVAL, "batch instance %d lives on device %d, instance 0 on %d", i, c->device, b->device);
+        }
+        if (!c->d_Re) {
+            int r = dev_alloc(&c->d_Re, (size_t)c->mesh.nelem() * 81);
+            if (r) {
+                igap4h_batch_destroy(b);
+                return r;
+            }
+        }
+        BatchInst I;
+        I.m = mesh_args(c);
+        I.Uh = d_U_hist[i];
+        I.coef = c->d_coef;
+        I.rowptr = c->d_rowptr;
+        I.mask = c->d_mask;
+        I.g = c->d_g;
+        I.fneu = c->d_fneu;
+        I.drows = c->d_drows;
+        I.ndrows = (int)c->drows.size();
+        I.Re = c->d_Re;
+        I.R = d_R[i];
+        I.vals = d_vals[i];
+        I.nnz = c->nnz;
+        b->h_inst.push_back(I);
+        b->eoff[i + 1] = b->eoff[i] + c->mesh.nelem();
+        for (int col = 0; col < 27; col++) {
+            int cx = col / 9, cy = (col / 3) % 3, cz = col % 3;
+            long long mx = (c->mesh.ne[0] - cx + 2) / 3, my = (c->mesh.ne[1] - cy + 2) / 3, mz = (c->mesh.ne[2] - cz + 2) / 3;
+            long long cnt = (mx > 0 && my > 0 && mz > 0) ? mx * my * mz : 0;
+            b->coff[(size_t)col * (n + 1) + i + 1] = b->coff[(size_t)col * (n + 1) + i] + cnt;
+        }
+        b->maxdof = std::max<long long>(b->maxdof, c->mesh.ndof());
+        b->maxnnz = std::max<long long>(b->maxnnz, c->nnz);
+        b->maxnd = std::max<int>(b->maxnd, (int)c->drows.size());
+    }
+    if (b->eoff[n] > 0x7fffffffLL) {
+        igap4h_batch_destroy(b);
+        return fail(IGAP4H_EINVAL, "batch has %lld elements; at most 2^31 - 1", b->eoff[n]);
+    }
+    hipError_t e;
+    if ((e = hipSetDevice(b->device)) != hipSuccess || (e = hipMalloc(&b->d_inst, sizeof(BatchInst) * n)) != hipSuccess ||
+        (e = hipMalloc(&b->d_off, sizeof(long long) * 28 * (n + 1))) != hipSuccess ||
+        (e = hipMemcpy(b->d_inst, b->h_inst.data(), sizeof(BatchInst) * n, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->d_off, b->eoff.data(), sizeof(long long) * (n + 1), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->d_off + (n + 1), b->coff.data(), sizeof(long long) * 27 * (n + 1), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking)) != hipSuccess) {
+        igap4h_batch_destroy(b);
+        return fail(IGAP4H_EHIP, "batch set-up: %s", hipGetErrorString(e));
+    }
+    *out = b;
+    return 0;
+}
+
+extern "C" int igap4h_batch_sizes(const igap4h_batch *b, int64_t *nelem, int *qc)
+{
+    if (!b) return fail(IGAP4H_EINVAL, "NULL batch");
+    if (nelem) *nelem = b->eoff[b->n];
+    if (qc) *qc = b->qc;
+    return 0;
+}
+
+extern "C" int igap4h_batch_set_variant(igap4h_batch *b, int qc)
+{
+    if (!b) return fail(IGAP4H_EINVAL, "NULL batch");
+    const int *cand = kBatchQC[b->prec == IGAP4H_FP32];
+    if (qc != cand[0] && qc != cand[1] && qc != cand[2]) return fail(IGAP4H_EINVAL, "variant %d is not in the tuning space", qc);
+    if (qc != b->qc) batch_drop_graph(b);
+    b->qc = qc;
+    return 0;
+}
+
+// Autotuner: time the batched tangent of every variant (reps launches each,
+// HIP events on the batch's own stream), keep the fastest.  ms[k] receives
+// the mean time of candidate k (3 entries), qcs[k] its chunk size.
+extern "C" int igap4h_batch_tune(igap4h_batch *b, int reps, int *best_qc, double *ms, int *qcs)
+{
+    if (!b) return fail(IGAP4H_EINVAL, "NULL batch");
+    if (reps < 1) reps = 3;
+    HIPCHK(hipSetDevice(b->device));
+    const int *cand = kBatchQC[b->prec == IGAP4H_FP32];
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    double best = 1e300;
+    int bq = b->qc, rc = 0;
+    for (int k = 0; k < 3 && !rc; k++) {
+        if ((rc = batch_tangent(b, cand[k], b->cap))) break;  // warm-up
+        if (hipEventRecord(e0, b->cap) != hipSuccess) { rc = fail(IGAP4H_EHIP, "hipEventRecord failed"); break; }
+        for (int r = 0; r < reps && !rc; r++) rc = batch_tangent(b, cand[k], b->cap);
+        if (rc) break;
+        float t = 0.f;
+        if (hipEventRecord(e1, b->cap) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+            rc = fail(IGAP4H_EHIP, "batch tuning timing failed");
+            break;
+        }
+        double mean = t / reps;
+        if (ms) ms[k] = mean;
+        if (qcs) qcs[k] = cand[k];
+        if (mean < best) {
+            best = mean;
+            bq = cand[k];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (bq != b->qc) batch_drop_graph(b);
+    b->qc = bq;
+    if (best_qc) *best_qc = bq;
+    return 0;
+}
+
+// R and K of every instance: one graph replay (captured at the first call
+// and after a variant change).  U_hist, R and vals are the arrays given at
+// create; calls that overlap on one batch race.
+extern "C" int igap4h_batch_assemble(igap4h_batch *b, void *stream)
+{
+    if (!b) return fail(IGAP4H_EINVAL, "NULL batch");
+    HIPCHK(hipSetDevice(b->device));
+    if (!b->exec) {
+        HIPCHK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeRelaxed));
+        int rc = batch_sequence(b, b->cap);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->cap, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(IGAP4H_EHIP, "batch capture: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            b->exec = nullptr;
+            return fail(IGAP4H_EHIP, "batch graph instantiation: %s", hipGetErrorString(e));
+        }
+    }
+    HIPCHK(hipGraphLaunch(b->exec, (hipStream_t)stream));
     return 0;
 }

@@ -211,6 +211,27 @@ int igap4h_energy_dev(igap4h_ctx *ctx, const double *d_U, double *energy, void *
 int igap4h_residual_vjp_coef_dev(igap4h_ctx *ctx, const double *d_U, const double *d_lambda, double *d_g,
                                  void *stream);
 
+/* ---- batched assembly (many independent phys/ge instances) ------------ */
+/* No reference counterpart (the reference assembles one patch per rank).
+ * Instance i is a context with its own device U_hist (d_U_hist[i]), residual
+ * d_R[i] and CSR values d_vals[i] (nnz of vals_type); prec / vals_type as for
+ * igap4h_tangent_dev.  igap4h_batch_assemble computes R and K of every
+ * instance: one residual launch over all elements, one gather, one tangent
+ * launch per colour over all instances, and the Dirichlet diagonals,
+ * captured once into a HIP graph and replayed.  The contexts and arrays must
+ * outlive the batch.                                                        */
+typedef struct igap4h_batch igap4h_batch;
+int igap4h_batch_create(igap4h_ctx *const *ctxs, int n, const double *const *d_U_hist, double *const *d_R,
+                        void *const *d_vals, int prec, int vals_type, igap4h_batch **batch);
+void igap4h_batch_destroy(igap4h_batch *batch);
+int igap4h_batch_sizes(const igap4h_batch *batch, int64_t *nelem, int *variant);
+/* Autotuner: times each tangent variant of the tuning space (Gauss points per
+ * MFMA chunk: FP64 {1, 2, 3}, FP32 {2, 3, 4}) on this batch, reps launches
+ * each, keeps the fastest.  ms[3] / variants[3] (may be NULL) get the table. */
+int igap4h_batch_tune(igap4h_batch *batch, int reps, int *best_variant, double *ms, int *variants);
+int igap4h_batch_set_variant(igap4h_batch *batch, int variant);
+int igap4h_batch_assemble(igap4h_batch *batch, void *stream);
+
 /* ---- Newton-Krylov driver ------------------------------------------- */
 /* The reference leaves the solve to PETSc (src/snes_init.c:8-47,
  * src/snes_solve.c:10-36: NEWTONLS + BT line search, GMRES with right
