@@ -94,7 +94,8 @@ SYMBOLS = ("igap4h_create", "igap4h_destroy", "igap4h_last_error", "igap4h_sizes
            "igap4h_residual_dev", "igap4h_tangent_dev", "igap4h_set_coef", "igap4h_energy_dev",
            "igap4h_newton_default_opts", "igap4h_newton_solve_dev", "igap4h_spmv_dev", "igap4h_linear_solve_dev",
            "igap4h_residual_vjp_coef_dev", "igap4h_batch_create", "igap4h_batch_destroy", "igap4h_batch_sizes",
-           "igap4h_batch_tune", "igap4h_batch_set_variant", "igap4h_batch_assemble")
+           "igap4h_batch_tune", "igap4h_batch_set_variant", "igap4h_batch_assemble", "igap4h_write_solution",
+           "igap4h_read_solution", "igap4h_knot_insert")
 
 _lib = None
 
@@ -139,6 +140,10 @@ def lib():
         L.igap4h_batch_tune.argtypes = [vp, ctypes.c_int, c_int_p, c_double_p, c_int_p]
         L.igap4h_batch_set_variant.argtypes = [vp, ctypes.c_int]
         L.igap4h_batch_assemble.argtypes = [vp, vp]
+        L.igap4h_write_solution.argtypes = [ctypes.c_char_p, c_double_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int]
+        L.igap4h_read_solution.argtypes = [ctypes.c_char_p, c_double_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int]
+        L.igap4h_knot_insert.argtypes = [c_double_p, c_int_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         c_int_p]
         L.igap4h_newton_default_opts.argtypes = [ctypes.POINTER(NewtonOpts)]
         L.igap4h_newton_solve_dev.argtypes = [vp, vp, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonInfo), vp]
         L.igap4h_spmv_dev.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp]

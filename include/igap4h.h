@@ -201,6 +201,21 @@ int igap4h_set_coef(igap4h_ctx *ctx, const double *elem_coef);
  * volume part, phys/ge density hh[0]); ge only.  d_U: device ndof.        */
 int igap4h_energy_dev(igap4h_ctx *ctx, const double *d_U, double *energy, void *stream);
 
+/* ---- host utilities (no device needed) --------------------------------- */
+/* write_solution / read_solution (src/ioutil.c:89-158) on one rank: slices
+ * ihist0..ihist1-1 of U_hist as raw doubles in universal order (= the local
+ * order on one rank), no header; read checks the file size exactly.       */
+int igap4h_write_solution(const char *fname, const double *U_hist, int64_t ndof, int ihist0, int ihist1);
+int igap4h_read_solution(const char *fname, double *U_hist, int64_t ndof, int ihist0, int ihist1);
+/* nurbs_knotinsert (mesh/nurbs/nurbs_knotinsert.c:7-107): refine a solution
+ * nref times by midpoint knot insertion, in place; U_universal is sized for
+ * the fine mesh nelem[3] and holds the coarse one (nelem / 2^nref) on entry.
+ * porder 2, ndim 3, non-periodic.  Exact refinement: equal to the
+ * reference's masks (:41-45) except where a coarse direction has a single
+ * element, for which those masks do not reproduce the spline.             */
+int igap4h_knot_insert(double *U_universal, const int nelem[3], int nref, int porder, int ndim, int ndof,
+                       const int bc_periodic[3]);
+
 /* ---- adjoint: forward/backward pair for the material coefficients ------ */
 /* Forward: igap4h_residual_dev with the coefficients of igap4h_set_coef.
  * Backward (phys/ge): d_g[6e + k] = lambda^T dR/dcoef[e][k] for every element

@@ -353,3 +353,45 @@ def ref_iga_dirichlet(par, face_id, bc_order, idof, kv3, nb3):
     ref().iga_dirichlet(_d(out), _d(np.ascontiguousarray(par, dtype=np.float64)), face_id, bc_order, idof,
                         arr, nk, nb, 3, 2)
     return out
+
+
+def knotinsert(U_universal, nelem, nref, ndof=3):
+    """Restatement of mesh/nurbs/nurbs_knotinsert.c:7-107 (porder 2,
+    non-periodic): the coarse control values (nelem / 2^nref) refined nref
+    times with the per-index masks of :41-45, scattered to fine indices
+    2 i - 2 + j (:87-91).  Returns the refined array (fine size)."""
+    ne = [n >> nref for n in nelem]
+    nb = [n + 2 for n in ne]
+    U = np.asarray(U_universal, dtype=np.float64)[:ndof * nb[0] * nb[1] * nb[2]].reshape(nb[0], nb[1], nb[2], ndof)
+
+    def mask(i, n):   # n = coarse element count
+        if i == 0:
+            return (0.0, 0.0, 1.0, 0.5)
+        if i == 1:
+            return (0.0, 0.5, 0.75, 0.25)
+        if i < n:
+            return (0.25, 0.75, 0.75, 0.25)
+        if i == n:
+            return (0.25, 0.75, 0.5, 0.0)
+        return (0.5, 1.0, 0.0, 0.0)
+
+    for _ in range(nref):
+        nbn = [2 * n + 2 for n in ne]
+        out = np.zeros((nbn[0], nbn[1], nbn[2], ndof))
+        for ix in range(nb[0]):
+            vx = mask(ix, ne[0])
+            for iy in range(nb[1]):
+                vy = mask(iy, ne[1])
+                for iz in range(nb[2]):
+                    vz = mask(iz, ne[2])
+                    for jx in range(4):
+                        for jy in range(4):
+                            for jz in range(4):
+                                w = vx[jx] * vy[jy] * vz[jz]
+                                if w != 0.0:
+                                    out[2 * ix - 2 + jx, 2 * iy - 2 + jy, 2 * iz - 2 + jz] += U[ix, iy, iz] * w
+        U = out
+        ne = [2 * n for n in ne]
+        nb = [n + 2 for n in ne]
+    return U.reshape(-1)
+
