@@ -95,7 +95,7 @@ SYMBOLS = ("igap4h_create", "igap4h_destroy", "igap4h_last_error", "igap4h_sizes
            "igap4h_newton_default_opts", "igap4h_newton_solve_dev", "igap4h_spmv_dev", "igap4h_linear_solve_dev",
            "igap4h_residual_vjp_coef_dev", "igap4h_batch_create", "igap4h_batch_destroy", "igap4h_batch_sizes",
            "igap4h_batch_tune", "igap4h_batch_set_variant", "igap4h_batch_assemble", "igap4h_write_solution",
-           "igap4h_read_solution", "igap4h_knot_insert")
+           "igap4h_read_solution", "igap4h_knot_insert", "igap4h_update_bc")
 
 _lib = None
 
@@ -140,6 +140,7 @@ def lib():
         L.igap4h_batch_tune.argtypes = [vp, ctypes.c_int, c_int_p, c_double_p, c_int_p]
         L.igap4h_batch_set_variant.argtypes = [vp, ctypes.c_int]
         L.igap4h_batch_assemble.argtypes = [vp, vp]
+        L.igap4h_update_bc.argtypes = [vp, ctypes.POINTER(BC)]
         L.igap4h_write_solution.argtypes = [ctypes.c_char_p, c_double_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int]
         L.igap4h_read_solution.argtypes = [ctypes.c_char_p, c_double_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int]
         L.igap4h_knot_insert.argtypes = [c_double_p, c_int_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,

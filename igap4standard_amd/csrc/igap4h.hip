@@ -1064,6 +1064,7 @@ extern "C" int igap4h_create(const IGA *app, const igap4h_opts *opts, igap4h_ctx
             }
     }
     c->U_hist_app = app->U_hist;
+    c->neumann = o.neumann;
     int rc = check_device(&c->device);
     if (rc) {
         delete c;
@@ -1170,6 +1171,51 @@ extern "C" int igap4h_dirichlet_rows(igap4h_ctx *c, int32_t *rows)
 {
     if (!c || !rows) return fail(IGAP4H_EINVAL, "NULL argument");
     if (!c->drows.empty()) std::memcpy(rows, c->drows.data(), c->drows.size() * sizeof(int32_t));
+    return 0;
+}
+
+// Re-read bc_type / par_dirichlet (the reference re-reads BC on every
+// callback, snesiga_Residual.c:326-412, snesiga_Tangent.c:203-272): a no-op
+// when nothing changed, else the Dirichlet rows, values and mask (and the
+// Neumann load, whose faces follow bc_type) are rebuilt and uploaded.
+extern "C" int igap4h_update_bc(igap4h_ctx *c, const BC *bc)
+{
+    if (!c || !bc || !bc->bc_type || !bc->par_dirichlet) return fail(IGAP4H_EINVAL, "NULL argument");
+    Mesh &m = c->mesh;
+    bool same = true;
+    for (int i = 0; i < 36 && same; i++) same = m.bc_type[i] == bc->bc_type[i];
+    for (int i = 0; i < 9 && same; i++) same = m.par_dirichlet[i] == bc->par_dirichlet[i];
+    if (same && m.par_neumann == bc->par_neumann) return 0;
+    for (int i = 0; i < 36; i++) m.bc_type[i] = bc->bc_type[i];
+    for (int i = 0; i < 9; i++) m.par_dirichlet[i] = bc->par_dirichlet[i];
+    m.par_neumann = bc->par_neumann;
+    HIPCHK(hipSetDevice(c->device));
+    const int64_t ndof = m.ndof();
+    std::vector<int32_t> rows = dirichlet_rows(m);
+    std::vector<unsigned char> mask;
+    std::vector<double> g;
+    dirichlet_values(m, mask, g);
+    std::vector<double> fneu = neumann_load(m, c->nquad, c->neumann);
+    if (rows.size() != c->drows.size()) {
+        (void)hipFree(c->d_drows);
+        c->d_drows = nullptr;
+        int r = dev_alloc(&c->d_drows, rows.size());
+        if (r) return r;
+    }
+    c->drows = rows;
+    if (!rows.empty()) HIPCHK(hipMemcpy(c->d_drows, rows.data(), rows.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_mask, mask.data(), (size_t)ndof, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_g, g.data(), (size_t)ndof * sizeof(double), hipMemcpyHostToDevice));
+    if (fneu.empty()) {
+        (void)hipFree(c->d_fneu);
+        c->d_fneu = nullptr;
+    } else {
+        if (!c->d_fneu) {
+            int r = dev_alloc(&c->d_fneu, (size_t)ndof);
+            if (r) return r;
+        }
+        HIPCHK(hipMemcpy(c->d_fneu, fneu.data(), (size_t)ndof * sizeof(double), hipMemcpyHostToDevice));
+    }
     return 0;
 }
 

@@ -46,6 +46,7 @@ struct igap4h_ctx {
     size_t d_vals_bytes = 0;
     double *d_Pe = nullptr;
     double *U_hist_app = nullptr;
+    igap4h_neumann_fn neumann = nullptr;  // traction hook given at create
     igap4h::Solver *solver = nullptr;  // Krylov work space, created on first use
 };
 

@@ -137,6 +137,15 @@ class App:
         L.check(L.lib().igap4h_pattern_dev(self.ctx, ctypes.byref(r), ctypes.byref(c)))
         return r.value, c.value
 
+    def update_bc(self):
+        """Push pb.bc_type / pb.par_dirichlet (e.g. a new load step) to the context."""
+        self._bc_type[:] = np.asarray(self.pb.bc_type, dtype=np.int32)
+        self._pd[:] = np.asarray(self.pb.par_dirichlet, dtype=np.float64)
+        L.check(L.lib().igap4h_update_bc(self.ctx, ctypes.byref(self.bc)))
+        nd = ctypes.c_int64()
+        L.check(L.lib().igap4h_sizes(self.ctx, None, None, ctypes.byref(nd), None, None))
+        self.ndirichlet = nd.value
+
     def set_coef(self, coef):
         self._coef = np.ascontiguousarray(coef, dtype=np.float64)
         L.check(L.lib().igap4h_set_coef(self.ctx, _dp(self._coef)))

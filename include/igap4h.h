@@ -164,6 +164,12 @@ const char *igap4h_last_error(void);
 int igap4h_sizes(const igap4h_ctx *ctx, int64_t *ndof, int64_t *nnz, int64_t *ndirichlet, int64_t *nelem,
                  int *nhist);
 
+/* The reference re-reads app->bc on every callback; a context copies it at
+ * create.  Call this after changing bc_type, par_dirichlet or par_neumann
+ * (load stepping): a no-op when nothing changed, else the Dirichlet rows,
+ * values and mask and the Neumann load are rebuilt.                        */
+int igap4h_update_bc(igap4h_ctx *ctx, const BC *bc);
+
 /* ---- index sets (bit-exact with the reference) ------------------------ */
 /* AIJ pattern of P (snesiga_init.c:112-124 with iga_setnz :341-359): rows in
  * globalIdx order, columns ascending.  Host outputs (ndof+1, nnz).        */
