@@ -71,7 +71,7 @@ class NewtonOpts(ctypes.Structure):
                 ("ksp_guess", ctypes.c_int), ("pc", ctypes.c_int), ("line_search", ctypes.c_int),
                 ("stagnation", ctypes.c_int), ("dirichlet_lift", ctypes.c_int),
                 ("mg_pre", ctypes.c_int), ("mg_post", ctypes.c_int), ("mg_graph", ctypes.c_int),
-                ("mg_smoother", ctypes.c_int)]
+                ("mg_smoother", ctypes.c_int), ("lag_jacobian", ctypes.c_int), ("lag_persist", ctypes.c_int)]
 
 
 class NewtonInfo(ctypes.Structure):
@@ -79,14 +79,15 @@ class NewtonInfo(ctypes.Structure):
     _fields_ = [("converged", ctypes.c_int), ("newton_its", ctypes.c_int), ("ksp_its", ctypes.c_int),
                 ("fnorm0", ctypes.c_double), ("fnorm", ctypes.c_double), ("t_residual", ctypes.c_double),
                 ("t_tangent", ctypes.c_double), ("t_solve", ctypes.c_double), ("t_total", ctypes.c_double),
-                ("fnorm_hist", ctypes.c_double * 64), ("ksp_its_hist", ctypes.c_int * 64)]
+                ("fnorm_hist", ctypes.c_double * 64), ("ksp_its_hist", ctypes.c_int * 64), ("tangents", ctypes.c_int)]
 
     def as_dict(self):
         k = min(self.newton_its + 1, 64)
         return {"converged": self.converged, "newton_its": self.newton_its, "ksp_its": self.ksp_its,
                 "fnorm0": self.fnorm0, "fnorm": self.fnorm, "t_residual": self.t_residual,
                 "t_tangent": self.t_tangent, "t_solve": self.t_solve, "t_total": self.t_total,
-                "fnorm_hist": list(self.fnorm_hist[:k]), "ksp_its_hist": list(self.ksp_its_hist[:k - 1])}
+                "fnorm_hist": list(self.fnorm_hist[:k]), "ksp_its_hist": list(self.ksp_its_hist[:k - 1]),
+                "tangents": self.tangents}
 
 # every entry point declared in include/igap4h.h
 SYMBOLS = ("igap4h_create", "igap4h_destroy", "igap4h_last_error", "igap4h_sizes", "igap4h_pattern",

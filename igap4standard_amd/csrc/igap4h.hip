@@ -1190,6 +1190,7 @@ extern "C" int igap4h_update_bc(igap4h_ctx *c, const BC *bc)
     for (int i = 0; i < 9; i++) m.par_dirichlet[i] = bc->par_dirichlet[i];
     m.par_neumann = bc->par_neumann;
     HIPCHK(hipSetDevice(c->device));
+    solver_invalidate_jacobian(c);
     const int64_t ndof = m.ndof();
     std::vector<int32_t> rows = dirichlet_rows(m);
     std::vector<unsigned char> mask;

@@ -62,4 +62,5 @@ static inline int dev_alloc(X **p, size_t n)
     return 0;
 }
 void solver_free(igap4h_ctx *c);
+void solver_invalidate_jacobian(igap4h_ctx *c);  // a lagged tangent no longer fits the problem
 }  // namespace igap4h

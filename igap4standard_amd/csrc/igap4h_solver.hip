@@ -825,6 +825,10 @@ struct igap4h::Solver {
     hipGraphExec_t mg_exec = nullptr;
     const void *key_vals = nullptr;
     int key_f32 = -1, key_pre = -1, key_post = -1, key_smoother = -1;
+    const void *pc_vals = nullptr;  // matrix the preconditioner was last set up on
+    bool jac_valid = false;  // s->vals holds a tangent (of jac_key) for lagged Newton steps
+    long long jac_key = -1;
+    long long jac_age = 0;
     double *mg_in = nullptr;
     long long ndof = 0, nnode = 0;
     int restart = 0;
@@ -841,6 +845,11 @@ struct igap4h::Solver {
     double *sc = nullptr;       // device scalars
     double *hp = nullptr;       // pinned host mirror of sc
 };
+
+void igap4h::solver_invalidate_jacobian(igap4h_ctx *c)
+{
+    if (c && c->solver) c->solver->jac_valid = false;
+}
 
 void igap4h::solver_free(igap4h_ctx *c)
 {
@@ -1368,6 +1377,7 @@ struct Op {
     }
     int setup_pc() const
     {
+        s->pc_vals = vals;
         if (pc == IGAP4H_PC_NONE) return 0;
         unsigned g = grid_for(sp.nnode, 256);
         if (f32) hipLaunchKernelGGL((k_block_diag<float>), dim3(g), dim3(256), 0, st, sp, (const float *)vals, s->Dblk, s->Dinv);
@@ -1663,6 +1673,8 @@ extern "C" int igap4h_newton_default_opts(igap4h_newton_opts *o)
     o->mg_post = 2;
     o->mg_graph = 1;
     o->mg_smoother = IGAP4H_MG_CHEBYSHEV;
+    o->lag_jacobian = 1;
+    o->lag_persist = 0;
     return 0;
 }
 
@@ -1745,15 +1757,28 @@ extern "C" int igap4h_newton_solve_dev(igap4h_ctx *c, double *d_Uh, const igap4h
             break;
         }
         auto t0 = clk::now();
-        if ((rc = igap4h_tangent_dev(c, d_Uh, o->tangent_prec, o->vals_type, s->vals, st))) return rc;
-        HIPCHK(hipStreamSynchronize(st));
+        // the tangent (and the preconditioner built on it) is rebuilt every
+        // lag_jacobian-th Newton step, counted across solves with lag_persist
+        const int lag = o->lag_jacobian > 1 ? o->lag_jacobian : 1;
+        const long long key = ((long long)o->tangent_prec << 1) | o->vals_type;
+        const bool stale = !s->jac_valid || s->jac_key != key || (!o->lag_persist && it == 0);
+        if (stale) s->jac_age = 0;
+        const bool rebuild = stale || s->jac_age % lag == 0;
+        if (rebuild) {
+            if ((rc = igap4h_tangent_dev(c, d_Uh, o->tangent_prec, o->vals_type, s->vals, st))) return rc;
+            HIPCHK(hipStreamSynchronize(st));
+            s->jac_valid = true;
+            s->jac_key = key;
+            info->tangents++;
+        }
+        s->jac_age++;
         auto t1 = clk::now();
         info->t_tangent += secs(t0, t1);
         if (!o->ksp_guess) {
             hipLaunchKernelGGL(k_zero, dim3(vgrid(n)), dim3(256), 0, st, s->Y, n);
             HIPCHK(hipGetLastError());
         }
-        if ((rc = A.setup_pc())) return rc;
+        if ((rebuild || s->pc_vals != s->vals) && (rc = A.setup_pc())) return rc;
         int kits = 0;
         double relres = 0.0;
         rc = linear_solve(A, s->R, s->Y, o, &kits, &relres);

@@ -59,6 +59,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--dt", type=float, default=1e-3)
     ap.add_argument("--fp64", action="store_true", help="FP64 tangent (default: FP32-MFMA tangent, FP32 values)")
+    ap.add_argument("--lag", type=int, default=1, help="rebuild the tangent every LAG Newton steps")
+    ap.add_argument("--persist", action="store_true", help="count the lag across time steps")
     args = ap.parse_args(argv)
     import torch
 
@@ -67,7 +69,7 @@ def main(argv=None):
 
     pb = P.transient_problem(args.n, dt=args.dt)
     v0 = P.initial_velocity(pb)
-    ts = TimeStepper(pb, mixed=not args.fp64)
+    ts = TimeStepper(pb, mixed=not args.fp64, lag_jacobian=args.lag, lag_persist=int(args.persist))
     ts.set_state(np.zeros(pb.ndof), -args.dt * v0)
     say("%d^3 getime, %d dofs, dt %g, %s tangent" % (args.n, pb.ndof, args.dt, "FP64" if args.fp64 else "FP32"))
     infos, norms = [], []
@@ -92,6 +94,7 @@ def main(argv=None):
            "ms_per_step": 1000.0 * wall / max(done, 1),
            "newton_its_mean": float(np.mean([i["newton_its"] for i in infos])),
            "ksp_its_mean": float(np.mean([i["ksp_its"] for i in infos])),
+           "tangents": int(sum(i["tangents"] for i in infos)), "lag_jacobian": args.lag, "lag_persist": args.persist,
            "max_final_fnorm": float(max(i["fnorm"] for i in infos)),
            "all_converged": all(i["converged"] == 1 for i in infos),
            "elements_per_s_per_newton_step": pb.n_elem * sum(i["newton_its"] for i in infos) / wall,

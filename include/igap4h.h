@@ -299,6 +299,11 @@ typedef struct {
     int mg_smoother;   /* IGAP4H_PC_MG: IGAP4H_MG_GS (block Gauss-Seidel, mg_pre/mg_post
                           sweeps) or IGAP4H_MG_CHEBYSHEV (block-Jacobi Chebyshev of
                           degree mg_pre/mg_post, largest eigenvalue by power iteration) */
+    int lag_jacobian;  /* rebuild the tangent and preconditioner every lag_jacobian-th
+                          Newton step (PETSc -snes_lag_jacobian; 1 = every step, the
+                          reference)                                              */
+    int lag_persist;   /* 1: the lag counts across solves (-snes_lag_jacobian_persists),
+                          so a time stepper reuses the tangent of earlier steps    */
 } igap4h_newton_opts;
 
 typedef struct {
@@ -311,6 +316,7 @@ typedef struct {
                                                         count in t_residual) */
     double fnorm_hist[64];  /* ||R|| before each Newton step and after the last */
     int ksp_its_hist[64];   /* Krylov iterations of each Newton step            */
+    int tangents;           /* tangent assemblies in this solve                  */
 } igap4h_newton_info;
 
 int igap4h_newton_default_opts(igap4h_newton_opts *opts);

@@ -44,3 +44,26 @@ def test_time_steps_match_oracle_residual(oracle):
     rel = np.linalg.norm(traj[True] - traj[False]) / np.linalg.norm(traj[False])
     assert rel < 1e-8, rel
     assert np.linalg.norm(traj[False][-1]) > 0.0
+
+
+def test_lagged_tangent_steps(oracle):
+    """Modified Newton (tangent of the first step reused for all steps,
+    lag_jacobian with lag_persist): every step still reaches the absolute
+    tolerance and the oracle's residual; one tangent assembly in total."""
+    from igap4standard_amd.transient import TimeStepper
+    pb = P.transient_problem(6, dt=1e-3)
+    v0 = P.initial_velocity(pb, std=1e-3, corr_len=0.3)
+    n = pb.ndof
+    ts = TimeStepper(pb, mixed=True, lag_jacobian=1000, lag_persist=1)
+    ts.set_state(np.zeros(n), -1e-3 * v0)
+    tangents = 0
+    for k in range(3):
+        before = ts.Uh.cpu().numpy()
+        info = ts.step()
+        assert info["converged"] == 1, info
+        tangents += info["tangents"]
+        U_new = ts.Uh.cpu().numpy()[:n]
+        R = oracle.residual(pb, np.concatenate([U_new, before[n:2 * n], before[2 * n:]]))
+        assert np.linalg.norm(R) < 1e-10
+    assert tangents == 1
+
