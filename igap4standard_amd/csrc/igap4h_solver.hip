@@ -94,6 +94,15 @@ __device__ __forceinline__ void node_rows(const SpArgs &s, long long node, const
     acc[2] = wave_sum(s2);
 }
 
+// Work-groups are dispatched to the 8 XCDs round robin (block b -> XCD b mod 8),
+// each XCD with its own L2.  This bijection gives XCD k a contiguous range of
+// block indices, so the x entries that neighbouring nodes share stay in one L2.
+__device__ __forceinline__ long long xcd_remap(long long b, long long nblk)
+{
+    const long long q = nblk >> 3, rem = nblk & 7, x = b & 7, i = b >> 3;
+    return x * q + (x < rem ? x : rem) + i;
+}
+
 // y = K x (RES = 0) or y = b - K x (RES = 1); with DOT, partial[block] =
 // sum over the block's rows of x . y (for CG's p.Kp)
 template <class VT, int RES, int DOT>
@@ -103,7 +112,7 @@ __global__ __launch_bounds__(256) void k_spmv(SpArgs s, const VT *__restrict__ v
 {
     __shared__ double sd[4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const long long node = (long long)blockIdx.x * 4 + wv;
+    const long long node = xcd_remap(blockIdx.x, gridDim.x) * 4 + wv;
     double d = 0.0;
     if (node < s.nnode) {
         double acc[3];
