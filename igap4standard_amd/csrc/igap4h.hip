@@ -541,7 +541,8 @@ __device__ __forceinline__ void load_compact_soa(const double *sC, int qp, Compa
     for (int f = 0; f < NCMP; f++) v[f] = sC[f * 64 + qp];
 }
 
-template <int J, class ACC>
+// T_ij(q, B) for dof block j = J and rows i of [I0, I1]
+template <int J, int I0, int I1, class ACC>
 __device__ __forceinline__ void ge_task(const double *sCmp, const double *sphi, const int *sQ, const double *sw,
                                         ACC *sT, const Coef &c, int ql, int B, int qp, int nqp)
 {
@@ -552,15 +553,15 @@ __device__ __forceinline__ void ge_task(const double *sCmp, const double *sphi, 
         basis_slots(sphi, q & 15, (q >> 4) & 15, (q >> 8) & 15, B, n);
         Compact<double> cp;
         load_compact_soa(sCmp, qp, cp);
-        tangent_apply<double, J>(cp, c, n, 0.0, sw[qp], out);
+        tangent_apply<double, J, I0, I1>(cp, c, n, 0.0, sw[qp], out);
     } else {
 #pragma unroll
-        for (int i = 0; i <= J; i++)
+        for (int i = I0; i <= I1; i++)
 #pragma unroll
             for (int s = 0; s < 10; s++) out[i][s] = 0.0;
     }
 #pragma unroll
-    for (int i = 0; i <= J; i++) {
+    for (int i = I0; i <= I1; i++) {
         ACC *col = sT + block_off(i) + (J - i) * 27 + B;
 #pragma unroll
         for (int s = 1; s < 10; s++) col[(ql * 9 + s - 1) * 192] = (ACC)out[i][s];
@@ -649,14 +650,15 @@ __device__ __forceinline__ void tangent_ge_body(const MeshArgs &m, const double 
 
     const int nchunk = (nqp + QC - 1) / QC;
     for (int ch = 0; ch < nchunk; ch++) {
-        if (wave < 3) {
-            // T_ij for dof block j = 2 - wave; lane = (ql, B)
-            for (int t = lane; t < QC * 27; t += 64) {
-                int ql = t / 27, B = t % 27, qp = ch * QC + ql;
-                if (wave == 0) ge_task<2, ACC>(sCmp, sphi, sQ, sw, sT, c, ql, B, qp, nqp);
-                else if (wave == 1) ge_task<1, ACC>(sCmp, sphi, sQ, sw, sT, c, ql, B, qp, nqp);
-                else ge_task<0, ACC>(sCmp, sphi, sQ, sw, sT, c, ql, B, qp, nqp);
-            }
+        // T_ij, lane = (ql, B); the six (i <= j) blocks over the four waves:
+        // wave 0 (i = 1, 2; j = 2), wave 1 (i = 0, 1; j = 1), wave 2
+        // (i = 0; j = 0), wave 3 (i = 0; j = 2)
+        for (int t = lane; t < QC * 27; t += 64) {
+            int ql = t / 27, B = t % 27, qp = ch * QC + ql;
+            if (wave == 0) ge_task<2, 1, 2, ACC>(sCmp, sphi, sQ, sw, sT, c, ql, B, qp, nqp);
+            else if (wave == 1) ge_task<1, 0, 1, ACC>(sCmp, sphi, sQ, sw, sT, c, ql, B, qp, nqp);
+            else if (wave == 2) ge_task<0, 0, 0, ACC>(sCmp, sphi, sQ, sw, sT, c, ql, B, qp, nqp);
+            else ge_task<2, 0, 0, ACC>(sCmp, sphi, sQ, sw, sT, c, ql, B, qp, nqp);
         }
         __syncthreads();
         const int ct0 = 3 * wave;

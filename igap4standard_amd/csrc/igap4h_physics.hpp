@@ -311,10 +311,11 @@ IGAP4H_HD void compact_of(const T *u, const Coef &c, Compact<T> &cp)
 // T_ij[s, B] for i in [0, J], one basis function with slot values n[10].
 // Writes out[i][s] for s = 0..9 (value slot only when mass != 0).  J is a
 // compile-time constant so that out[][] stays in registers.
-template <class T, int JB>
+template <class T, int JB, int IB0 = 0, int IB1 = JB>
 IGAP4H_HD void tangent_apply(const Compact<T> &cp, const Coef &c, const double n[10], double mass, double w,
                              double out[JB + 1][10])
 {
+    // rows i of [IB0, IB1] only (a caller may split one j over several lanes)
     constexpr int j = JB;
     // beta = sym(F_j^T grad n) in (00,11,22,01,02,12)
     T b[6];
@@ -342,7 +343,7 @@ IGAP4H_HD void tangent_apply(const Compact<T> &cp, const Coef &c, const double n
     }
     const T Sm[3][3] = {{cp.S[0], cp.S[3], cp.S[4]}, {cp.S[3], cp.S[1], cp.S[5]}, {cp.S[4], cp.S[5], cp.S[2]}};
 #pragma unroll
-    for (int i = 0; i <= j; i++) {
+    for (int i = IB0; i <= IB1; i++) {
         T tf[3], th[6];
         for (int J = 0; J < 3; J++) {
             T t = cp.F[i][0] * G[0][J] + cp.F[i][1] * G[1][J] + cp.F[i][2] * G[2][J];
