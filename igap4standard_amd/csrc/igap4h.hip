@@ -818,11 +818,15 @@ __global__ void k_cols(MeshArgs m, const long long *__restrict__ rowptr, int *__
         for (int j = 0; j < 3; j++) col[pos++] = (int)(3 * node_of(m, b0, b1, b2) + j);
 }
 
-// ============================================================== energy (ge)
+// ============================================================== energy
+// ge: sum_q w W(u) per element; getime (AV = 1): W at (u0 + u1) / 2 from
+// history slices 2 and 1 (phys/getime/density.c:6-50).
+template <int AV>
 __global__ __launch_bounds__(64) void k_energy_elem(MeshArgs m, const double *__restrict__ U,
                                                     const double *__restrict__ coef, double *__restrict__ Pe)
 {
-    __shared__ double sU[81];
+    constexpr int NS = AV ? 2 : 1;
+    __shared__ double sU[NS * 81];
     __shared__ double sphi[3 * 4 * 9];
     __shared__ double sred[64];
     const int tid = threadIdx.x;
@@ -831,9 +835,9 @@ __global__ __launch_bounds__(64) void k_energy_elem(MeshArgs m, const double *__
     ev[2] = (int)(e % m.ne[2]);
     ev[1] = (int)((e / m.ne[2]) % m.ne[1]);
     ev[0] = (int)(e / ((long long)m.ne[2] * m.ne[1]));
-    for (int idx = tid; idx < 81; idx += 64) {
-        int A = idx / 3, i = idx % 3;
-        sU[idx] = U[3 * node_of(m, ev[0] + A / 9, ev[1] + (A / 3) % 3, ev[2] + A % 3) + i];
+    for (int idx = tid; idx < NS * 81; idx += 64) {
+        int h = idx / 81, r = idx % 81, A = r / 3, i = r % 3;
+        sU[idx] = U[(AV ? (h + 1) * m.ndof : 0) + 3 * node_of(m, ev[0] + A / 9, ev[1] + (A / 3) % 3, ev[2] + A % 3) + i];
     }
     fill_phi(m, ev, sphi, tid, 64);
     __syncthreads();
@@ -841,8 +845,10 @@ __global__ __launch_bounds__(64) void k_energy_elem(MeshArgs m, const double *__
     double val = 0.0;
     if (tid < nqp) {
         int qx = tid / (m.nq * m.nq), qy = (tid / m.nq) % m.nq, qz = tid % m.nq;
-        double u[1][30], w;
-        interp_slots<1>(m, sU, sphi, ev, qx, qy, qz, u, &w);
+        double u[NS][30], w;
+        interp_slots<NS>(m, sU, sphi, ev, qx, qy, qz, u, &w);
+        if (AV)
+            for (int s = 0; s < 30; s++) u[0][s] = 0.5 * (u[0][s] + u[NS - 1][s]);
         val = w * energy_of<double>(u[0], load_coef(coef, e));
     }
     sred[tid] = val;
@@ -1385,7 +1391,6 @@ extern "C" int igap4h_residual_vjp_coef_dev(igap4h_ctx *c, const double *d_U, co
 extern "C" int igap4h_energy_dev(igap4h_ctx *c, const double *d_U, double *energy, void *stream)
 {
     if (!c || !d_U || !energy) return fail(IGAP4H_EINVAL, "NULL argument");
-    if (c->physics != IGAP4H_PHYS_GE) return fail(IGAP4H_EINVAL, "energy is implemented for phys/ge only");
     long long ne = c->mesh.nelem();
     if (!c->d_Pe) {
         int r = dev_alloc(&c->d_Pe, (size_t)ne);
@@ -1393,7 +1398,10 @@ extern "C" int igap4h_energy_dev(igap4h_ctx *c, const double *d_U, double *energ
     }
     hipStream_t st = (hipStream_t)stream;
     MeshArgs m = mesh_args(c);
-    hipLaunchKernelGGL(k_energy_elem, dim3((unsigned)ne), dim3(64), 0, st, m, d_U, (const double *)c->d_coef, c->d_Pe);
+    if (c->physics == IGAP4H_PHYS_GE)
+        hipLaunchKernelGGL(k_energy_elem<0>, dim3((unsigned)ne), dim3(64), 0, st, m, d_U, (const double *)c->d_coef, c->d_Pe);
+    else
+        hipLaunchKernelGGL(k_energy_elem<1>, dim3((unsigned)ne), dim3(64), 0, st, m, d_U, (const double *)c->d_coef, c->d_Pe);
     HIPCHK(hipGetLastError());
     std::vector<double> h((size_t)ne);
     HIPCHK(hipMemcpyAsync(h.data(), c->d_Pe, (size_t)ne * sizeof(double), hipMemcpyDeviceToHost, st));

@@ -203,8 +203,10 @@ int igap4h_tangent_dev(igap4h_ctx *ctx, const double *d_U_hist, int prec, int va
 int igap4h_set_coef(igap4h_ctx *ctx, const double *elem_coef);
 
 /* ---- energy ----------------------------------------------------------- */
-/* Volume energy Pi = sum_e sum_q w W(u) (src/iga_intdensity.c:10-293,
- * volume part, phys/ge density hh[0]); ge only.  d_U: device ndof.        */
+/* Volume energy Pi = sum_e sum_q w W (src/iga_intdensity.c:10-293, volume
+ * part, density hh[0]).  ge: W(u), d_U = device ndof (slice 0).  getime:
+ * W((u0 + u1) / 2) (phys/getime/density.c:6-50), d_U = device U_hist with
+ * its three slices (u1 = slice 1, u0 = slice 2).                          */
 int igap4h_energy_dev(igap4h_ctx *ctx, const double *d_U, double *energy, void *stream);
 
 /* ---- host utilities (no device needed) --------------------------------- */

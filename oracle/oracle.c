@@ -1020,7 +1020,14 @@ double or_energy(const or_problem *pb, const double *U_hist)
                 for (int qz = 0; qz < nq; qz++) {
                     elem_qp(pb, &c, ex, ey, ez, qx, qy, qz, N, xq, &w);
                     interp_u(pb, &c, U_hist, ex, ey, ez, N, xq, u);
-                    Pi += w * or_ge_energy(u, coef);
+                    if (pb->physics == 1) {
+                        /* phys/getime/density.c:6-50: W at (u0 + u1) / 2 (slices 2 and 1) */
+                        double ub[30];
+                        for (int k = 0; k < 30; k++) ub[k] = 0.5 * (u[30 + k] + u[60 + k]);
+                        Pi += w * or_ge_energy(ub, coef);
+                    } else {
+                        Pi += w * or_ge_energy(u, coef);
+                    }
                 }
     }
     return Pi;

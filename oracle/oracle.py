@@ -311,6 +311,14 @@ def ref_ge_energy(u, par):
     return h[0]
 
 
+def ref_getime_energy(u90, par):
+    """phys/getime/density.c:6-50 (hh[0])."""
+    h = np.zeros(2)
+    ref().getime_density(_d(h), _d(np.ascontiguousarray(u90, dtype=np.float64)),
+                         _d(np.ascontiguousarray(par, dtype=np.float64)))
+    return float(h[0])
+
+
 def ref_getime_residual(u90, par):
     r = np.zeros(30)
     ref().getime_residual(_d(r), _d(np.ascontiguousarray(u90, dtype=np.float64)),

@@ -67,3 +67,18 @@ def test_lagged_tangent_steps(oracle):
         assert np.linalg.norm(R) < 1e-10
     assert tangents == 1
 
+
+
+def test_getime_energy_matches_oracle(oracle):
+    """igap4h_energy_dev for getime: W at (u0 + u1) / 2 integrated over the
+    volume (phys/getime/density.c via iga_intdensity), against the oracle."""
+    import torch
+    from igap4standard_amd.snesiga import App
+    pb = P.reference_example(3, physics=P.GETIME)
+    rng = np.random.default_rng(4)
+    Uh = rng.uniform(-1, 1, 3 * pb.ndof) * 0.02
+    app = App(pb)
+    d = torch.tensor(Uh, device="cuda:0")
+    e = app.energy_dev(d.data_ptr())
+    e0 = oracle.energy(pb, Uh)
+    assert abs(e - e0) <= 1e-12 * abs(e0), (e, e0)

@@ -185,3 +185,23 @@ def test_live_reference_random(oracle):
         u90 = np.concatenate([u, u * 0.9, u * 0.8])
         r = oracle.ref_getime_residual(u90, PAR)
         assert np.abs(oracle.getime_residual(u90, coef, PAR[5:8]) - r)[GRAD].max() <= 1e-12 * np.abs(r[GRAD]).max()
+
+
+def test_getime_energy_matches_reference_density(oracle):
+    """phys/getime/density.c: W of the ge energy at (u0 + u1) / 2, against the
+    compiled reference (live) at random states."""
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built")
+    import numpy as np
+    par = np.array([0.25, 0.5, 500.0, 250.0, 0.025, 1.0, 1.0, 5e-4, 0.0])
+    coef = oracle.coef_from_par(par)
+    rng = np.random.default_rng(21)
+    for amp in (1e-3, 1e-2, 1e-1):
+        u90 = rng.uniform(-1, 1, 90) * amp
+        for i in range(3):
+            for blk in range(3):
+                u90[30 * blk + 10 * i] = 0.0
+        ub = 0.5 * (u90[30:60] + u90[60:90])
+        e_or = oracle.ge_energy(ub, coef)
+        e_ref = oracle.ref_getime_energy(u90, par)
+        assert abs(e_or - e_ref) <= 1e-12 * max(1.0, abs(e_ref)), (amp, e_or, e_ref)
