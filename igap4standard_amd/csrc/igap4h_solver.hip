@@ -388,6 +388,9 @@ inline unsigned vgrid(long long n) { return (unsigned)std::min<long long>(NRB, (
 // of every level is this stencil cut at the boundary, so the Galerkin
 // operator P^T K P of a level keeps it.
 constexpr int MGW = 4;  // nonzeros per row/column of a 1D transfer
+// value type of the coarse levels' stencil storage: the hierarchy only
+// preconditions, so FP32 values (products and sums in FP64) halve its traffic
+using SMT = float;
 
 struct Xfer {                 // one direction: fine nf <-> coarse nc
     const int *c2f_lo;        // [nc]: first fine index of coarse A's support
@@ -409,12 +412,12 @@ __device__ __forceinline__ void decode_node(const int nb[3], long long node, int
 
 // acc[i] = sum over the stencil of K[(n,i),(n+d,j)] x[n+d, j] (one wave;
 // loads issued together as in node_rows)
-__device__ __forceinline__ void node_rows_sm(const SpArgs &s, long long node, const double *__restrict__ sm,
+__device__ __forceinline__ void node_rows_sm(const SpArgs &s, long long node, const SMT *__restrict__ sm,
                                              const double *__restrict__ x, int lane, double acc[3])
 {
     int a[3];
     decode_node(s.nb, node, a);
-    const double *v = sm + node * 1125;
+    const SMT *v = sm + node * 1125;
     double xv[6], v0[6], v1[6], v2[6];
 #pragma unroll
     for (int it = 0; it < 6; it++) {
@@ -449,12 +452,12 @@ template <int SM, class VT>
 __device__ __forceinline__ void level_rows(const SpArgs &s, long long node, const VT *__restrict__ vals,
                                            const double *__restrict__ x, int lane, double acc[3])
 {
-    if (SM) node_rows_sm(s, node, (const double *)vals, x, lane, acc);
+    if (SM) node_rows_sm(s, node, (const SMT *)vals, x, lane, acc);
     else node_rows<VT>(s, node, vals, x, lane, acc);
 }
 
 // y = b - K x on a level in stencil storage
-__global__ __launch_bounds__(256) void k_resid_sm(SpArgs s, const double *__restrict__ sm, const double *__restrict__ x,
+__global__ __launch_bounds__(256) void k_resid_sm(SpArgs s, const SMT *__restrict__ sm, const double *__restrict__ x,
                                                   const double *__restrict__ b, double *__restrict__ y)
 {
     const int lane = threadIdx.x & 63;
@@ -491,7 +494,7 @@ __global__ __launch_bounds__(256) void k_gs_colour(SpArgs s, int c0, int c1, int
 // for the small coarse levels, where a launch per colour costs more than
 // the colour's work.  The work-group barrier orders the colours.
 template <int BWD>
-__global__ __launch_bounds__(1024) void k_gs_sweep_wg(SpArgs s, const double *__restrict__ sm, const double *__restrict__ Dinv,
+__global__ __launch_bounds__(1024) void k_gs_sweep_wg(SpArgs s, const SMT *__restrict__ sm, const double *__restrict__ Dinv,
                                                       const double *__restrict__ b, double *x)
 {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -650,7 +653,7 @@ __global__ void k_hash_vec(long long n, double *v)
 }
 
 // diagonal 3x3 blocks of a level in stencil storage, and their inverses
-__global__ void k_block_diag_sm(long long nnode, const double *__restrict__ sm, double *__restrict__ Dblk, double *__restrict__ Dinv)
+__global__ void k_block_diag_sm(long long nnode, const SMT *__restrict__ sm, double *__restrict__ Dblk, double *__restrict__ Dinv)
 {
     long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= nnode) return;
@@ -670,7 +673,7 @@ __global__ void k_block_diag_sm(long long nnode, const double *__restrict__ sm, 
 // stencil storage on nbo (nbo[d] coarse).  One thread per (node, offset).
 template <int IN_CSR, class VT>
 __global__ __launch_bounds__(256) void k_galerkin(int d, int3 nbi, int3 nbo, const long long *__restrict__ rowptr,
-                                                  const VT *__restrict__ in, Xfer x, double *__restrict__ out)
+                                                  const VT *__restrict__ in, Xfer x, SMT *__restrict__ out)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int nbi_[3] = {nbi.x, nbi.y, nbi.z}, nbo_[3] = {nbo.x, nbo.y, nbo.z};
@@ -724,11 +727,11 @@ __global__ __launch_bounds__(256) void k_galerkin(int d, int3 nbi, int3 nbo, con
             }
         }
     }
-    double *po = out + N * 1125 + o * 3;
+    SMT *po = out + N * 1125 + o * 3;
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++) po[i * 375 + j] = acc[3 * i + j];
+        for (int j = 0; j < 3; j++) po[i * 375 + j] = (SMT)acc[3 * i + j];
 }
 
 // bc = P^T rf (restriction), one thread per coarse node
@@ -813,10 +816,10 @@ namespace {
 struct MGLevel {
     int ne[3], nb[3];
     long long nnode = 0, ndof = 0;
-    double *sm = nullptr;                 // stencil storage (levels >= 1)
+    SMT *sm = nullptr;                    // stencil storage (levels >= 1)
     double *Dblk = nullptr, *Dinv = nullptr;
     double *x = nullptr, *b = nullptr, *r = nullptr;
-    double *stage[2] = {nullptr, nullptr};  // Galerkin intermediates towards the next level
+    SMT *stage[2] = {nullptr, nullptr};   // Galerkin intermediates towards the next level
     int *xfer_i = nullptr;                // transfer tables to the next level (device)
     double *xfer_w = nullptr;
     Xfer3 T;                              // views into xfer_i / xfer_w
@@ -874,8 +877,8 @@ void igap4h::solver_free(igap4h_ctx *c)
     if (s->cap) (void)hipStreamDestroy(s->cap);
     (void)hipFree(s->mg_in);
     for (MGLevel &L : s->mg) {
-        for (double *p : {L.sm, L.Dblk, L.Dinv, L.x, L.b, L.r, L.stage[0], L.stage[1], L.xfer_w, L.Ainv, L.d[0], L.d[1]})
-            (void)hipFree(p);
+        for (double *p : {L.Dblk, L.Dinv, L.x, L.b, L.r, L.xfer_w, L.Ainv, L.d[0], L.d[1]}) (void)hipFree(p);
+        for (SMT *p : {L.sm, L.stage[0], L.stage[1]}) (void)hipFree(p);
         (void)hipFree(L.xfer_i);
     }
     delete s;
@@ -1095,7 +1098,7 @@ constexpr int CHEB_POWER_ITS = 12;      // power iterations for lmax
         const MGLevel &L_ = s->mg[l];                                                                                  \
         SpArgs a_ = (l) ? level_args(L_) : sp;                                                                         \
         dim3 g_(grid_for(a_.nnode, 4));                                                                                \
-        if (l) hipLaunchKernelGGL((KERNEL<1, double>), g_, dim3(256), 0, st, a_, (const double *)L_.sm, __VA_ARGS__);  \
+        if (l) hipLaunchKernelGGL((KERNEL<1, SMT>), g_, dim3(256), 0, st, a_, (const SMT *)L_.sm, __VA_ARGS__);        \
         else if (f32) hipLaunchKernelGGL((KERNEL<0, float>), g_, dim3(256), 0, st, a_, (const float *)vals, __VA_ARGS__); \
         else hipLaunchKernelGGL((KERNEL<0, double>), g_, dim3(256), 0, st, a_, (const double *)vals, __VA_ARGS__);    \
         HIPCHK(hipGetLastError());                                                                                     \
@@ -1141,8 +1144,8 @@ struct Op {
         const MGLevel &L = s->mg[l];
         SpArgs a = l ? level_args(L) : sp;
         if (l && L.nnode <= MG_WG_NODES) {
-            if (bwd) hipLaunchKernelGGL(k_gs_sweep_wg<1>, dim3(1), dim3(1024), 0, st, a, (const double *)L.sm, (const double *)L.Dinv, b, x);
-            else hipLaunchKernelGGL(k_gs_sweep_wg<0>, dim3(1), dim3(1024), 0, st, a, (const double *)L.sm, (const double *)L.Dinv, b, x);
+            if (bwd) hipLaunchKernelGGL(k_gs_sweep_wg<1>, dim3(1), dim3(1024), 0, st, a, (const SMT *)L.sm, (const double *)L.Dinv, b, x);
+            else hipLaunchKernelGGL(k_gs_sweep_wg<0>, dim3(1), dim3(1024), 0, st, a, (const SMT *)L.sm, (const double *)L.Dinv, b, x);
             HIPCHK(hipGetLastError());
             return 0;
         }
@@ -1154,7 +1157,7 @@ struct Op {
             long long ncol = (long long)m0 * m1 * m2;
             if (ncol <= 0) continue;
             dim3 g(grid_for(ncol, 4));
-            if (l) hipLaunchKernelGGL((k_gs_colour<1, double>), g, dim3(256), 0, st, a, c0, c1, c2, m1, m2, ncol, (const double *)L.sm, Dinv, b, x);
+            if (l) hipLaunchKernelGGL((k_gs_colour<1, SMT>), g, dim3(256), 0, st, a, c0, c1, c2, m1, m2, ncol, (const SMT *)L.sm, Dinv, b, x);
             else if (f32) hipLaunchKernelGGL((k_gs_colour<0, float>), g, dim3(256), 0, st, a, c0, c1, c2, m1, m2, ncol, (const float *)vals, Dinv, b, x);
             else hipLaunchKernelGGL((k_gs_colour<0, double>), g, dim3(256), 0, st, a, c0, c1, c2, m1, m2, ncol, (const double *)vals, Dinv, b, x);
             HIPCHK(hipGetLastError());
@@ -1176,13 +1179,13 @@ struct Op {
                 if (f32) hipLaunchKernelGGL((k_galerkin<1, float>), dim3(nthr(g1)), dim3(256), 0, st, 0, g0, g1, sp.rowptr, (const float *)vals, F.T.d[0], F.stage[0]);
                 else hipLaunchKernelGGL((k_galerkin<1, double>), dim3(nthr(g1)), dim3(256), 0, st, 0, g0, g1, sp.rowptr, (const double *)vals, F.T.d[0], F.stage[0]);
             } else {
-                hipLaunchKernelGGL((k_galerkin<0, double>), dim3(nthr(g1)), dim3(256), 0, st, 0, g0, g1, (const long long *)nullptr, (const double *)F.sm, F.T.d[0], F.stage[0]);
+                hipLaunchKernelGGL((k_galerkin<0, SMT>), dim3(nthr(g1)), dim3(256), 0, st, 0, g0, g1, (const long long *)nullptr, (const SMT *)F.sm, F.T.d[0], F.stage[0]);
             }
             HIPCHK(hipGetLastError());
-            hipLaunchKernelGGL((k_galerkin<0, double>), dim3(nthr(g2)), dim3(256), 0, st, 1, g1, g2, (const long long *)nullptr, (const double *)F.stage[0], F.T.d[1], F.stage[1]);
-            hipLaunchKernelGGL((k_galerkin<0, double>), dim3(nthr(g3)), dim3(256), 0, st, 2, g2, g3, (const long long *)nullptr, (const double *)F.stage[1], F.T.d[2], C.sm);
+            hipLaunchKernelGGL((k_galerkin<0, SMT>), dim3(nthr(g2)), dim3(256), 0, st, 1, g1, g2, (const long long *)nullptr, (const SMT *)F.stage[0], F.T.d[1], F.stage[1]);
+            hipLaunchKernelGGL((k_galerkin<0, SMT>), dim3(nthr(g3)), dim3(256), 0, st, 2, g2, g3, (const long long *)nullptr, (const SMT *)F.stage[1], F.T.d[2], C.sm);
             HIPCHK(hipGetLastError());
-            hipLaunchKernelGGL(k_block_diag_sm, dim3(grid_for(C.nnode, 256)), dim3(256), 0, st, C.nnode, (const double *)C.sm, C.Dblk, C.Dinv);
+            hipLaunchKernelGGL(k_block_diag_sm, dim3(grid_for(C.nnode, 256)), dim3(256), 0, st, C.nnode, (const SMT *)C.sm, C.Dblk, C.Dinv);
             HIPCHK(hipGetLastError());
         }
         if (smoother == 1)
@@ -1190,8 +1193,9 @@ struct Op {
                 if ((rc = estimate_lmax(l))) return rc;
         MGLevel &B = s->mg.back();
         if (B.dense) {
-            std::vector<double> h((size_t)B.nnode * 1125), A((size_t)B.ndof * B.ndof, 0.0), inv;
-            HIPCHK(hipMemcpyAsync(h.data(), B.sm, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+            std::vector<SMT> h((size_t)B.nnode * 1125);
+            std::vector<double> A((size_t)B.ndof * B.ndof, 0.0), inv;
+            HIPCHK(hipMemcpyAsync(h.data(), B.sm, h.size() * sizeof(SMT), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
             for (long long n = 0; n < B.nnode; n++) {
                 int a[3] = {(int)(n / ((long long)B.nb[2] * B.nb[1])), (int)((n / B.nb[2]) % B.nb[1]), (int)(n % B.nb[2])};
@@ -1327,7 +1331,7 @@ struct Op {
         const double *res = b;  // with no pre-smoothing x = 0 and the residual is b
         if (nu_pre > 0) {
             if (l) {
-                hipLaunchKernelGGL(k_resid_sm, dim3(grid_for(L.nnode, 4)), dim3(256), 0, st, level_args(L), (const double *)L.sm, (const double *)x, b, L.r);
+                hipLaunchKernelGGL(k_resid_sm, dim3(grid_for(L.nnode, 4)), dim3(256), 0, st, level_args(L), (const SMT *)L.sm, (const double *)x, b, L.r);
                 HIPCHK(hipGetLastError());
             } else if ((rc = spmv(x, L.r, b))) {
                 return rc;
