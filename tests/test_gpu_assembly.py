@@ -151,3 +151,41 @@ def test_energy_gradient_on_gpu(oracle):
         torch.cuda.synchronize()
         assert abs((ep - em) / (2 * h) - R[k]) < 1e-6 * np.abs(R).max()
     assert abs(app.energy_dev(torch.tensor(U, device=dev).data_ptr()) - oracle.energy(pb, U)) < 1e-12 * abs(oracle.energy(pb, U))
+
+
+@pytest.mark.parametrize("nelem,nquad", [((1, 3, 2), 4), ((2, 1, 1), 2), ((1, 1, 1), 1), ((3, 2, 1), 3)])
+def test_ragged_meshes_and_quadrature(oracle, nelem, nquad):
+    """One-element directions and every Gauss order 1..4: pattern, residual
+    and both tangent forms against the oracle (chunks of Gauss points and
+    colour grids degenerate here)."""
+    say("ragged", nelem, nquad)
+    pb = P.stretch_problem(nelem, a=0.01, noise=0.1, nquad=nquad)
+    U = state(pb)
+    app = App(pb)
+    rowptr, col = app.pattern()
+    r0, c0 = oracle.pattern(pb)
+    assert np.array_equal(rowptr, r0) and np.array_equal(col, c0)
+    R = app.residual(U)
+    assert rel(R, oracle.residual(pb, U)) < TOL64
+    _, _, v0 = oracle.tangent(pb, U, rowptr, col)
+    assert rel(app.tangent(U), v0) < TOL64
+    assert rel(app.tangent(U, prec=L.IGAP4H_FP32), v0) < TOL32
+
+
+def test_getime_ragged_q2(oracle):
+    pb = P.reference_example(2, physics=P.GETIME, nquad=2)
+    pb.nelem = (2, 1, 2)
+    pb.knots = [P.uniform_knots(n) for n in pb.nelem]
+    rng = np.random.default_rng(7)
+    n = pb.ndof
+    U0 = rng.uniform(-1, 1, n) * 0.01
+    U1 = U0 + rng.uniform(-1, 1, n) * 0.001
+    U = U1 + rng.uniform(-1, 1, n) * 0.001
+    Uh = np.concatenate([U, U1, U0])
+    app = App(pb)
+    app.U_hist[:] = Uh
+    R = app.residual(U)
+    assert rel(R, oracle.residual(pb, Uh)) < TOL64
+    rowptr, col = app.pattern()
+    _, _, v0 = oracle.tangent(pb, Uh, rowptr, col)
+    assert rel(app.tangent(U), v0) < TOL64
