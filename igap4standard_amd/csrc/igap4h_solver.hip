@@ -132,8 +132,14 @@ __global__ __launch_bounds__(256) void k_spmv(SpArgs s, const VT *__restrict__ v
     }
 }
 
-__device__ __forceinline__ void inv3(const double A[9], double B[9])
+__device__ __forceinline__ void inv3(const double Ain[9], double B[9])
 {
+    // a dof whose row is empty (coarse functions supported only on Dirichlet
+    // dofs) gets a unit diagonal
+    double A[9];
+    for (int k = 0; k < 9; k++) A[k] = Ain[k];
+    for (int i = 0; i < 3; i++)
+        if (A[3 * i] == 0.0 && A[3 * i + 1] == 0.0 && A[3 * i + 2] == 0.0) A[4 * i] = 1.0;
     double c00 = A[4] * A[8] - A[5] * A[7], c01 = A[5] * A[6] - A[3] * A[8], c02 = A[3] * A[7] - A[4] * A[6];
     double det = A[0] * c00 + A[1] * c01 + A[2] * c02;
     if (det == 0.0 || !isfinite(det)) {
@@ -673,7 +679,8 @@ __global__ void k_block_diag_sm(long long nnode, const SMT *__restrict__ sm, dou
 // stencil storage on nbo (nbo[d] coarse).  One thread per (node, offset).
 template <int IN_CSR, class VT>
 __global__ __launch_bounds__(256) void k_galerkin(int d, int3 nbi, int3 nbo, const long long *__restrict__ rowptr,
-                                                  const VT *__restrict__ in, Xfer x, SMT *__restrict__ out)
+                                                  const VT *__restrict__ in, Xfer x, SMT *__restrict__ out,
+                                                  const unsigned char *__restrict__ dmask)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int nbi_[3] = {nbi.x, nbi.y, nbi.z}, nbo_[3] = {nbo.x, nbo.y, nbo.z};
@@ -712,10 +719,14 @@ __global__ __launch_bounds__(256) void k_galerkin(int d, int3 nbi, int3 nbo, con
                     const long long L = 3LL * wd[0] * wd[1] * wd[2];
                     const long long pos = rowptr[3 * nin] +
                         3LL * (((Ai[0] + Di[0] - lo[0]) * wd[1] + (Ai[1] + Di[1] - lo[1])) * wd[2] + (Ai[2] + Di[2] - lo[2]));
+                    // Dirichlet rows and columns (identity in K) are left out of the
+                    // coarse operators: the coarse problem sees them as removed dofs
+                    const long long ncl = ((long long)(Ai[0] + Di[0]) * nbi_[1] + (Ai[1] + Di[1])) * nbi_[2] + (Ai[2] + Di[2]);
 #pragma unroll
                     for (int i = 0; i < 3; i++)
 #pragma unroll
-                        for (int j = 0; j < 3; j++) acc[3 * i + j] += w * (double)in[pos + i * L + j];
+                        for (int j = 0; j < 3; j++)
+                            if (!dmask || !(dmask[3 * nin + i] | dmask[3 * ncl + j])) acc[3 * i + j] += w * (double)in[pos + i * L + j];
                 } else {
                     const int oi = ((Di[0] + 2) * 5 + (Di[1] + 2)) * 5 + (Di[2] + 2);
                     const VT *v = in + nin * 1125 + oi * 3;
@@ -735,7 +746,8 @@ __global__ __launch_bounds__(256) void k_galerkin(int d, int3 nbi, int3 nbo, con
 }
 
 // bc = P^T rf (restriction), one thread per coarse node
-__global__ void k_restrict(Xfer3 x, const double *__restrict__ rf, double *__restrict__ bc)
+// (dmask: the fine residual's Dirichlet entries are left out)
+__global__ void k_restrict(Xfer3 x, const double *__restrict__ rf, double *__restrict__ bc, const unsigned char *__restrict__ dmask)
 {
     const long long nc = (long long)x.nbc[0] * x.nbc[1] * x.nbc[2];
     const long long C = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -754,9 +766,15 @@ __global__ void k_restrict(Xfer3 x, const double *__restrict__ rf, double *__res
                 double w = w1 * x.d[2].c2f_w[A[2] * MGW + k2];
                 if (w == 0.0) continue;
                 long long f = (((long long)(l0 + k0)) * x.nbf[1] + (l1 + k1)) * x.nbf[2] + (l2 + k2);
-                s0 += w * rf[3 * f];
-                s1 += w * rf[3 * f + 1];
-                s2 += w * rf[3 * f + 2];
+                if (dmask) {
+                    s0 += dmask[3 * f] ? 0.0 : w * rf[3 * f];
+                    s1 += dmask[3 * f + 1] ? 0.0 : w * rf[3 * f + 1];
+                    s2 += dmask[3 * f + 2] ? 0.0 : w * rf[3 * f + 2];
+                } else {
+                    s0 += w * rf[3 * f];
+                    s1 += w * rf[3 * f + 1];
+                    s2 += w * rf[3 * f + 2];
+                }
             }
         }
     }
@@ -766,7 +784,8 @@ __global__ void k_restrict(Xfer3 x, const double *__restrict__ rf, double *__res
 }
 
 // xf += P xc (prolongation), one thread per fine node
-__global__ void k_prolong_add(Xfer3 x, const double *__restrict__ xc, double *__restrict__ xf)
+// (dmask: no correction on the fine Dirichlet dofs)
+__global__ void k_prolong_add(Xfer3 x, const double *__restrict__ xc, double *__restrict__ xf, const unsigned char *__restrict__ dmask)
 {
     const long long nf = (long long)x.nbf[0] * x.nbf[1] * x.nbf[2];
     const long long F = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -791,9 +810,15 @@ __global__ void k_prolong_add(Xfer3 x, const double *__restrict__ xc, double *__
             }
         }
     }
-    xf[3 * F] += s0;
-    xf[3 * F + 1] += s1;
-    xf[3 * F + 2] += s2;
+    if (dmask) {
+        if (!dmask[3 * F]) xf[3 * F] += s0;
+        if (!dmask[3 * F + 1]) xf[3 * F + 1] += s1;
+        if (!dmask[3 * F + 2]) xf[3 * F + 2] += s2;
+    } else {
+        xf[3 * F] += s0;
+        xf[3 * F + 1] += s1;
+        xf[3 * F + 2] += s2;
+    }
 }
 
 // x = Ainv b (dense n x n), one wave per row
@@ -1176,14 +1201,15 @@ struct Op {
             int3 g3 = make_int3(C.nb[0], C.nb[1], C.nb[2]);
             auto nthr = [](int3 g) { return grid_for((long long)g.x * g.y * g.z * 125, 256); };
             if (l == 0) {
-                if (f32) hipLaunchKernelGGL((k_galerkin<1, float>), dim3(nthr(g1)), dim3(256), 0, st, 0, g0, g1, sp.rowptr, (const float *)vals, F.T.d[0], F.stage[0]);
-                else hipLaunchKernelGGL((k_galerkin<1, double>), dim3(nthr(g1)), dim3(256), 0, st, 0, g0, g1, sp.rowptr, (const double *)vals, F.T.d[0], F.stage[0]);
+                const unsigned char *dm = c->d_mask;
+                if (f32) hipLaunchKernelGGL((k_galerkin<1, float>), dim3(nthr(g1)), dim3(256), 0, st, 0, g0, g1, sp.rowptr, (const float *)vals, F.T.d[0], F.stage[0], dm);
+                else hipLaunchKernelGGL((k_galerkin<1, double>), dim3(nthr(g1)), dim3(256), 0, st, 0, g0, g1, sp.rowptr, (const double *)vals, F.T.d[0], F.stage[0], dm);
             } else {
-                hipLaunchKernelGGL((k_galerkin<0, SMT>), dim3(nthr(g1)), dim3(256), 0, st, 0, g0, g1, (const long long *)nullptr, (const SMT *)F.sm, F.T.d[0], F.stage[0]);
+                hipLaunchKernelGGL((k_galerkin<0, SMT>), dim3(nthr(g1)), dim3(256), 0, st, 0, g0, g1, (const long long *)nullptr, (const SMT *)F.sm, F.T.d[0], F.stage[0], (const unsigned char *)nullptr);
             }
             HIPCHK(hipGetLastError());
-            hipLaunchKernelGGL((k_galerkin<0, SMT>), dim3(nthr(g2)), dim3(256), 0, st, 1, g1, g2, (const long long *)nullptr, (const SMT *)F.stage[0], F.T.d[1], F.stage[1]);
-            hipLaunchKernelGGL((k_galerkin<0, SMT>), dim3(nthr(g3)), dim3(256), 0, st, 2, g2, g3, (const long long *)nullptr, (const SMT *)F.stage[1], F.T.d[2], C.sm);
+            hipLaunchKernelGGL((k_galerkin<0, SMT>), dim3(nthr(g2)), dim3(256), 0, st, 1, g1, g2, (const long long *)nullptr, (const SMT *)F.stage[0], F.T.d[1], F.stage[1], (const unsigned char *)nullptr);
+            hipLaunchKernelGGL((k_galerkin<0, SMT>), dim3(nthr(g3)), dim3(256), 0, st, 2, g2, g3, (const long long *)nullptr, (const SMT *)F.stage[1], F.T.d[2], C.sm, (const unsigned char *)nullptr);
             HIPCHK(hipGetLastError());
             hipLaunchKernelGGL(k_block_diag_sm, dim3(grid_for(C.nnode, 256)), dim3(256), 0, st, C.nnode, (const SMT *)C.sm, C.Dblk, C.Dinv);
             HIPCHK(hipGetLastError());
@@ -1207,6 +1233,11 @@ struct Op {
                         for (int j = 0; j < 3; j++)
                             A[(size_t)(3 * n + i) * B.ndof + 3 * m + j] = h[(size_t)n * 1125 + i * 375 + o * 3 + j];
                 }
+            }
+            for (long long r = 0; r < B.ndof; r++) {  // dofs removed with the Dirichlet rows
+                bool empty = true;
+                for (long long q = 0; q < B.ndof && empty; q++) empty = A[(size_t)r * B.ndof + q] == 0.0;
+                if (empty) A[(size_t)r * B.ndof + r] = 1.0;
             }
             if (!dense_inverse(A, (int)B.ndof, inv)) return fail(IGAP4H_EINVAL, "multigrid: coarsest operator is singular");
             HIPCHK(hipMemcpyAsync(B.Ainv, inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice, st));
@@ -1316,10 +1347,10 @@ struct Op {
                 hipLaunchKernelGGL(k_zero, dim3(vgrid(n)), dim3(256), 0, st, x, n);
                 HIPCHK(hipGetLastError());
             }
-            hipLaunchKernelGGL(k_restrict, dim3(grid_for(C.nnode, 256)), dim3(256), 0, st, L.T, res, C.b);
+            hipLaunchKernelGGL(k_restrict, dim3(grid_for(C.nnode, 256)), dim3(256), 0, st, L.T, res, C.b, l ? (const unsigned char *)nullptr : (const unsigned char *)c->d_mask);
             HIPCHK(hipGetLastError());
             if ((rc = vcycle(l + 1, C.b, C.x))) return rc;
-            hipLaunchKernelGGL(k_prolong_add, dim3(grid_for(L.nnode, 256)), dim3(256), 0, st, L.T, (const double *)C.x, x);
+            hipLaunchKernelGGL(k_prolong_add, dim3(grid_for(L.nnode, 256)), dim3(256), 0, st, L.T, (const double *)C.x, x, l ? (const unsigned char *)nullptr : (const unsigned char *)c->d_mask);
             HIPCHK(hipGetLastError());
             if (nu_post > 0) return cheb(l, b, x, nu_post, false, false);
             return 0;
@@ -1338,10 +1369,10 @@ struct Op {
             }
             res = L.r;
         }
-        hipLaunchKernelGGL(k_restrict, dim3(grid_for(C.nnode, 256)), dim3(256), 0, st, L.T, res, C.b);
+        hipLaunchKernelGGL(k_restrict, dim3(grid_for(C.nnode, 256)), dim3(256), 0, st, L.T, res, C.b, l ? (const unsigned char *)nullptr : (const unsigned char *)c->d_mask);
         HIPCHK(hipGetLastError());
         if ((rc = vcycle(l + 1, C.b, C.x))) return rc;
-        hipLaunchKernelGGL(k_prolong_add, dim3(grid_for(L.nnode, 256)), dim3(256), 0, st, L.T, (const double *)C.x, x);
+        hipLaunchKernelGGL(k_prolong_add, dim3(grid_for(L.nnode, 256)), dim3(256), 0, st, L.T, (const double *)C.x, x, l ? (const unsigned char *)nullptr : (const unsigned char *)c->d_mask);
         HIPCHK(hipGetLastError());
         for (int k = 0; k < nu_post; k++)
             if ((rc = gs_sweep(l, true, b, x))) return rc;
