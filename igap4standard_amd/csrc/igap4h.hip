@@ -112,11 +112,12 @@ __device__ __forceinline__ void fill_phi(const MeshArgs &m, const int e[3], doub
 }
 
 // slot values of the basis function A = 9 ax + 3 ay + az at point (qx,qy,qz)
-__device__ __forceinline__ void basis_slots(const double *sphi, int qx, int qy, int qz, int A, double n[10])
+template <class R>
+__device__ __forceinline__ void basis_slots(const R *sphi, int qx, int qy, int qz, int A, R n[10])
 {
-    const double *px = sphi + (0 * 4 + qx) * 9 + (A / 9) * 3;
-    const double *py = sphi + (1 * 4 + qy) * 9 + ((A / 3) % 3) * 3;
-    const double *pz = sphi + (2 * 4 + qz) * 9 + (A % 3) * 3;
+    const R *px = sphi + (0 * 4 + qx) * 9 + (A / 9) * 3;
+    const R *py = sphi + (1 * 4 + qy) * 9 + ((A / 3) % 3) * 3;
+    const R *pz = sphi + (2 * 4 + qz) * 9 + (A % 3) * 3;
     n[0] = px[0] * py[0] * pz[0];
     n[1] = px[1] * py[0] * pz[0];
     n[2] = px[0] * py[1] * pz[0];
@@ -168,6 +169,69 @@ __device__ __forceinline__ void interp_slots(const MeshArgs &m, const double *sU
     *w = m.wq[qx] * m.wq[qy] * m.wq[qz] * hh[0] * hh[1] * hh[2];
 }
 
+// u-slots of one element at all Gauss points by sum factorisation over the
+// tensor-product basis, all threads of the block: z contracted first, then y,
+// then x (9 + 18 + 30 products per point instead of 27 x 40).  Scratch
+// (double): V1 [i][ax][ay][dz][qz] 324, V2 [i][ax][c][qy][qz] 864 with
+// c = (dy,dz) in (00,10,01,20,11,02), out [30 slots][qx*16 + qy*4 + qz] 1920.
+// Ends with a barrier.  Periodic F0 terms and weights are left to the caller.
+constexpr int SF_SCRATCH = 324 + 864 + 1920;
+__device__ __forceinline__ void interp_slots_sf(int nq, const double *sU, const double *sphi, double *scr, int tid,
+                                                int nthr)
+{
+    double *V1 = scr, *V2 = scr + 324, *uo = scr + 324 + 864;
+    for (int idx = tid; idx < 324; idx += nthr) {
+        const int qz = idx & 3, dz = (idx >> 2) % 3, r = idx / 12; // r = (i*3 + ax)*3 + ay
+        if (qz < nq) {
+            const int ay = r % 3, ax = (r / 3) % 3, i = r / 9;
+            const double *z = sphi + (8 + qz) * 9 + dz;
+            const double *u = sU + 3 * (9 * ax + 3 * ay) + i;
+            V1[idx] = z[0] * u[0] + z[3] * u[3] + z[6] * u[6];
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < 864; idx += nthr) {
+        const int qz = idx & 3, qy = (idx >> 2) & 3, c = (idx >> 4) % 6, r = idx / 96; // r = i*3 + ax
+        if (qy < nq && qz < nq) {
+            const int dy = (0x012010 >> (4 * c)) & 15, dz = (0x210100 >> (4 * c)) & 15;
+            const double *y = sphi + (4 + qy) * 9 + dy;
+            const double *v = V1 + (r * 9 + dz) * 4 + qz; // ay stride 12
+            V2[idx] = y[0] * v[0] + y[3] * v[12] + y[6] * v[24];
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < 1920; idx += nthr) {
+        const int q = idx & 63, sl = idx >> 6, i = sl / 10, sv = sl % 10;
+        const int qx = q >> 4, qy = (q >> 2) & 3, qz = q & 3;
+        if (qx < nq && qy < nq && qz < nq) {
+            const int dx = (int)((0x0001120010ULL >> (4 * sv)) & 15), c = (int)((0x5432102100ULL >> (4 * sv)) & 15);
+            const double *x = sphi + qx * 9 + dx;
+            const double *v = V2 + ((i * 3) * 6 + c) * 16 + qy * 4 + qz; // ax stride 96
+            uo[idx] = x[0] * v[0] + x[3] * v[96] + x[6] * v[192];
+        }
+    }
+    __syncthreads();
+}
+
+// periodic F0 terms of one point (snesiga_Residual.c:125-144) and its weight
+__device__ __forceinline__ void point_affine(const MeshArgs &m, const int e[3], int qx, int qy, int qz, double u[30],
+                                             double *w)
+{
+    double xq[3], hh[3];
+    int q[3] = {qx, qy, qz};
+    for (int d = 0; d < 3; d++) {
+        const double *k = m.knot[d] + e[d];
+        double xi = m.cq[q[d]];
+        xq[d] = (-xi + 1.0) * k[2] + xi * k[3];
+        hh[d] = k[3] - k[2];
+    }
+    for (int i = 0; i < 3; i++) {
+        u[10 * i] += m.pp[3 * i] * xq[0] + m.pp[3 * i + 1] * xq[1] + m.pp[3 * i + 2] * xq[2];
+        for (int d = 0; d < 3; d++) u[10 * i + 1 + d] += m.pp[3 * i + d];
+    }
+    *w = m.wq[qx] * m.wq[qy] * m.wq[qz] * hh[0] * hh[1] * hh[2];
+}
+
 __device__ __forceinline__ Coef load_coef(const double *coef, long long e)
 {
     const double *c = coef + 6 * e;
@@ -177,8 +241,12 @@ __device__ __forceinline__ Coef load_coef(const double *coef, long long e)
 }
 
 // ============================================================== residual
-// One wave per element, lane = Gauss point.  Writes the element vector
-// R_e[81] (snesiga_Residual.c:104-160) to Re; k_gather sums it per dof.
+// One wave per element.  The u-slots at the Gauss points come from a sum-
+// factorised interpolation (interp_slots_sf), the pointwise stress runs with
+// lane = Gauss point, and the weak-form projection
+//   R_e[3A + i] = sum_q w_q sum_s N_{A,s}(q) r_{i,s}(q)   (snesiga_Residual.c:147-159)
+// is sum-factorised the other way: x contracted first, then y, then z.
+// Writes R_e[81] to Re; k_gather sums it per dof.
 template <int PK>
 __device__ __forceinline__ void residual_elem_body(const MeshArgs &m, const double *__restrict__ Uh,
                                                    const double *__restrict__ coef, double *__restrict__ Re, long long e)
@@ -186,7 +254,7 @@ __device__ __forceinline__ void residual_elem_body(const MeshArgs &m, const doub
     constexpr int NH = PhysT<PK>::NH;
     __shared__ double sU[NH * 81];
     __shared__ double sphi[3 * 4 * 9];
-    __shared__ double sred[64 * 81];
+    __shared__ double sScr[SF_SCRATCH];
     const int tid = threadIdx.x;
     int ev[3];
     ev[2] = (int)(e % m.ne[2]);
@@ -199,16 +267,23 @@ __device__ __forceinline__ void residual_elem_body(const MeshArgs &m, const doub
     }
     fill_phi(m, ev, sphi, tid, 64);
     __syncthreads();
-    const int nqp = m.nq * m.nq * m.nq;
+    const int nq = m.nq, nqp = nq * nq * nq;
     const Coef c = load_coef(coef, e);
+    const int qx = tid / (nq * nq), qy = (tid / nq) % nq, qz = tid % nq, qi = qx * 16 + qy * 4 + qz;
+    double u[NH][30];
+    for (int h = 0; h < NH; h++) {
+        interp_slots_sf(nq, sU + h * 81, sphi, sScr, tid, 64);
+        if (tid < nqp) {
+#pragma unroll
+            for (int sl = 0; sl < 30; sl++) u[h][sl] = sScr[324 + 864 + sl * 64 + qi];
+        }
+        __syncthreads();
+    }
     double r[30];
     for (int s = 0; s < 30; s++) r[s] = 0.0;
-    int qx = 0, qy = 0, qz = 0;
     double w = 0.0;
     if (tid < nqp) {
-        qx = tid / (m.nq * m.nq); qy = (tid / m.nq) % m.nq; qz = tid % m.nq;
-        double u[NH][30];
-        interp_slots<NH>(m, sU, sphi, ev, qx, qy, qz, u, &w);
+        for (int h = 0; h < NH; h++) point_affine(m, ev, qx, qy, qz, u[h], &w);
         if (PK == 0) {
             residual_grad<double>(u[0], c, r);
         } else {
@@ -224,24 +299,58 @@ __device__ __forceinline__ void residual_elem_body(const MeshArgs &m, const doub
                        (m.rho * (u[0][s] + u[NH - 1][s] - 2 * u[1 % NH][s])) / (m.dt * m.dt);
             }
         }
-    }
-    // per-lane projection w sum_s N_{A,s} r_{i,s}, then a column sum over lanes
-    for (int A = 0; A < 27; A++) {
-        double n[10];
-        if (tid < nqp) basis_slots(sphi, qx, qy, qz, A, n);
-        for (int i = 0; i < 3; i++) {
-            double t = 0.0;
-            if (tid < nqp) {
+        // weighted point values, [30 slots][qx*16 + qy*4 + qz]
 #pragma unroll
-                for (int s = 0; s < 10; s++) t += n[s] * r[10 * i + s];
+        for (int sl = 0; sl < 30; sl++) sScr[sl * 64 + qi] = w * r[sl];
+    }
+    __syncthreads();
+    double *rw = sScr, *P1 = sScr + 1920, *P2 = sScr + 1920 + 864;
+    // x: P1[i][ax][c][qy][qz] = sum_qx sum_{s: (dy,dz)(s) = c} X(qx, ax, dx(s)) rw[10i + s]
+    //    c = (dy,dz) in (00,10,01,20,11,02); slots of c with dx = 0, 1, 2:
+    //    (0,1,4) (2,5,-) (3,6,-) (7,-,-) (8,-,-) (9,-,-)
+    for (int idx = tid; idx < 864; idx += 64) {
+        const int qz2 = idx & 3, qy2 = (idx >> 2) & 3, cc = (idx >> 4) % 6, r2 = idx / 96; // r2 = i*3 + ax
+        if (qy2 < nq && qz2 < nq) {
+            const int i = r2 / 3, ax = r2 % 3;
+            const int s0 = 10 * i + ((0x987320 >> (4 * cc)) & 15), s1 = 10 * i + ((0x000651 >> (4 * cc)) & 15), s2 = 10 * i + 4;
+            const double m1 = cc < 3 ? 1.0 : 0.0, m2 = cc == 0 ? 1.0 : 0.0;
+            double t = 0.0;
+            for (int qx2 = 0; qx2 < nq; qx2++) {
+                const double *x = sphi + qx2 * 9 + ax * 3;
+                const int o = qx2 * 16 + qy2 * 4 + qz2;
+                t += x[0] * rw[s0 * 64 + o] + m1 * (x[1] * rw[s1 * 64 + o]) + m2 * (x[2] * rw[s2 * 64 + o]);
             }
-            sred[tid * 81 + 3 * A + i] = w * t;
+            P1[idx] = t;
         }
     }
     __syncthreads();
+    // y: P2[i][ax][ay][dz][qz] = sum_qy sum_{c: dz(c) = dz} Y(qy, ay, dy(c)) P1[i][ax][c][qy][qz];
+    //    combos of dz with dy = 0, 1, 2: (0,1,3) (2,4,-) (5,-,-)
+    for (int idx = tid; idx < 324; idx += 64) {
+        const int qz2 = idx & 3, dz = (idx >> 2) % 3, r3 = idx / 12; // r3 = (i*3 + ax)*3 + ay
+        if (qz2 < nq) {
+            const int ay = r3 % 3, r2 = r3 / 3;
+            const int c0 = (0x520 >> (4 * dz)) & 15, c1 = (0x041 >> (4 * dz)) & 15, c2 = 3;
+            const double m1 = dz < 2 ? 1.0 : 0.0, m2 = dz == 0 ? 1.0 : 0.0;
+            double t = 0.0;
+            for (int qy2 = 0; qy2 < nq; qy2++) {
+                const double *y = sphi + (4 + qy2) * 9 + ay * 3;
+                const double *p = P1 + r2 * 96 + qy2 * 4 + qz2;
+                t += y[0] * p[c0 * 16] + m1 * (y[1] * p[c1 * 16]) + m2 * (y[2] * p[c2 * 16]);
+            }
+            P2[idx] = t;
+        }
+    }
+    __syncthreads();
+    // z: R_e[3A + i] = sum_qz sum_dz Z(qz, az, dz) P2[i][ax][ay][dz][qz]
     for (int col = tid; col < 81; col += 64) {
+        const int A = col / 3, i = col % 3, ax = A / 9, ay = (A / 3) % 3, az = A % 3;
+        const double *p = P2 + ((i * 3 + ax) * 3 + ay) * 12;
         double t = 0.0;
-        for (int l = 0; l < nqp; l++) t += sred[l * 81 + col];
+        for (int qz2 = 0; qz2 < nq; qz2++) {
+            const double *z = sphi + (8 + qz2) * 9 + az * 3;
+            t += z[0] * p[qz2] + z[1] * p[4 + qz2] + z[2] * p[8 + qz2];
+        }
         Re[e * 81 + col] = t;
     }
 }
@@ -528,37 +637,42 @@ struct GeTile {
     static constexpr int NTS = KCP * 192;             // T strip
 };
 
-__device__ __forceinline__ void store_compact_soa(double *sC, int qp, const Compact<double> &cp)
+// compact data of one Gauss point, structure of arrays (stride 64), rounded
+// once to the tangent's arithmetic type ACC
+template <class ACC>
+__device__ __forceinline__ void store_compact_soa(ACC *sC, int qp, const Compact<double> &cp)
 {
     const double *v = reinterpret_cast<const double *>(&cp);
 #pragma unroll
-    for (int f = 0; f < NCMP; f++) sC[f * 64 + qp] = v[f];
+    for (int f = 0; f < NCMP; f++) sC[f * 64 + qp] = (ACC)v[f];
 }
-__device__ __forceinline__ void load_compact_soa(const double *sC, int qp, Compact<double> &cp)
+template <class ACC>
+__device__ __forceinline__ void load_compact_soa(const ACC *sC, int qp, Compact<ACC> &cp)
 {
-    double *v = reinterpret_cast<double *>(&cp);
+    ACC *v = reinterpret_cast<ACC *>(&cp);
 #pragma unroll
     for (int f = 0; f < NCMP; f++) v[f] = sC[f * 64 + qp];
 }
 
-// T_ij(q, B) for dof block j = J and rows i of [I0, I1]
+// T_ij(q, B) for dof block j = J and rows i of [I0, I1], in the arithmetic
+// of ACC (FP64, or FP32 for the FP32 tangent whose MFMA operands are FP32)
 template <int J, int I0, int I1, class ACC>
-__device__ __forceinline__ void ge_task(const double *sCmp, const double *sphi, const int *sQ, const double *sw,
+__device__ __forceinline__ void ge_task(const ACC *sCmp, const ACC *sphi, const int *sQ, const double *sw,
                                         ACC *sT, const Coef &c, int ql, int B, int qp, int nqp)
 {
-    double out[J + 1][10];
+    ACC out[J + 1][10];
     if (qp < nqp) {
-        double n[10];
+        ACC n[10];
         int q = sQ[qp];
-        basis_slots(sphi, q & 15, (q >> 4) & 15, (q >> 8) & 15, B, n);
-        Compact<double> cp;
-        load_compact_soa(sCmp, qp, cp);
-        tangent_apply<double, J, I0, I1>(cp, c, n, 0.0, sw[qp], out);
+        basis_slots<ACC>(sphi, q & 15, (q >> 4) & 15, (q >> 8) & 15, B, n);
+        Compact<ACC> cp;
+        load_compact_soa<ACC>(sCmp, qp, cp);
+        tangent_apply<ACC, J, I0, I1, ACC>(cp, c, n, (ACC)0, (ACC)sw[qp], out);
     } else {
 #pragma unroll
         for (int i = I0; i <= I1; i++)
 #pragma unroll
-            for (int s = 0; s < 10; s++) out[i][s] = 0.0;
+            for (int s = 0; s < 10; s++) out[i][s] = (ACC)0;
     }
 #pragma unroll
     for (int i = I0; i <= I1; i++) {
@@ -576,19 +690,22 @@ __device__ __forceinline__ void tangent_ge_body(const MeshArgs &m, const double 
     using G = GeTile<QC>;
     using V = typename V4<ACC>::t;
     // region: [compact SoA | T strip], reused for the element matrix
-    constexpr int CMP_BYTES = NCMP * 64 * 8;
-    constexpr int OP_BYTES = (G::NKA + G::NTS) * (int)sizeof(ACC);
+    constexpr int CMP_BYTES = NCMP * 64 * (int)sizeof(ACC);
+    constexpr int OP_BYTES0 = (G::NKA + G::NTS) * (int)sizeof(ACC);
+    // the operand area doubles as the interpolation scratch of the prologue
+    constexpr int OP_BYTES = OP_BYTES0 > SF_SCRATCH * 8 ? OP_BYTES0 : SF_SCRATCH * 8;
     constexpr int KE_BYTES = 81 * 81 * (int)sizeof(ACC);
     constexpr int REGION = (CMP_BYTES + OP_BYTES) > KE_BYTES ? (CMP_BYTES + OP_BYTES) : KE_BYTES;
     __shared__ __attribute__((aligned(16))) unsigned char sRegion[REGION];
     __shared__ double sU[81];
     __shared__ double sphi[3 * 4 * 9];
+    __shared__ ACC sphiA[3 * 4 * 9]; // the same tables in the arithmetic type
     __shared__ double sw[64];
     __shared__ int sQ[64];
     __shared__ long long sRowBase[81];
     __shared__ int sRowSx[81], sRowSy[81];
     __shared__ unsigned char sMask[81];
-    double *sCmp = reinterpret_cast<double *>(sRegion);
+    ACC *sCmp = reinterpret_cast<ACC *>(sRegion);
     ACC *sNk = reinterpret_cast<ACC *>(sRegion + CMP_BYTES);
     ACC *sT = sNk + G::NKA;
 
@@ -611,22 +728,29 @@ __device__ __forceinline__ void tangent_ge_body(const MeshArgs &m, const double 
         int qp = tid - 128;
         sQ[qp] = qp < nqp ? ((qp / (nq * nq)) | (((qp / nq) % nq) << 4) | ((qp % nq) << 8)) : 0;
     }
-    // padded contraction rows of the T strip stay zero
-    for (int idx = tid; idx < (G::KCP - G::KC) * 192; idx += 256) sT[G::KC * 192 + idx] = (ACC)0;
     fill_phi(m, ev, sphi, tid, 256);
     __syncthreads();
     const Coef c = load_coef(coef, e);
 
+    double *sScr = reinterpret_cast<double *>(sRegion + CMP_BYTES);
+    interp_slots_sf(nq, sU, sphi, sScr, tid, 256);
     if (tid < nqp) {
         int q = sQ[tid];
-        double u[1][30], w;
-        interp_slots<1>(m, sU, sphi, ev, q & 15, (q >> 4) & 15, (q >> 8) & 15, u, &w);
+        const int qx = q & 15, qy = (q >> 4) & 15, qz = (q >> 8) & 15, qi = qx * 16 + qy * 4 + qz;
+        double u[30], w;
+#pragma unroll
+        for (int sl = 0; sl < 30; sl++) u[sl] = sScr[324 + 864 + sl * 64 + qi];
+        point_affine(m, ev, qx, qy, qz, u, &w);
         Compact<double> cp;
-        compact_of<double>(u[0], c, cp);
-        store_compact_soa(sCmp, tid, cp);
+        compact_of<double>(u, c, cp);
+        store_compact_soa<ACC>(sCmp, tid, cp);
         sw[tid] = w;
+    } else if (tid >= 128 && tid < 128 + 3 * 4 * 9) {
+        sphiA[tid - 128] = (ACC)sphi[tid - 128];
     }
     __syncthreads();
+    // padded contraction rows of the T strip stay zero (after the scratch use)
+    for (int idx = tid; idx < (G::KCP - G::KC) * 192; idx += 256) sT[G::KC * 192 + idx] = (ACC)0;
 
     V acc[6];
 #pragma unroll
@@ -646,7 +770,7 @@ __device__ __forceinline__ void tangent_ge_body(const MeshArgs &m, const double 
     const int A0 = lane & 15, A1 = A0 + 16, krow = lane >> 4;
     const int ox0 = (A0 / 9) * 3, oy0 = ((A0 / 3) % 3) * 3, oz0 = (A0 % 3) * 3;
     const int ox1 = (A1 < 27 ? A1 / 9 : 0) * 3, oy1 = (A1 < 27 ? (A1 / 3) % 3 : 0) * 3, oz1 = (A1 < 27 ? A1 % 3 : 0) * 3;
-    const double a1mask = A1 < 27 ? 1.0 : 0.0;
+    const ACC a1mask = A1 < 27 ? (ACC)1 : (ACC)0;
 
     const int nchunk = (nqp + QC - 1) / QC;
     for (int ch = 0; ch < nchunk; ch++) {
@@ -655,35 +779,40 @@ __device__ __forceinline__ void tangent_ge_body(const MeshArgs &m, const double 
         // (i = 0; j = 0), wave 3 (i = 0; j = 2)
         for (int t = lane; t < QC * 27; t += 64) {
             int ql = t / 27, B = t % 27, qp = ch * QC + ql;
-            if (wave == 0) ge_task<2, 1, 2, ACC>(sCmp, sphi, sQ, sw, sT, c, ql, B, qp, nqp);
-            else if (wave == 1) ge_task<1, 0, 1, ACC>(sCmp, sphi, sQ, sw, sT, c, ql, B, qp, nqp);
-            else if (wave == 2) ge_task<0, 0, 0, ACC>(sCmp, sphi, sQ, sw, sT, c, ql, B, qp, nqp);
-            else ge_task<2, 0, 0, ACC>(sCmp, sphi, sQ, sw, sT, c, ql, B, qp, nqp);
+            if (wave == 0) ge_task<2, 1, 2, ACC>(sCmp, sphiA, sQ, sw, sT, c, ql, B, qp, nqp);
+            else if (wave == 1) ge_task<1, 0, 1, ACC>(sCmp, sphiA, sQ, sw, sT, c, ql, B, qp, nqp);
+            else if (wave == 2) ge_task<0, 0, 0, ACC>(sCmp, sphiA, sQ, sw, sT, c, ql, B, qp, nqp);
+            else ge_task<2, 0, 0, ACC>(sCmp, sphiA, sQ, sw, sT, c, ql, B, qp, nqp);
         }
         __syncthreads();
         const int ct0 = 3 * wave;
+        // all operands of the chunk first (branch-free, so that the LDS reads
+        // overlap), then the MFMA stream: A fragments N_{A,s}(q) from the
+        // per-element basis tables, B fragments from the T strip
+        constexpr int NKS = G::KCP / 4;
+        ACC fa0[NKS], fa1[NKS], fb[NKS][3];
 #pragma unroll
-        for (int ks = 0; ks < G::KCP / 4; ks++) {
+        for (int ks = 0; ks < NKS; ks++) {
             const int k = 4 * ks + krow;
-            // A fragments N_{A,s}(q) from the per-element basis tables
-            double v0 = 0.0, v1 = 0.0;
             const int ql = k / 9, sl = 1 + k % 9, qp = ch * QC + ql;
-            if (k < G::KC && qp < nqp) {
-                const int q = sQ[qp];
-                const double *px = sphi + (q & 15) * 9 + slot_d(sl, 0);
-                const double *py = sphi + (4 + ((q >> 4) & 15)) * 9 + slot_d(sl, 1);
-                const double *pz = sphi + (8 + ((q >> 8) & 15)) * 9 + slot_d(sl, 2);
-                v0 = px[ox0] * py[oy0] * pz[oz0];
-                v1 = a1mask * (px[ox1] * py[oy1] * pz[oz1]);
-            }
-            const ACC a0 = (ACC)v0, a1 = (ACC)v1;
+            const bool ok = k < G::KC && qp < nqp;
+            const int q = sQ[ok ? qp : 0];
+            const ACC *px = sphiA + (q & 15) * 9 + slot_d(sl, 0);
+            const ACC *py = sphiA + (4 + ((q >> 4) & 15)) * 9 + slot_d(sl, 1);
+            const ACC *pz = sphiA + (8 + ((q >> 8) & 15)) * 9 + slot_d(sl, 2);
+            const ACC okm = ok ? (ACC)1 : (ACC)0;
+            fa0[ks] = okm * (px[ox0] * py[oy0] * pz[oz0]);
+            fa1[ks] = (okm * a1mask) * (px[ox1] * py[oy1] * pz[oz1]);
+#pragma unroll
+            for (int t = 0; t < 3; t++) fb[ks][t] = sT[k * 192 + (ct0 + t) * 16 + (lane & 15)];
+        }
+#pragma unroll
+        for (int ks = 0; ks < NKS; ks++)
 #pragma unroll
             for (int t = 0; t < 3; t++) {
-                ACC bb = sT[k * 192 + (ct0 + t) * 16 + (lane & 15)];
-                acc[2 * t] = mfma16(a0, bb, acc[2 * t]);
-                acc[2 * t + 1] = mfma16(a1, bb, acc[2 * t + 1]);
+                acc[2 * t] = mfma16(fa0[ks], fb[ks][t], acc[2 * t]);
+                acc[2 * t + 1] = mfma16(fa1[ks], fb[ks][t], acc[2 * t + 1]);
             }
-        }
         if (sizeof(ACC) == 4) {
 #pragma unroll
             for (int t = 0; t < 6; t++)
@@ -727,7 +856,7 @@ __device__ __forceinline__ void tangent_ge_body(const MeshArgs &m, const double 
 }
 
 template <class ACC, int QC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_tangent_ge(MeshArgs m, const double *__restrict__ Uh,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(ACC) == 4 ? 3 : 2))) void k_tangent_ge(MeshArgs m, const double *__restrict__ Uh,
                                                     const double *__restrict__ coef,
                                                     const long long *__restrict__ rowptr,
                                                     const unsigned char *__restrict__ dmask, void *__restrict__ vals,

@@ -117,6 +117,7 @@ struct Num<Jet<N>> {
 
 // (1/2)(c0/2 + c1/3 + c2/4): averaged tangent of a degree-2 series
 IGAP4H_HD double avg_tangent(double x) { return x; }
+IGAP4H_HD float avg_tangent(float x) { return x; }
 template <int N>
 IGAP4H_HD double avg_tangent(const Jet<N> &x)
 {
@@ -310,36 +311,39 @@ IGAP4H_HD void compact_of(const T *u, const Coef &c, Compact<T> &cp)
 
 // T_ij[s, B] for i in [0, J], one basis function with slot values n[10].
 // Writes out[i][s] for s = 0..9 (value slot only when mass != 0).  J is a
-// compile-time constant so that out[][] stays in registers.
-template <class T, int JB, int IB0 = 0, int IB1 = JB>
-IGAP4H_HD void tangent_apply(const Compact<T> &cp, const Coef &c, const double n[10], double mass, double w,
-                             double out[JB + 1][10])
+// compile-time constant so that out[][] stays in registers.  R is the
+// arithmetic type of the product: double, or float for the FP32 tangent
+// (compact data rounded once from FP64; every constant is cast to R so that
+// no operation widens to double).
+template <class T, int JB, int IB0 = 0, int IB1 = JB, class R = double>
+IGAP4H_HD void tangent_apply(const Compact<T> &cp, const Coef &c, const R n[10], R mass, R w, R out[JB + 1][10])
 {
     // rows i of [IB0, IB1] only (a caller may split one j over several lanes)
     constexpr int j = JB;
+    const R half = R(0.5), third = R(1.0 / 3.0), e345 = R(c.e345), l2 = R(2.0 * c.lsq);
     // beta = sym(F_j^T grad n) in (00,11,22,01,02,12)
     T b[6];
     b[0] = cp.F[j][0] * n[1];
     b[1] = cp.F[j][1] * n[2];
     b[2] = cp.F[j][2] * n[3];
-    b[3] = (cp.F[j][1] * n[1] + cp.F[j][0] * n[2]) * 0.5;
-    b[4] = (cp.F[j][2] * n[1] + cp.F[j][0] * n[3]) * 0.5;
-    b[5] = (cp.F[j][2] * n[2] + cp.F[j][1] * n[3]) * 0.5;
+    b[3] = (cp.F[j][1] * n[1] + cp.F[j][0] * n[2]) * half;
+    b[4] = (cp.F[j][2] * n[1] + cp.F[j][0] * n[3]) * half;
+    b[5] = (cp.F[j][2] * n[2] + cp.F[j][1] * n[3]) * half;
     // Gamma (symmetric 3x3): normal block P b, shear e345 b
     T G[3][3];
     G[0][0] = cp.P[0] * b[0] + cp.P[1] * b[1] + cp.P[2] * b[2];
     G[1][1] = cp.P[1] * b[0] + cp.P[3] * b[1] + cp.P[4] * b[2];
     G[2][2] = cp.P[2] * b[0] + cp.P[4] * b[1] + cp.P[5] * b[2];
-    G[0][1] = G[1][0] = b[3] * c.e345;
-    G[0][2] = G[2][0] = b[4] * c.e345;
-    G[1][2] = G[2][1] = b[5] * c.e345;
+    G[0][1] = G[1][0] = b[3] * e345;
+    G[0][2] = G[2][0] = b[4] * e345;
+    G[1][2] = G[2][1] = b[5] * e345;
     // mu_IK = 2 le^2 dev_I (u_j,IK n_,I + F_jI n_,IK)
     T mu[3][3];
     for (int K = 0; K < 3; K++) {
         T nu[3];
         for (int I = 0; I < 3; I++) nu[I] = cp.H[j][hidx(I, K)] * n[1 + I] + cp.F[j][I] * n[4 + hidx(I, K)];
-        T m = (nu[0] + nu[1] + nu[2]) * (1.0 / 3.0);
-        for (int I = 0; I < 3; I++) mu[I][K] = (nu[I] - m) * (2.0 * c.lsq);
+        T m = (nu[0] + nu[1] + nu[2]) * third;
+        for (int I = 0; I < 3; I++) mu[I][K] = (nu[I] - m) * l2;
     }
     const T Sm[3][3] = {{cp.S[0], cp.S[3], cp.S[4]}, {cp.S[3], cp.S[1], cp.S[5]}, {cp.S[4], cp.S[5], cp.S[2]}};
 #pragma unroll
@@ -368,7 +372,7 @@ IGAP4H_HD void tangent_apply(const Compact<T> &cp, const Coef &c, const double n
             th[4] = th[4] + cp.tau[1][2] * n[2] + cp.tau[2][1] * n[3];
             th[5] = th[5] + cp.tau[2][2] * n[3];
         }
-        out[i][0] = (i == j) ? w * mass * n[0] : 0.0;
+        out[i][0] = (i == j) ? w * mass * n[0] : R(0);
         for (int J = 0; J < 3; J++) out[i][1 + J] = w * avg_tangent(tf[J]);
         for (int h = 0; h < 6; h++) out[i][4 + h] = w * avg_tangent(th[h]);
     }
