@@ -88,7 +88,10 @@ def main():
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--forms", default="fp64,mixed", help="comma list of fp64, mixed")
+    ap.add_argument("--forms", default="fp64,mixed,mixed_f16s",
+                    help="comma list of fp64, mixed, mixed_f16s / mixed_bf16s (the mixed form with the "
+                         "fine-level multigrid smoothing passes on a row-scaled binary16 / bfloat16 copy "
+                         "of the FP32 tangent)")
     ap.add_argument("--no-newton", action="store_true", help="assembly and SpMV only")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -130,12 +133,16 @@ def main():
     dR = torch.empty(app.ndof, device=dev, dtype=torch.float64)
     dK64 = torch.empty(app.nnz, device=dev, dtype=torch.float64)
     dK32 = torch.empty(app.nnz, device=dev, dtype=torch.float32)
-    forms = {"fp64": (L.IGAP4H_FP64, L.IGAP4H_VALS_F64, dK64), "mixed": (L.IGAP4H_FP32, L.IGAP4H_VALS_F32, dK32)}
+    forms = {"fp64": (L.IGAP4H_FP64, L.IGAP4H_VALS_F64, dK64), "mixed": (L.IGAP4H_FP32, L.IGAP4H_VALS_F32, dK32),
+             "mixed_f16s": (L.IGAP4H_FP32, L.IGAP4H_VALS_F32, dK32),
+             "mixed_bf16s": (L.IGAP4H_FP32, L.IGAP4H_VALS_F32, dK32)}
+    smooth = {"mixed_f16s": L.IGAP4H_SMOOTH_F16, "mixed_bf16s": L.IGAP4H_SMOOTH_BF16}
     wanted = [f for f in args.forms.split(",") if f in forms]
+    wanted_asm = [f for f in wanted if f not in smooth] or ["mixed"]  # the 16-bit forms assemble as mixed
 
     # ---- assembly alone (residual + tangent), HIP events on the launch stream
     asm = {}
-    for name in wanted:
+    for name in wanted_asm:
         prec, vt, dK = forms[name]
         for _ in range(2):
             app.residual_dev(dUh.data_ptr(), dR.data_ptr(), sp)
@@ -163,7 +170,7 @@ def main():
     spmv = {}
     x = torch.tensor(np.random.default_rng(1).standard_normal(app.ndof), device=dev)
     y = torch.empty_like(x)
-    for name in wanted:
+    for name in wanted_asm:
         prec, vt, dK = forms[name]
         app.tangent_dev(dUh.data_ptr(), dK.data_ptr(), prec, vt, sp)
         for _ in range(3):
@@ -188,7 +195,8 @@ def main():
         Uw = torch.empty_like(U0)
         for name in wanted:
             prec, vt, _ = forms[name]
-            o = app.newton_opts(rtol=1e-10, atol=0.0, dirichlet_lift=1, tangent_prec=prec, vals_type=vt)
+            o = app.newton_opts(rtol=1e-10, atol=0.0, dirichlet_lift=1, tangent_prec=prec, vals_type=vt,
+                                mg_smooth_vals=smooth.get(name, L.IGAP4H_SMOOTH_SAME))
             for _ in range(args.warmup):
                 Uw.copy_(U0)
                 app.newton_solve_dev(Uw.data_ptr(), o, sp)
@@ -212,8 +220,8 @@ def main():
                             "fnorm_hist": ["%.3e" % f for f in info["fnorm_hist"]]}
             say("newton", name, json.dumps(newton[name]))
 
-    main_asm = asm.get("fp64") or asm[wanted[0]]
-    main_sp = spmv.get("fp64") or spmv[wanted[0]]
+    main_asm = asm.get("fp64") or asm[wanted_asm[0]]
+    main_sp = spmv.get("fp64") or spmv[wanted_asm[0]]
     main_nt = newton.get("fp64") or (newton[wanted[0]] if newton else None)
     tangent_kernel = "k_tangent_ge<double,2>" if "fp64" in asm else "k_tangent_ge<float,4>"
     spmv_kernel = "k_spmv<double,0,0>" if "fp64" in spmv else "k_spmv<float,0,0>"
@@ -255,8 +263,9 @@ def main():
     if "mixed" in asm and "fp64" in asm:
         line["assembly"]["mixed"]["roofline_frac_f32"] = asm["mixed"]["tangent_tflops"] / PEAK["f32"]
         line["assembly"]["mixed"]["tangent_speedup_vs_fp64"] = asm["fp64"]["tangent_ms"] / asm["mixed"]["tangent_ms"]
-    if "mixed" in newton and "fp64" in newton:
-        line["newton"]["mixed_speedup_vs_fp64"] = newton["fp64"]["solve_s"] / newton["mixed"]["solve_s"]
+    for name in ("mixed", "mixed_f16s", "mixed_bf16s"):
+        if name in newton and "fp64" in newton:
+            line["newton"][name + "_speedup_vs_fp64"] = newton["fp64"]["solve_s"] / newton[name]["solve_s"]
     if not args.no_cpu:
         say("CPU baseline (budget %.0f s)" % args.cpu_budget)
         line["cpu_baseline"] = cpu_baseline(pb, U, args.cpu_budget)

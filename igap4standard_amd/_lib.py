@@ -61,6 +61,7 @@ IGAP4H_PHYS_GE, IGAP4H_PHYS_GETIME, IGAP4H_PHYS_FROM_TORDER = 0, 1, -1
 IGAP4H_PC_NONE, IGAP4H_PC_BJACOBI3, IGAP4H_PC_SGS, IGAP4H_PC_MG = 0, 1, 2, 3
 IGAP4H_KSP_GMRES, IGAP4H_KSP_CG = 0, 1
 IGAP4H_MG_GS, IGAP4H_MG_CHEBYSHEV = 0, 1
+IGAP4H_SMOOTH_SAME, IGAP4H_SMOOTH_BF16, IGAP4H_SMOOTH_F16 = 0, 1, 2
 
 
 class NewtonOpts(ctypes.Structure):
@@ -71,7 +72,8 @@ class NewtonOpts(ctypes.Structure):
                 ("ksp_guess", ctypes.c_int), ("pc", ctypes.c_int), ("line_search", ctypes.c_int),
                 ("stagnation", ctypes.c_int), ("dirichlet_lift", ctypes.c_int),
                 ("mg_pre", ctypes.c_int), ("mg_post", ctypes.c_int), ("mg_graph", ctypes.c_int),
-                ("mg_smoother", ctypes.c_int), ("lag_jacobian", ctypes.c_int), ("lag_persist", ctypes.c_int)]
+                ("mg_smoother", ctypes.c_int), ("lag_jacobian", ctypes.c_int), ("lag_persist", ctypes.c_int),
+                ("mg_smooth_vals", ctypes.c_int)]
 
 
 class NewtonInfo(ctypes.Structure):

@@ -132,6 +132,135 @@ __global__ __launch_bounds__(256) void k_spmv(SpArgs s, const VT *__restrict__ v
     }
 }
 
+// ---------------------------------------------------------------- 16-bit smoothing copies
+// Node n's three rows sit back to back from noff[n], each padded to an even
+// length Lp = L + (L & 1) with a zero, so a lane reads two values per 4-byte
+// load.  bfloat16 = the upper half of the FP32 bit pattern, rounded to nearest
+// even.  binary16 (F16 = 1): row r is divided by rs[r] = 2^e with
+// max_k |K_rk| < 2^e first, exactly, so every entry lies in (-1, 1).
+__host__ __device__ __forceinline__ int even_len(int L) { return L + (L & 1); }
+
+__device__ __forceinline__ unsigned short f32_to_bf16(float f)
+{
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+template <class VT, int F16>
+__global__ __launch_bounds__(256) void k_pack_16(SpArgs s, const VT *__restrict__ vals, const long long *__restrict__ noff,
+                                                 unsigned short *__restrict__ out, double *__restrict__ rs)
+{
+    const int lane = threadIdx.x & 63;
+    const long long node = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (node >= s.nnode) return;
+    int a[3], lo[3], w[3];
+    node_box(s, node, a, lo, w);
+    const int L = 3 * w[0] * w[1] * w[2], Lp = even_len(L);
+    const long long base = s.rowptr[3 * node], ob = noff[node];
+    for (int r = 0; r < 3; r++) {
+        const VT *row = vals + base + (long long)r * L;
+        double inv = 1.0;
+        if (F16) {
+            double m = 0.0;
+            for (int k = lane; k < L; k += 64) m = fmax(m, fabs((double)row[k]));
+            m = wave_max(m);
+            int e = 0;
+            (void)frexp(m, &e);  // m < 2^e
+            const double sc = m > 0.0 ? ldexp(1.0, e) : 1.0;
+            inv = 1.0 / sc;  // exact (power of two)
+            if (lane == 0) rs[3 * node + r] = sc;
+        }
+        for (int k = lane; k < Lp; k += 64) {
+            unsigned short h = 0;
+            if (k < L) {
+                if (F16) {
+                    _Float16 v = (_Float16)((double)row[k] * inv);
+                    h = __builtin_bit_cast(unsigned short, v);
+                } else {
+                    h = f32_to_bf16((float)row[k]);
+                }
+            }
+            out[ob + (long long)r * Lp + k] = h;
+        }
+    }
+}
+
+__device__ __forceinline__ float lo16(unsigned u, int f16)
+{
+    return f16 ? (float)__builtin_bit_cast(_Float16, (unsigned short)(u & 0xFFFFu)) : __uint_as_float(u << 16);
+}
+__device__ __forceinline__ float hi16(unsigned u, int f16)
+{
+    return f16 ? (float)__builtin_bit_cast(_Float16, (unsigned short)(u >> 16)) : __uint_as_float(u & 0xFFFF0000u);
+}
+
+// stencil column k of a node -> x index (k < L)
+__device__ __forceinline__ long long col_index(const SpArgs &s, const int lo[3], const int w[3], float r1, float r2, int k)
+{
+    int t = (k * 43691) >> 17;  // k / 3 for k < 131072
+    int j = k - 3 * t;
+    int t2 = (int)(((float)t + 0.5f) * r2);  // t / w2 (exact: t < 128, w2 <= 5)
+    int o2 = t - t2 * w[2];
+    int o0 = (int)(((float)t2 + 0.5f) * r1);
+    int o1 = t2 - o0 * w[1];
+    return 3 * (((long long)(lo[0] + o0) * s.nb[1] + (lo[1] + o1)) * s.nb[2] + (lo[2] + o2)) + j;
+}
+
+// y = K~ x (RES = 0) or y = b - K~ x (RES = 1) on a 16-bit copy, one wave
+// per node, two columns per lane and load
+template <int RES, int F16>
+__global__ __launch_bounds__(256) void k_spmv_16(SpArgs s, const unsigned *__restrict__ v32, const long long *__restrict__ noff,
+                                                 const double *__restrict__ rs, const double *__restrict__ x,
+                                                 const double *__restrict__ b, double *__restrict__ y)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const long long node = xcd_remap(blockIdx.x, gridDim.x) * 4 + wv;
+    if (node >= s.nnode) return;
+    int a[3], lo[3], w[3];
+    node_box(s, node, a, lo, w);
+    const int L = 3 * w[0] * w[1] * w[2], Lp = even_len(L);
+    const float r2 = 1.0f / (float)w[2], r1 = 1.0f / (float)w[1];
+    const long long ob = noff[node] >> 1;  // in 4-byte words
+    double xa[3], xb[3];
+    unsigned u0[3], u1[3], u2[3];
+#pragma unroll
+    for (int it = 0; it < 3; it++) {
+        const int p = lane + 64 * it, k = 2 * p;
+        xa[it] = xb[it] = 0.0;
+        u0[it] = u1[it] = u2[it] = 0u;
+        if (k < L) {
+            xa[it] = x[col_index(s, lo, w, r1, r2, k)];
+            if (k + 1 < L) xb[it] = x[col_index(s, lo, w, r1, r2, k + 1)];
+            u0[it] = v32[ob + p];
+            u1[it] = v32[ob + (Lp >> 1) + p];
+            u2[it] = v32[ob + Lp + p];
+        }
+    }
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int it = 0; it < 3; it++) {
+        s0 += (double)lo16(u0[it], F16) * xa[it] + (double)hi16(u0[it], F16) * xb[it];
+        s1 += (double)lo16(u1[it], F16) * xa[it] + (double)hi16(u1[it], F16) * xb[it];
+        s2 += (double)lo16(u2[it], F16) * xa[it] + (double)hi16(u2[it], F16) * xb[it];
+    }
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    if (lane == 0) {
+        double acc[3] = {s0, s1, s2};
+        if (F16)
+            for (int i = 0; i < 3; i++) acc[i] *= rs[3 * node + i];
+        for (int i = 0; i < 3; i++) y[3 * node + i] = RES ? b[3 * node + i] - acc[i] : acc[i];
+    }
+}
+
 __device__ __forceinline__ void inv3(const double Ain[9], double B[9])
 {
     // a dof whose row is empty (coarse functions supported only on Dirichlet
@@ -862,7 +991,12 @@ struct igap4h::Solver {
     hipGraphExec_t mg_exec = nullptr;
     const void *key_vals = nullptr;
     int key_f32 = -1, key_pre = -1, key_post = -1, key_smoother = -1;
+    int key_svt = -1;
     const void *pc_vals = nullptr;  // matrix the preconditioner was last set up on
+    unsigned short *sm16 = nullptr; // bfloat16 copy of the fine matrix for smoothing (padded rows)
+    size_t sm16_n = 0;
+    long long *noff = nullptr;      // node offsets into sm16
+    double *rs16 = nullptr;         // row scales of the binary16 copy
     bool jac_valid = false;  // s->vals holds a tangent (of jac_key) for lagged Newton steps
     long long jac_key = -1;
     long long jac_age = 0;
@@ -893,6 +1027,9 @@ void igap4h::solver_free(igap4h_ctx *c)
     Solver *s = c ? c->solver : nullptr;
     if (!s) return;
     (void)hipFree(s->vals);
+    (void)hipFree(s->sm16);
+    (void)hipFree(s->noff);
+    (void)hipFree(s->rs16);
     for (double *p : {s->Dblk, s->Dinv, s->V, s->r, s->z, s->p, s->q, s->w, s->t, s->Y, s->U0, s->R, s->partial,
                       s->pspmv, s->hd, s->sc})
         (void)hipFree(p);
@@ -1140,6 +1277,60 @@ struct Op {
     int nu_pre = 1, nu_post = 1;  // Gauss-Seidel sweeps before / after the coarse correction
     int mg_graph = 1;             // replay the V-cycle from a captured graph
     int smoother = 0;             // 0 block Gauss-Seidel, 1 Chebyshev (degree nu_pre / nu_post)
+    int svt = IGAP4H_SMOOTH_SAME; // matrix of the fine-level Chebyshev passes
+
+    // fine-level row pass of the smoother: the Krylov matrix or its 16-bit copy
+    int spmv_s(const double *x, double *y, const double *b = nullptr) const
+    {
+        if (svt == IGAP4H_SMOOTH_SAME) return spmv(x, y, b);
+        unsigned g = grid_for(sp.nnode, 4);
+        const unsigned *v32 = (const unsigned *)s->sm16;
+        const long long *no = s->noff;
+        const double *rs = s->rs16;
+        if (svt == IGAP4H_SMOOTH_F16) {
+            if (b) hipLaunchKernelGGL((k_spmv_16<1, 1>), dim3(g), dim3(256), 0, st, sp, v32, no, rs, x, b, y);
+            else hipLaunchKernelGGL((k_spmv_16<0, 1>), dim3(g), dim3(256), 0, st, sp, v32, no, rs, x, b, y);
+        } else {
+            if (b) hipLaunchKernelGGL((k_spmv_16<1, 0>), dim3(g), dim3(256), 0, st, sp, v32, no, rs, x, b, y);
+            else hipLaunchKernelGGL((k_spmv_16<0, 0>), dim3(g), dim3(256), 0, st, sp, v32, no, rs, x, b, y);
+        }
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    // the 16-bit copy of the current matrix (node offsets computed once)
+    int pack_16() const
+    {
+        if (!s->noff) {
+            std::vector<long long> off((size_t)sp.nnode + 1);
+            off[0] = 0;
+            for (long long n = 0; n < sp.nnode; n++) {
+                int a[3] = {(int)(n / ((long long)sp.nb[2] * sp.nb[1])), (int)((n / sp.nb[2]) % sp.nb[1]), (int)(n % sp.nb[2])};
+                int L = 3;
+                for (int d = 0; d < 3; d++) L *= std::min(sp.nb[d] - 1, a[d] + 2) - std::max(0, a[d] - 2) + 1;
+                off[n + 1] = off[n] + 3LL * even_len(L);
+            }
+            int rc;
+            if ((rc = dev_alloc(&s->noff, off.size()))) return rc;
+            HIPCHK(hipMemcpy(s->noff, off.data(), off.size() * sizeof(long long), hipMemcpyHostToDevice));
+            s->sm16_n = (size_t)off.back();
+            if (hipMalloc(&s->sm16, s->sm16_n * sizeof(unsigned short)) != hipSuccess) {
+                s->sm16 = nullptr;
+                return fail(IGAP4H_ENOMEM, "hipMalloc of %zu bytes failed", s->sm16_n * sizeof(unsigned short));
+            }
+            if ((rc = dev_alloc(&s->rs16, 3 * (size_t)sp.nnode))) return rc;
+        }
+        unsigned g = grid_for(sp.nnode, 4);
+        const long long *no = s->noff;
+        if (svt == IGAP4H_SMOOTH_F16) {
+            if (f32) hipLaunchKernelGGL((k_pack_16<float, 1>), dim3(g), dim3(256), 0, st, sp, (const float *)vals, no, s->sm16, s->rs16);
+            else hipLaunchKernelGGL((k_pack_16<double, 1>), dim3(g), dim3(256), 0, st, sp, (const double *)vals, no, s->sm16, s->rs16);
+        } else {
+            if (f32) hipLaunchKernelGGL((k_pack_16<float, 0>), dim3(g), dim3(256), 0, st, sp, (const float *)vals, no, s->sm16, s->rs16);
+            else hipLaunchKernelGGL((k_pack_16<double, 0>), dim3(g), dim3(256), 0, st, sp, (const double *)vals, no, s->sm16, s->rs16);
+        }
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
 
     int spmv(const double *x, double *y, const double *b = nullptr) const
     {
@@ -1292,7 +1483,7 @@ struct Op {
             hipLaunchKernelGGL(k_cheb_zero, gn, dim3(256), 0, st, L.nnode, Dinv, b, r, d0, x, 1.0 / theta);
             HIPCHK(hipGetLastError());
         } else if (l == 0) {  // fine level: the row pass is the SpMV kernel
-            if ((rc = spmv(x, r, b))) return rc;
+            if ((rc = spmv_s(x, r, b))) return rc;
             hipLaunchKernelGGL(k_cheb_dfromr, gn, dim3(256), 0, st, L.nnode, Dinv, (const double *)r, d0, 1.0 / theta);
             HIPCHK(hipGetLastError());
         } else {
@@ -1307,7 +1498,7 @@ struct Op {
             }
             const double rn = 1.0 / (2.0 * sigma - rho);
             if (l == 0) {
-                if ((rc = spmv(d0, L.x))) return rc;
+                if ((rc = spmv_s(d0, L.x))) return rc;
                 hipLaunchKernelGGL(k_cheb_update, gn, dim3(256), 0, st, L.nnode, Dinv, (const double *)L.x, (const double *)d0, d1, r, x,
                                    rn * rho, 2.0 * rn / delta, last ? 0 : 1);
                 HIPCHK(hipGetLastError());
@@ -1384,7 +1575,7 @@ struct Op {
         if (z != s->z) return vcycle(0, r, z);
         const long long n = 3 * sp.nnode;
         if (!s->mg_exec || s->key_vals != vals || s->key_f32 != f32 || s->key_pre != nu_pre || s->key_post != nu_post ||
-            s->key_smoother != smoother) {
+            s->key_smoother != smoother || s->key_svt != svt) {
             if (s->mg_exec) {
                 HIPCHK(hipGraphExecDestroy(s->mg_exec));
                 s->mg_exec = nullptr;
@@ -1414,6 +1605,7 @@ struct Op {
             s->key_pre = nu_pre;
             s->key_post = nu_post;
             s->key_smoother = smoother;
+            s->key_svt = svt;
         }
         HIPCHK(hipMemcpyAsync(s->mg_in, r, n * sizeof(double), hipMemcpyDeviceToDevice, st));
         HIPCHK(hipGraphLaunch(s->mg_exec, st));
@@ -1430,6 +1622,7 @@ struct Op {
         if (pc == IGAP4H_PC_MG) {
             int rc = mg_build(c, s);
             if (rc) return rc;
+            if (smoother == 1 && svt != IGAP4H_SMOOTH_SAME && (rc = pack_16())) return rc;
             return mg_setup();
         }
         return 0;
@@ -1679,6 +1872,8 @@ int check_opts(const igap4h_newton_opts *o)
     if (o->pc == IGAP4H_PC_MG && (o->mg_pre < 0 || o->mg_pre > 4 || o->mg_post < 0 || o->mg_post > 4 || o->mg_pre + o->mg_post < 1))
         return fail(IGAP4H_EINVAL, "multigrid sweeps pre %d / post %d outside 0..4 (sum >= 1)", o->mg_pre, o->mg_post);
     if (o->mg_smoother != IGAP4H_MG_GS && o->mg_smoother != IGAP4H_MG_CHEBYSHEV) return fail(IGAP4H_EINVAL, "bad mg_smoother %d", o->mg_smoother);
+    if (o->mg_smooth_vals != IGAP4H_SMOOTH_SAME && o->mg_smooth_vals != IGAP4H_SMOOTH_BF16 && o->mg_smooth_vals != IGAP4H_SMOOTH_F16)
+        return fail(IGAP4H_EINVAL, "bad mg_smooth_vals %d", o->mg_smooth_vals);
     if (o->vals_type != IGAP4H_VALS_F64 && o->vals_type != IGAP4H_VALS_F32) return fail(IGAP4H_EINVAL, "bad vals_type %d", o->vals_type);
     if (o->tangent_prec != IGAP4H_FP64 && o->tangent_prec != IGAP4H_FP32) return fail(IGAP4H_EINVAL, "bad tangent_prec %d", o->tangent_prec);
     return 0;
@@ -1719,6 +1914,7 @@ extern "C" int igap4h_newton_default_opts(igap4h_newton_opts *o)
     o->mg_smoother = IGAP4H_MG_CHEBYSHEV;
     o->lag_jacobian = 1;
     o->lag_persist = 0;
+    o->mg_smooth_vals = IGAP4H_SMOOTH_SAME;
     return 0;
 }
 
@@ -1744,6 +1940,7 @@ extern "C" int igap4h_linear_solve_dev(igap4h_ctx *c, const void *d_vals, int va
     A.nu_post = o->mg_post;
     A.mg_graph = o->mg_graph;
     A.smoother = o->mg_smoother;
+    A.svt = o->mg_smooth_vals;
     if ((rc = A.setup_pc())) return rc;
     return linear_solve(A, d_b, d_x, o, its, relres);
 }
@@ -1766,6 +1963,7 @@ extern "C" int igap4h_newton_solve_dev(igap4h_ctx *c, double *d_Uh, const igap4h
     A.nu_post = o->mg_post;
     A.mg_graph = o->mg_graph;
     A.smoother = o->mg_smoother;
+    A.svt = o->mg_smooth_vals;
     double *U = d_Uh;  // slice 0: current iterate
 
     auto t_all = clk::now();

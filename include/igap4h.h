@@ -269,6 +269,7 @@ int igap4h_batch_assemble(igap4h_batch *batch, void *stream);
 enum { IGAP4H_PC_NONE = 0, IGAP4H_PC_BJACOBI3 = 1, IGAP4H_PC_SGS = 2, IGAP4H_PC_MG = 3 };
 enum { IGAP4H_KSP_GMRES = 0, IGAP4H_KSP_CG = 1 };
 enum { IGAP4H_MG_GS = 0, IGAP4H_MG_CHEBYSHEV = 1 };
+enum { IGAP4H_SMOOTH_SAME = 0, IGAP4H_SMOOTH_BF16 = 1, IGAP4H_SMOOTH_F16 = 2 };
 
 typedef struct {
     double atol;       /* stop when ||R||_2 < atol (reference 1e-10); <= 0 off  */
@@ -306,6 +307,13 @@ typedef struct {
                           reference)                                              */
     int lag_persist;   /* 1: the lag counts across solves (-snes_lag_jacobian_persists),
                           so a time stepper reuses the tangent of earlier steps    */
+    int mg_smooth_vals;/* IGAP4H_PC_MG with IGAP4H_MG_CHEBYSHEV: matrix of the fine-level
+                          smoothing passes.  IGAP4H_SMOOTH_SAME: the Krylov matrix
+                          (vals_type); IGAP4H_SMOOTH_BF16: a bfloat16 copy made when
+                          the preconditioner is set up (half the bytes of FP32; the
+                          Krylov matrix, residual and Newton update are unchanged);
+                          IGAP4H_SMOOTH_F16: a binary16 copy of the rows scaled by a
+                          power of two each (entries in (-1, 1), 2^-11 rounding)   */
 } igap4h_newton_opts;
 
 typedef struct {

@@ -109,17 +109,27 @@ def test_multigrid_mesh_independent(torch_dev, n):
     dv = torch.tensor(vals, device=dev)
     db = torch.tensor(b, device=dev)
     dx = torch.zeros_like(db)
-    for smoother, deg in ((L.IGAP4H_MG_GS, 1), (L.IGAP4H_MG_CHEBYSHEV, 2)):
+    its_of = {}
+    for smoother, deg, svals in ((L.IGAP4H_MG_GS, 1, L.IGAP4H_SMOOTH_SAME), (L.IGAP4H_MG_CHEBYSHEV, 2, L.IGAP4H_SMOOTH_SAME),
+                                 (L.IGAP4H_MG_CHEBYSHEV, 2, L.IGAP4H_SMOOTH_BF16),
+                                 (L.IGAP4H_MG_CHEBYSHEV, 2, L.IGAP4H_SMOOTH_F16)):
         dx.zero_()
         its, relres = app.linear_solve_dev(dv.data_ptr(), L.IGAP4H_VALS_F64, db.data_ptr(), dx.data_ptr(),
                                            pc=L.IGAP4H_PC_MG, restart=60, ksp_rtol=1e-8, mg_smoother=smoother,
-                                           mg_pre=deg, mg_post=deg)
+                                           mg_pre=deg, mg_post=deg, mg_smooth_vals=svals)
         torch.cuda.synchronize()
         x = dx.cpu().numpy()
         true_rel = np.linalg.norm(b - K @ x) / np.linalg.norm(b)
-        say("  smoother", smoother, "its", its, "true", true_rel)
+        say("  smoother", smoother, "smoothing matrix", svals, "its", its, "true", true_rel)
         assert true_rel < 2e-8
         assert its <= 80, its
+        its_of[(smoother, svals)] = its
+    # the 16-bit copies (relative rounding 2^-9 bfloat16, 2^-12 binary16)
+    # smooth about as well as the matrix itself: a wrong row pass would show
+    # as many more iterations
+    cheb = its_of[(L.IGAP4H_MG_CHEBYSHEV, L.IGAP4H_SMOOTH_SAME)]
+    assert its_of[(L.IGAP4H_MG_CHEBYSHEV, L.IGAP4H_SMOOTH_BF16)] <= cheb + 3
+    assert its_of[(L.IGAP4H_MG_CHEBYSHEV, L.IGAP4H_SMOOTH_F16)] <= cheb + 2
 
 
 def test_linear_solve_deterministic(torch_dev):
@@ -183,6 +193,28 @@ def test_newton_mixed_same_solution(torch_dev, oracle):
     assert np.linalg.norm(oracle.residual(pb, U64)) < 1e-10 * i64["fnorm0"] * 10
 
 
+@pytest.mark.parametrize("svals", [L.IGAP4H_SMOOTH_BF16, L.IGAP4H_SMOOTH_F16])
+def test_newton_16bit_smoothing_same_solution(torch_dev, oracle, svals):
+    """8^3, Lame +-10 %: the mixed form with the fine-level smoothing passes
+    on a 16-bit copy of the FP32 tangent converges to the FP64 solution (the
+    smoother only preconditions; residual and Newton update are FP64)."""
+    torch, dev = torch_dev
+    say("newton mixed + 16-bit smoothing vs fp64", svals)
+    pb = P.stretch_problem(8, a=0.01, noise=0.1)
+    app = App(pb)
+    U0 = P.affine_guess(pb)
+    U64, i64 = newton_gpu(app, pb, U0, torch, dev, rtol=1e-10, atol=0.0, dirichlet_lift=1)
+    kw = dict(rtol=1e-10, atol=0.0, dirichlet_lift=1, tangent_prec=L.IGAP4H_FP32, vals_type=L.IGAP4H_VALS_F32)
+    U32, i32 = newton_gpu(app, pb, U0, torch, dev, **kw)
+    U16, i16 = newton_gpu(app, pb, U0, torch, dev, mg_smooth_vals=svals, **kw)
+    say("  its fp64", i64["ksp_its_hist"][:i64["newton_its"]], "mixed", i32["ksp_its_hist"][:i32["newton_its"]],
+        "16-bit smoothing", i16["ksp_its_hist"][:i16["newton_its"]])
+    assert i16["converged"] == 1 and i16["fnorm"] < 1e-10 * i16["fnorm0"]
+    assert i16["newton_its"] == i32["newton_its"]
+    assert rel(U16, U64) < 1e-9, rel(U16, U64)
+    assert i16["ksp_its"] <= 1.1 * i32["ksp_its"] + 5
+
+
 def test_newton_getime_step(torch_dev, oracle):
     """One implicit getime step on 3^3, all faces clamped: R(U) = 0 with the
     history slices held fixed.  The history is zero on the clamped dofs (as
@@ -214,3 +246,5 @@ def test_newton_options_rejected(torch_dev):
         app.newton_solve_dev(Uh.data_ptr(), restart=0)
     with pytest.raises(L.IgapError):
         app.newton_solve_dev(Uh.data_ptr(), pc=7)
+    with pytest.raises(L.IgapError):
+        app.newton_solve_dev(Uh.data_ptr(), mg_smooth_vals=3)
