@@ -103,32 +103,134 @@ __device__ __forceinline__ long long xcd_remap(long long b, long long nblk)
     return x * q + (x < rem ? x : rem) + i;
 }
 
-// y = K x (RES = 0) or y = b - K x (RES = 1); with DOT, partial[block] =
-// sum over the block's rows of x . y (for CG's p.Kp)
-template <class VT, int RES, int DOT>
-__global__ __launch_bounds__(256) void k_spmv(SpArgs s, const VT *__restrict__ vals, const double *__restrict__ x,
-                                              const double *__restrict__ b, double *__restrict__ y,
-                                              double *__restrict__ partial)
+// ---------------------------------------------------------------- line-staged row pass
+// One work-group per segment [z0, z0 + seglen) of a z-line (a0, a1): the x
+// entries of the 5 x 5 neighbouring lines over [z0 - 2, z0 + seglen + 2) are
+// staged in LDS once (contiguous runs, coalesced), then the four waves take
+// the segment's nodes in turn (LW waves), one node per wave, streaming the node's three
+// rows and reading x from LDS.  This replaces 375 scattered L2 reads per node
+// by ~5 per node and keeps the wave's memory traffic to the matrix values.
+constexpr int LW = 8;  // waves (nodes in flight) per line work-group
+
+struct LineGeom {
+    int nseg, seglen;  // segments per line, nodes per segment
+};
+
+static LineGeom line_geom(const int nb[3])
 {
-    __shared__ double sd[4];
+    LineGeom g;
+    g.nseg = (nb[2] + 63) / 64;
+    g.seglen = (nb[2] + g.nseg - 1) / g.nseg;
+    return g;
+}
+static size_t line_lds_bytes(const LineGeom &g) { return (size_t)25 * (g.seglen + 4) * 3 * sizeof(double); }
+
+// stage x; returns the segment's node range.  sx[(l0*5 + l1) * SW3 + (z - zb) * 3 + j]
+__device__ __forceinline__ void line_stage(const SpArgs &s, const LineGeom g, long long blk, const double *__restrict__ x,
+                                           double *sx, int &a0, int &a1, int &z0, int &z1)
+{
+    const int seg = (int)(blk % g.nseg);
+    const long long line = blk / g.nseg;
+    a1 = (int)(line % s.nb[1]);
+    a0 = (int)(line / s.nb[1]);
+    z0 = seg * g.seglen;
+    z1 = min(s.nb[2], z0 + g.seglen);
+    const int zb = z0 - 2, SW3 = 3 * (g.seglen + 4);
+    const int zl = max(0, z0 - 2), zh = min(s.nb[2], z1 + 2), nz3 = 3 * (zh - zl);
+    for (int t = threadIdx.x; t < 25 * nz3; t += blockDim.x) {
+        const int l = t / nz3, r = t - l * nz3;
+        const int b0 = a0 + l / 5 - 2, b1 = a1 + l % 5 - 2;
+        if (b0 < 0 || b0 >= s.nb[0] || b1 < 0 || b1 >= s.nb[1]) continue;
+        sx[l * SW3 + (zl - zb) * 3 + r] = x[3 * (((long long)b0 * s.nb[1] + b1) * s.nb[2] + zl) + r];
+    }
+    __syncthreads();
+}
+
+// LDS index of stencil column k (k < L) of node (a0, a1, z)
+__device__ __forceinline__ int line_col(const int a[3], const int lo[3], const int w[3], float r1, float r2, int zb, int SW3, int k)
+{
+    int t = (k * 43691) >> 17;  // k / 3 for k < 131072
+    int j = k - 3 * t;
+    int t2 = (int)(((float)t + 0.5f) * r2);  // t / w2 (exact: t < 128, w2 <= 5)
+    int o2 = t - t2 * w[2];
+    int o0 = (int)(((float)t2 + 0.5f) * r1);
+    int o1 = t2 - o0 * w[1];
+    return ((lo[0] + o0 - a[0] + 2) * 5 + (lo[1] + o1 - a[1] + 2)) * SW3 + (lo[2] + o2 - zb) * 3 + j;
+}
+
+__device__ __forceinline__ void line_box(const SpArgs &s, int a0, int a1, int z, int a[3], int lo[3], int w[3])
+{
+    a[0] = a0;
+    a[1] = a1;
+    a[2] = z;
+    for (int d = 0; d < 3; d++) {
+        lo[d] = max(0, a[d] - 2);
+        w[d] = min(s.nb[d] - 1, a[d] + 2) - lo[d] + 1;
+    }
+}
+
+// y = K x (RES = 0) or y = b - K x (RES = 1) on the AIJ values; with DOT,
+// partial[block] = sum over the block's rows of x . y (CG's p.Kp)
+template <class VT, int RES, int DOT>
+__global__ __launch_bounds__(LW * 64) void k_spmv_line(SpArgs s, LineGeom g, const VT *__restrict__ vals, const double *__restrict__ x,
+                                                   const double *__restrict__ b, double *__restrict__ y,
+                                                   double *__restrict__ partial)
+{
+    extern __shared__ double sx[];
+    __shared__ double sd[LW];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const long long node = xcd_remap(blockIdx.x, gridDim.x) * 4 + wv;
-    double d = 0.0;
-    if (node < s.nnode) {
-        double acc[3];
-        node_rows<VT>(s, node, vals, x, lane, acc);
+    int a0, a1, z0, z1;
+    line_stage(s, g, xcd_remap(blockIdx.x, gridDim.x), x, sx, a0, a1, z0, z1);
+    const int zb = z0 - 2, SW3 = 3 * (g.seglen + 4);
+    double dsum = 0.0;
+    for (int z = z0 + wv; z < z1; z += LW) {
+        int a[3], lo[3], w[3];
+        line_box(s, a0, a1, z, a, lo, w);
+        const long long node = ((long long)a0 * s.nb[1] + a1) * s.nb[2] + z;
+        const long long base = s.rowptr[3 * node];
+        const int L = 3 * w[0] * w[1] * w[2];
+        const float r2 = 1.0f / (float)w[2], r1 = 1.0f / (float)w[1];
+        double xv[6];
+        VT v0[6], v1[6], v2[6];
+#pragma unroll
+        for (int it = 0; it < 6; it++) {
+            const int k = lane + 64 * it;
+            xv[it] = 0.0;
+            v0[it] = v1[it] = v2[it] = (VT)0;
+            if (k < L) {
+                v0[it] = vals[base + k];
+                v1[it] = vals[base + L + k];
+                v2[it] = vals[base + 2 * L + k];
+                xv[it] = sx[line_col(a, lo, w, r1, r2, zb, SW3, k)];
+            }
+        }
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+        for (int it = 0; it < 6; it++) {
+            s0 += (double)v0[it] * xv[it];
+            s1 += (double)v1[it] * xv[it];
+            s2 += (double)v2[it] * xv[it];
+        }
+        s0 = wave_sum(s0);
+        s1 = wave_sum(s1);
+        s2 = wave_sum(s2);
         if (lane == 0) {
+            const double acc[3] = {s0, s1, s2};
             for (int i = 0; i < 3; i++) {
-                double v = RES ? b[3 * node + i] - acc[i] : acc[i];
+                const double v = RES ? b[3 * node + i] - acc[i] : acc[i];
                 y[3 * node + i] = v;
-                if (DOT) d += x[3 * node + i] * v;
+                if (DOT) dsum += x[3 * node + i] * v;
             }
         }
     }
     if (DOT) {
-        if (lane == 0) sd[wv] = d;
+        if (lane == 0) sd[wv] = dsum;
         __syncthreads();
-        if (threadIdx.x == 0) partial[blockIdx.x] = ((sd[0] + sd[1]) + sd[2]) + sd[3];
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int i = 0; i < LW; i++) t += sd[i];
+            partial[blockIdx.x] = t;
+        }
     }
 }
 
@@ -201,63 +303,57 @@ __device__ __forceinline__ float hi16(unsigned u, int f16)
     return f16 ? (float)__builtin_bit_cast(_Float16, (unsigned short)(u >> 16)) : __uint_as_float(u & 0xFFFF0000u);
 }
 
-// stencil column k of a node -> x index (k < L)
-__device__ __forceinline__ long long col_index(const SpArgs &s, const int lo[3], const int w[3], float r1, float r2, int k)
-{
-    int t = (k * 43691) >> 17;  // k / 3 for k < 131072
-    int j = k - 3 * t;
-    int t2 = (int)(((float)t + 0.5f) * r2);  // t / w2 (exact: t < 128, w2 <= 5)
-    int o2 = t - t2 * w[2];
-    int o0 = (int)(((float)t2 + 0.5f) * r1);
-    int o1 = t2 - o0 * w[1];
-    return 3 * (((long long)(lo[0] + o0) * s.nb[1] + (lo[1] + o1)) * s.nb[2] + (lo[2] + o2)) + j;
-}
-
-// y = K~ x (RES = 0) or y = b - K~ x (RES = 1) on a 16-bit copy, one wave
-// per node, two columns per lane and load
+// y = K~ x (RES = 0) or y = b - K~ x (RES = 1) on a 16-bit copy, line-staged
+// x as k_spmv_line, two columns per lane and 4-byte load
 template <int RES, int F16>
-__global__ __launch_bounds__(256) void k_spmv_16(SpArgs s, const unsigned *__restrict__ v32, const long long *__restrict__ noff,
-                                                 const double *__restrict__ rs, const double *__restrict__ x,
-                                                 const double *__restrict__ b, double *__restrict__ y)
+__global__ __launch_bounds__(LW * 64) void k_spmv_16(SpArgs s, LineGeom g, const unsigned *__restrict__ v32,
+                                                 const long long *__restrict__ noff, const double *__restrict__ rs,
+                                                 const double *__restrict__ x, const double *__restrict__ b,
+                                                 double *__restrict__ y)
 {
+    extern __shared__ double sx[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const long long node = xcd_remap(blockIdx.x, gridDim.x) * 4 + wv;
-    if (node >= s.nnode) return;
-    int a[3], lo[3], w[3];
-    node_box(s, node, a, lo, w);
-    const int L = 3 * w[0] * w[1] * w[2], Lp = even_len(L);
-    const float r2 = 1.0f / (float)w[2], r1 = 1.0f / (float)w[1];
-    const long long ob = noff[node] >> 1;  // in 4-byte words
-    double xa[3], xb[3];
-    unsigned u0[3], u1[3], u2[3];
+    int a0, a1, z0, z1;
+    line_stage(s, g, xcd_remap(blockIdx.x, gridDim.x), x, sx, a0, a1, z0, z1);
+    const int zb = z0 - 2, SW3 = 3 * (g.seglen + 4);
+    for (int z = z0 + wv; z < z1; z += LW) {
+        int a[3], lo[3], w[3];
+        line_box(s, a0, a1, z, a, lo, w);
+        const long long node = ((long long)a0 * s.nb[1] + a1) * s.nb[2] + z;
+        const int L = 3 * w[0] * w[1] * w[2], Lp = even_len(L);
+        const float r2 = 1.0f / (float)w[2], r1 = 1.0f / (float)w[1];
+        const long long ob = noff[node] >> 1;  // in 4-byte words
+        double xa[3], xb[3];
+        unsigned u0[3], u1[3], u2[3];
 #pragma unroll
-    for (int it = 0; it < 3; it++) {
-        const int p = lane + 64 * it, k = 2 * p;
-        xa[it] = xb[it] = 0.0;
-        u0[it] = u1[it] = u2[it] = 0u;
-        if (k < L) {
-            xa[it] = x[col_index(s, lo, w, r1, r2, k)];
-            if (k + 1 < L) xb[it] = x[col_index(s, lo, w, r1, r2, k + 1)];
-            u0[it] = v32[ob + p];
-            u1[it] = v32[ob + (Lp >> 1) + p];
-            u2[it] = v32[ob + Lp + p];
+        for (int it = 0; it < 3; it++) {
+            const int p = lane + 64 * it, k = 2 * p;
+            xa[it] = xb[it] = 0.0;
+            u0[it] = u1[it] = u2[it] = 0u;
+            if (k < L) {
+                u0[it] = v32[ob + p];
+                u1[it] = v32[ob + (Lp >> 1) + p];
+                u2[it] = v32[ob + Lp + p];
+                xa[it] = sx[line_col(a, lo, w, r1, r2, zb, SW3, k)];
+                if (k + 1 < L) xb[it] = sx[line_col(a, lo, w, r1, r2, zb, SW3, k + 1)];
+            }
         }
-    }
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
-    for (int it = 0; it < 3; it++) {
-        s0 += (double)lo16(u0[it], F16) * xa[it] + (double)hi16(u0[it], F16) * xb[it];
-        s1 += (double)lo16(u1[it], F16) * xa[it] + (double)hi16(u1[it], F16) * xb[it];
-        s2 += (double)lo16(u2[it], F16) * xa[it] + (double)hi16(u2[it], F16) * xb[it];
-    }
-    s0 = wave_sum(s0);
-    s1 = wave_sum(s1);
-    s2 = wave_sum(s2);
-    if (lane == 0) {
-        double acc[3] = {s0, s1, s2};
-        if (F16)
-            for (int i = 0; i < 3; i++) acc[i] *= rs[3 * node + i];
-        for (int i = 0; i < 3; i++) y[3 * node + i] = RES ? b[3 * node + i] - acc[i] : acc[i];
+        for (int it = 0; it < 3; it++) {
+            s0 += (double)lo16(u0[it], F16) * xa[it] + (double)hi16(u0[it], F16) * xb[it];
+            s1 += (double)lo16(u1[it], F16) * xa[it] + (double)hi16(u1[it], F16) * xb[it];
+            s2 += (double)lo16(u2[it], F16) * xa[it] + (double)hi16(u2[it], F16) * xb[it];
+        }
+        s0 = wave_sum(s0);
+        s1 = wave_sum(s1);
+        s2 = wave_sum(s2);
+        if (lane == 0) {
+            double acc[3] = {s0, s1, s2};
+            if (F16)
+                for (int i = 0; i < 3; i++) acc[i] *= rs[3 * node + i];
+            for (int i = 0; i < 3; i++) y[3 * node + i] = RES ? b[3 * node + i] - acc[i] : acc[i];
+        }
     }
 }
 
@@ -876,40 +972,41 @@ __global__ __launch_bounds__(256) void k_galerkin(int d, int3 nbi, int3 nbo, con
 
 // bc = P^T rf (restriction), one thread per coarse node
 // (dmask: the fine residual's Dirichlet entries are left out)
-__global__ void k_restrict(Xfer3 x, const double *__restrict__ rf, double *__restrict__ bc, const unsigned char *__restrict__ dmask)
+// bc = P^T rf (restriction), one wave per coarse node: lane = one of the
+// MGW^3 = 64 fine nodes of its support, then a wave sum (dmask: the fine
+// Dirichlet dofs contribute nothing)
+static_assert(MGW * MGW * MGW == 64, "one lane per fine node of a coarse support");
+__global__ __launch_bounds__(256) void k_restrict(Xfer3 x, const double *__restrict__ rf, double *__restrict__ bc,
+                                                  const unsigned char *__restrict__ dmask)
 {
     const long long nc = (long long)x.nbc[0] * x.nbc[1] * x.nbc[2];
-    const long long C = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long C = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (C >= nc) return;
+    const int lane = threadIdx.x & 63, k0 = lane >> 4, k1 = (lane >> 2) & 3, k2 = lane & 3;
     int A[3];
     decode_node(x.nbc, C, A);
+    const double w = (x.d[0].c2f_w[A[0] * MGW + k0] * x.d[1].c2f_w[A[1] * MGW + k1]) * x.d[2].c2f_w[A[2] * MGW + k2];
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    const int l0 = x.d[0].c2f_lo[A[0]], l1 = x.d[1].c2f_lo[A[1]], l2 = x.d[2].c2f_lo[A[2]];
-    for (int k0 = 0; k0 < MGW; k0++) {
-        double w0 = x.d[0].c2f_w[A[0] * MGW + k0];
-        if (w0 == 0.0) continue;
-        for (int k1 = 0; k1 < MGW; k1++) {
-            double w1 = w0 * x.d[1].c2f_w[A[1] * MGW + k1];
-            if (w1 == 0.0) continue;
-            for (int k2 = 0; k2 < MGW; k2++) {
-                double w = w1 * x.d[2].c2f_w[A[2] * MGW + k2];
-                if (w == 0.0) continue;
-                long long f = (((long long)(l0 + k0)) * x.nbf[1] + (l1 + k1)) * x.nbf[2] + (l2 + k2);
-                if (dmask) {
-                    s0 += dmask[3 * f] ? 0.0 : w * rf[3 * f];
-                    s1 += dmask[3 * f + 1] ? 0.0 : w * rf[3 * f + 1];
-                    s2 += dmask[3 * f + 2] ? 0.0 : w * rf[3 * f + 2];
-                } else {
-                    s0 += w * rf[3 * f];
-                    s1 += w * rf[3 * f + 1];
-                    s2 += w * rf[3 * f + 2];
-                }
-            }
+    if (w != 0.0) {
+        const long long f = (((long long)(x.d[0].c2f_lo[A[0]] + k0)) * x.nbf[1] + (x.d[1].c2f_lo[A[1]] + k1)) * x.nbf[2] +
+                            (x.d[2].c2f_lo[A[2]] + k2);
+        s0 = w * rf[3 * f];
+        s1 = w * rf[3 * f + 1];
+        s2 = w * rf[3 * f + 2];
+        if (dmask) {
+            if (dmask[3 * f]) s0 = 0.0;
+            if (dmask[3 * f + 1]) s1 = 0.0;
+            if (dmask[3 * f + 2]) s2 = 0.0;
         }
     }
-    bc[3 * C] = s0;
-    bc[3 * C + 1] = s1;
-    bc[3 * C + 2] = s2;
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    if (lane == 0) {
+        bc[3 * C] = s0;
+        bc[3 * C + 1] = s1;
+        bc[3 * C + 2] = s2;
+    }
 }
 
 // xf += P xc (prolongation), one thread per fine node
@@ -1060,7 +1157,7 @@ static int solver_ws(igap4h_ctx *c, int restart, size_t vals_bytes)
             (rc = dev_alloc(&s->r, n)) || (rc = dev_alloc(&s->z, n)) || (rc = dev_alloc(&s->p, n)) ||
             (rc = dev_alloc(&s->q, n)) || (rc = dev_alloc(&s->w, n)) || (rc = dev_alloc(&s->t, n)) ||
             (rc = dev_alloc(&s->Y, n)) || (rc = dev_alloc(&s->U0, n)) || (rc = dev_alloc(&s->R, n)) ||
-            (rc = dev_alloc(&s->sc, 64)) || (rc = dev_alloc(&s->pspmv, (size_t)(s->nnode + 3) / 4)))
+            (rc = dev_alloc(&s->sc, 64)) || (rc = dev_alloc(&s->pspmv, (size_t)s->nnode + 1)))
             return rc;
         if (hipHostMalloc((void **)&s->hp, 64 * sizeof(double)) != hipSuccess) {
             s->hp = nullptr;
@@ -1283,16 +1380,18 @@ struct Op {
     int spmv_s(const double *x, double *y, const double *b = nullptr) const
     {
         if (svt == IGAP4H_SMOOTH_SAME) return spmv(x, y, b);
-        unsigned g = grid_for(sp.nnode, 4);
+        const LineGeom lg = line_geom(sp.nb);
+        const unsigned g = (unsigned)((long long)sp.nb[0] * sp.nb[1] * lg.nseg);
+        const size_t sh = line_lds_bytes(lg);
         const unsigned *v32 = (const unsigned *)s->sm16;
         const long long *no = s->noff;
         const double *rs = s->rs16;
         if (svt == IGAP4H_SMOOTH_F16) {
-            if (b) hipLaunchKernelGGL((k_spmv_16<1, 1>), dim3(g), dim3(256), 0, st, sp, v32, no, rs, x, b, y);
-            else hipLaunchKernelGGL((k_spmv_16<0, 1>), dim3(g), dim3(256), 0, st, sp, v32, no, rs, x, b, y);
+            if (b) hipLaunchKernelGGL((k_spmv_16<1, 1>), dim3(g), dim3(LW * 64), sh, st, sp, lg, v32, no, rs, x, b, y);
+            else hipLaunchKernelGGL((k_spmv_16<0, 1>), dim3(g), dim3(LW * 64), sh, st, sp, lg, v32, no, rs, x, b, y);
         } else {
-            if (b) hipLaunchKernelGGL((k_spmv_16<1, 0>), dim3(g), dim3(256), 0, st, sp, v32, no, rs, x, b, y);
-            else hipLaunchKernelGGL((k_spmv_16<0, 0>), dim3(g), dim3(256), 0, st, sp, v32, no, rs, x, b, y);
+            if (b) hipLaunchKernelGGL((k_spmv_16<1, 0>), dim3(g), dim3(LW * 64), sh, st, sp, lg, v32, no, rs, x, b, y);
+            else hipLaunchKernelGGL((k_spmv_16<0, 0>), dim3(g), dim3(LW * 64), sh, st, sp, lg, v32, no, rs, x, b, y);
         }
         HIPCHK(hipGetLastError());
         return 0;
@@ -1332,15 +1431,18 @@ struct Op {
         return 0;
     }
 
+    // line-staged row pass: one work-group per z-line segment
     int spmv(const double *x, double *y, const double *b = nullptr) const
     {
-        unsigned g = grid_for(sp.nnode, 4);
+        const LineGeom lg = line_geom(sp.nb);
+        const unsigned g = (unsigned)((long long)sp.nb[0] * sp.nb[1] * lg.nseg);
+        const size_t sh = line_lds_bytes(lg);
         if (f32) {
-            if (b) hipLaunchKernelGGL((k_spmv<float, 1, 0>), dim3(g), dim3(256), 0, st, sp, (const float *)vals, x, b, y, nullptr);
-            else hipLaunchKernelGGL((k_spmv<float, 0, 0>), dim3(g), dim3(256), 0, st, sp, (const float *)vals, x, b, y, nullptr);
+            if (b) hipLaunchKernelGGL((k_spmv_line<float, 1, 0>), dim3(g), dim3(LW * 64), sh, st, sp, lg, (const float *)vals, x, b, y, nullptr);
+            else hipLaunchKernelGGL((k_spmv_line<float, 0, 0>), dim3(g), dim3(LW * 64), sh, st, sp, lg, (const float *)vals, x, b, y, nullptr);
         } else {
-            if (b) hipLaunchKernelGGL((k_spmv<double, 1, 0>), dim3(g), dim3(256), 0, st, sp, (const double *)vals, x, b, y, nullptr);
-            else hipLaunchKernelGGL((k_spmv<double, 0, 0>), dim3(g), dim3(256), 0, st, sp, (const double *)vals, x, b, y, nullptr);
+            if (b) hipLaunchKernelGGL((k_spmv_line<double, 1, 0>), dim3(g), dim3(LW * 64), sh, st, sp, lg, (const double *)vals, x, b, y, nullptr);
+            else hipLaunchKernelGGL((k_spmv_line<double, 0, 0>), dim3(g), dim3(LW * 64), sh, st, sp, lg, (const double *)vals, x, b, y, nullptr);
         }
         HIPCHK(hipGetLastError());
         return 0;
@@ -1348,9 +1450,11 @@ struct Op {
     // y = K x and partial[block] of x.y; returns the number of partials
     int spmv_dot(const double *x, double *y, double *partial, int *nparts) const
     {
-        unsigned g = grid_for(sp.nnode, 4);
-        if (f32) hipLaunchKernelGGL((k_spmv<float, 0, 1>), dim3(g), dim3(256), 0, st, sp, (const float *)vals, x, nullptr, y, partial);
-        else hipLaunchKernelGGL((k_spmv<double, 0, 1>), dim3(g), dim3(256), 0, st, sp, (const double *)vals, x, nullptr, y, partial);
+        const LineGeom lg = line_geom(sp.nb);
+        const unsigned g = (unsigned)((long long)sp.nb[0] * sp.nb[1] * lg.nseg);
+        const size_t sh = line_lds_bytes(lg);
+        if (f32) hipLaunchKernelGGL((k_spmv_line<float, 0, 1>), dim3(g), dim3(LW * 64), sh, st, sp, lg, (const float *)vals, x, nullptr, y, partial);
+        else hipLaunchKernelGGL((k_spmv_line<double, 0, 1>), dim3(g), dim3(LW * 64), sh, st, sp, lg, (const double *)vals, x, nullptr, y, partial);
         HIPCHK(hipGetLastError());
         *nparts = (int)g;
         return 0;
@@ -1538,7 +1642,7 @@ struct Op {
                 hipLaunchKernelGGL(k_zero, dim3(vgrid(n)), dim3(256), 0, st, x, n);
                 HIPCHK(hipGetLastError());
             }
-            hipLaunchKernelGGL(k_restrict, dim3(grid_for(C.nnode, 256)), dim3(256), 0, st, L.T, res, C.b, l ? (const unsigned char *)nullptr : (const unsigned char *)c->d_mask);
+            hipLaunchKernelGGL(k_restrict, dim3(grid_for(C.nnode, 4)), dim3(256), 0, st, L.T, res, C.b, l ? (const unsigned char *)nullptr : (const unsigned char *)c->d_mask);
             HIPCHK(hipGetLastError());
             if ((rc = vcycle(l + 1, C.b, C.x))) return rc;
             hipLaunchKernelGGL(k_prolong_add, dim3(grid_for(L.nnode, 256)), dim3(256), 0, st, L.T, (const double *)C.x, x, l ? (const unsigned char *)nullptr : (const unsigned char *)c->d_mask);
@@ -1560,7 +1664,7 @@ struct Op {
             }
             res = L.r;
         }
-        hipLaunchKernelGGL(k_restrict, dim3(grid_for(C.nnode, 256)), dim3(256), 0, st, L.T, res, C.b, l ? (const unsigned char *)nullptr : (const unsigned char *)c->d_mask);
+        hipLaunchKernelGGL(k_restrict, dim3(grid_for(C.nnode, 4)), dim3(256), 0, st, L.T, res, C.b, l ? (const unsigned char *)nullptr : (const unsigned char *)c->d_mask);
         HIPCHK(hipGetLastError());
         if ((rc = vcycle(l + 1, C.b, C.x))) return rc;
         hipLaunchKernelGGL(k_prolong_add, dim3(grid_for(L.nnode, 256)), dim3(256), 0, st, L.T, (const double *)C.x, x, l ? (const unsigned char *)nullptr : (const unsigned char *)c->d_mask);
