@@ -398,9 +398,9 @@ __global__ __launch_bounds__(256) void k_gather(MeshArgs m, const double *__rest
 }
 
 // ============================================================== tangent
+// One work-group of 256 threads (4 waves) per element of one colour; each
+// wave owns 6 of the 24 16x16 output tiles of the i <= j dof blocks.
 constexpr int TNT = 256;  // threads per element
-constexpr int QC = 4;     // Gauss points per contraction chunk
-constexpr int TPW = 6;    // output tiles per wave (24 tiles of 16x16 over 4 waves)
 
 // Column tiles: row-block i of K holds columns (j >= i, B) padded to 32 per
 // j-block: widths 96 | 64 | 32 at offsets 0 | 96 | 160 of a 192-wide strip.
@@ -437,185 +437,6 @@ __device__ __forceinline__ void scatter_rows(const ACC *sKe, const long long *sR
         for (int m = 0; m < NB; m++)
             if (pos[m] >= 0) vals[pos[m]] = old[m] + add[m];
     }
-}
-
-// One (Gauss point, basis function B, dof block j) task of the T phase:
-// T_ij[s, B] for i <= j written straight into the LDS strip.
-template <class T, class ACC, int J, int NS, int S0>
-__device__ __forceinline__ void tangent_task(const Compact<T> *sCmp, const double *sNd, ACC *sT, const Coef &c,
-                                             double mass, const double *sw, int ql, int B, int qp, int nqp)
-{
-    double out[J + 1][10];
-    if (qp < nqp) {
-        double n[10];
-#pragma unroll
-        for (int s = 0; s < 10; s++) n[s] = sNd[(ql * 27 + B) * 10 + s];
-        tangent_apply<T, J>(sCmp[qp], c, n, mass, sw[qp], out);
-    } else {
-#pragma unroll
-        for (int i = 0; i <= J; i++)
-#pragma unroll
-            for (int s = 0; s < 10; s++) out[i][s] = 0.0;
-    }
-#pragma unroll
-    for (int i = 0; i <= J; i++) {
-        ACC *col = sT + block_off(i) + (J - i) * 27 + B;
-#pragma unroll
-        for (int s = S0; s < 10; s++) col[(ql * NS + s - S0) * 192] = (ACC)out[i][s];
-    }
-}
-
-// One workgroup per element of one colour (ex = cx + 3 bx, ...): elements of
-// a colour share no basis function, so the read-modify-write scatter into the
-// CSR values needs no atomics and the sum order is fixed.
-template <int PK, class ACC>
-__global__ __launch_bounds__(TNT) void k_tangent(MeshArgs m, const double *__restrict__ Uh,
-                                                 const double *__restrict__ coef, double mass,
-                                                 const long long *__restrict__ rowptr,
-                                                 const unsigned char *__restrict__ dmask, void *__restrict__ vals,
-                                                 int vals_f32, int cx, int cy, int cz, int my, int mz)
-{
-    using T = typename PhysT<PK>::T;
-    constexpr int NS = PhysT<PK>::NS, S0 = PhysT<PK>::S0, NH = PhysT<PK>::NH;
-    constexpr int KC = QC * NS;
-    constexpr int NKA = KC * 32;          // A operand: N^T, [k][A(32)]
-    constexpr int NBUF = KC * (32 + 192); // A operand + T strip
-    static_assert(NBUF >= 81 * 81, "element matrix staging must fit");
-    static_assert(KC % 4 == 0, "k chunk must be a multiple of 4");
-    using V = typename V4<ACC>::t;
-
-    __shared__ double sU[NH * 81];
-    __shared__ double sphi[3 * 4 * 9];
-    __shared__ double sw[64];
-    __shared__ Compact<T> sCmp[64];
-    __shared__ double sNd[QC * 27 * 10];
-    __shared__ __attribute__((aligned(16))) ACC sBuf[NBUF];
-    __shared__ long long sRowBase[81]; // CSR position of column node e in row r
-    __shared__ int sRowSx[81], sRowSy[81];
-    __shared__ unsigned char sMask[81];
-    ACC *sNk = sBuf;
-    ACC *sT = sBuf + NKA;
-
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int b = blockIdx.x;
-    int ev[3];
-    ev[2] = cz + 3 * (b % mz);
-    ev[1] = cy + 3 * ((b / mz) % my);
-    ev[0] = cx + 3 * (b / (mz * my));
-    const long long e = ((long long)ev[0] * m.ne[1] + ev[1]) * m.ne[2] + ev[2];
-    const int nq = m.nq, nqp = nq * nq * nq;
-
-    for (int idx = tid; idx < NH * 81; idx += TNT) {
-        int h = idx / 81, r = idx % 81, A = r / 3, i = r % 3;
-        long long nd = node_of(m, ev[0] + A / 9, ev[1] + (A / 3) % 3, ev[2] + A % 3);
-        sU[idx] = Uh[h * m.ndof + 3 * nd + i];
-    }
-    fill_phi(m, ev, sphi, tid, TNT);
-    if (tid < 81) {
-        // row r = 3A + i of the element: CSR position of column (B, j) is
-        // base + Bx sx + By sy + 3 Bz + j (columns ascending, stencil lo..hi)
-        int A = tid / 3, i = tid % 3;
-        int a0 = ev[0] + A / 9, a1 = ev[1] + (A / 3) % 3, a2 = ev[2] + A % 3;
-        long long grow = 3 * node_of(m, a0, a1, a2) + i;
-        int lo0 = max(0, a0 - 2), lo1 = max(0, a1 - 2), lo2 = max(0, a2 - 2);
-        int w1 = min(m.nb[1] - 1, a1 + 2) - lo1 + 1, w2 = min(m.nb[2] - 1, a2 + 2) - lo2 + 1;
-        sRowBase[tid] = rowptr[grow] + 3LL * (((long long)(ev[0] - lo0) * w1 + (ev[1] - lo1)) * w2 + (ev[2] - lo2));
-        sRowSx[tid] = 3 * w1 * w2;
-        sRowSy[tid] = 3 * w2;
-        sMask[tid] = dmask[grow];
-    }
-    __syncthreads();
-    const Coef c = load_coef(coef, e);
-
-    // per-point compact data
-    if (tid < nqp) {
-        int qx = tid / (nq * nq), qy = (tid / nq) % nq, qz = tid % nq;
-        double u[NH][30], w;
-        interp_slots<NH>(m, sU, sphi, ev, qx, qy, qz, u, &w);
-        T uj[30];
-        if (PK == 0) {
-            for (int s = 0; s < 30; s++) uj[s] = Num<T>::cst(u[0][s]);
-        } else {
-            for (int s = 0; s < 30; s++) uj[s] = Num<T>::lin((u[NH - 1][s] + u[1 % NH][s]) / 2.0, (u[0][s] - u[NH - 1][s]) / 2.0);
-        }
-        Compact<T> cp;
-        compact_of<T>(uj, c, cp);
-        sCmp[tid] = cp;
-        sw[tid] = w;
-    }
-    __syncthreads();
-
-    V acc[TPW];
-    for (int t = 0; t < TPW; t++)
-        for (int r = 0; r < 4; r++) acc[t][r] = (ACC)0;
-
-    const int nchunk = (nqp + QC - 1) / QC;
-    for (int ch = 0; ch < nchunk; ch++) {
-        // (a) basis slot values for the chunk; A operand N^T[k][A]
-        for (int idx = tid; idx < QC * 270; idx += TNT) {
-            int ql = idx / 270, rem = idx % 270, A = rem / 10, s = rem % 10;
-            int qp = ch * QC + ql;
-            double v = 0.0;
-            if (qp < nqp) {
-                int qx = qp / (nq * nq), qy = (qp / nq) % nq, qz = qp % nq;
-                const double *px = sphi + (0 * 4 + qx) * 9 + (A / 9) * 3;
-                const double *py = sphi + (1 * 4 + qy) * 9 + ((A / 3) % 3) * 3;
-                const double *pz = sphi + (2 * 4 + qz) * 9 + (A % 3) * 3;
-                v = px[slot_d(s, 0)] * py[slot_d(s, 1)] * pz[slot_d(s, 2)];
-            }
-            sNd[idx] = v;
-            if (s >= S0) sNk[(ql * NS + s - S0) * 32 + A] = (ACC)v;
-        }
-        __syncthreads();
-        // (b) B operand T_ij[k][B] = w sum_t D[(i,s),(j,t)] N_{B,t}, i <= j.
-        // Tasks ordered j = 2, 1, 0 (costliest first) so the ragged second
-        // round gets the cheapest ones.
-        for (int task = tid; task < QC * 81; task += TNT) {
-            int j = 2 - task / (QC * 27), rem = task % (QC * 27), ql = rem / 27, B = rem % 27;
-            int qp = ch * QC + ql;
-            if (j == 2) tangent_task<T, ACC, 2, NS, S0>(sCmp, sNd, sT, c, mass, sw, ql, B, qp, nqp);
-            else if (j == 1) tangent_task<T, ACC, 1, NS, S0>(sCmp, sNd, sT, c, mass, sw, ql, B, qp, nqp);
-            else tangent_task<T, ACC, 0, NS, S0>(sCmp, sNd, sT, c, mass, sw, ql, B, qp, nqp);
-        }
-        __syncthreads();
-        // (c) K_i,(j>=i) += N^T T on MFMA
-#pragma unroll
-        for (int ks = 0; ks < KC / 4; ks++) {
-            int k = 4 * ks + (lane >> 4);
-#pragma unroll
-            for (int t = 0; t < TPW; t++) {
-                int tg = wave * TPW + t, ct = tg >> 1, at = tg & 1;
-                ACC a = sNk[k * 32 + at * 16 + (lane & 15)];
-                ACC bb = sT[k * 192 + ct * 16 + (lane & 15)];
-                acc[t] = mfma16(a, bb, acc[t]);
-            }
-        }
-        __syncthreads();
-    }
-
-    // stage the full 81x81 element matrix (mirror the i < j blocks)
-    ACC *sKe = sBuf;
-    for (int t = 0; t < TPW; t++) {
-        int tg = wave * TPW + t, ct = tg >> 1, at = tg & 1;
-        int i = ctile_block(ct);
-        int cl = ct * 16 + (lane & 15) - block_off(i);
-        if (cl >= (3 - i) * 27) continue;
-        int j = i + cl / 27, B = cl % 27;
-        for (int r = 0; r < 4; r++) {
-            int A = at * 16 + acc_row((ACC)0, lane, r);
-            if (A >= 27) continue;
-            sKe[(3 * A + i) * 81 + 3 * B + j] = acc[t][r];
-            if (i != j) sKe[(3 * B + j) * 81 + 3 * A + i] = acc[t][r];
-        }
-    }
-    __syncthreads();
-    // scatter-add into the CSR values (Dirichlet rows/columns skipped);
-    // consecutive threads take consecutive columns of a row, which are runs
-    // of 9 consecutive CSR positions
-    if (vals_f32)
-        scatter_rows<float>(sKe, sRowBase, sRowSx, sRowSy, sMask, (float *)vals, tid);
-    else
-        scatter_rows<double>(sKe, sRowBase, sRowSx, sRowSy, sMask, (double *)vals, tid);
 }
 
 // ---------------------------------------------------------------- ge, high occupancy
@@ -865,6 +686,253 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(ACC)
     const int b = blockIdx.x;
     const int ev[3] = {cx + 3 * (b / (mz * my)), cy + 3 * ((b / mz) % my), cz + 3 * (b % mz)};
     tangent_ge_body<ACC, QC>(m, Uh, coef, rowptr, dmask, vals, vals_f32, ev);
+}
+
+// ---------------------------------------------------------------- getime
+// phys/getime tangent (torder 2) on the structure of k_tangent_ge: per chunk
+// of QC Gauss points the four waves build T_ij with wave-uniform dof blocks
+// (lane = (point, basis function B)), then contract on MFMA.  The per-point
+// compact data are jets along u(s) = ubar + s (u - u0)/2, ubar = (u0 + u1)/2
+// (phys/getime/tangent.c:4127-5026: tangent = (1/2)(H0/2 + H1/3 + H2/4) of
+// the series of d2W), with coefficients in the arithmetic type ACC, stored
+// in LDS and read at use.  All 10 slots enter the contraction: the value
+// slot carries the mass term rho/dt^2 + c/(2 dt) (phys/getime/tangent.c:4127).
+// ubar and (u - u0)/2 are interpolated directly (interpolation is linear);
+// the periodic F0 terms belong to ubar only.
+using JetA3f = Jet<3, float>;
+template <class ACC>
+using GtCompact = Compact<Jet<3, ACC>>;
+
+template <int QC>
+struct GtTile {
+    static constexpr int NS = 10;
+    static constexpr int KC = QC * NS;               // contraction depth per chunk
+    static constexpr int KCP = (KC + 3) / 4 * 4;     // padded to the MFMA k of 4
+    static constexpr int NTS = KCP * 192;            // T strip
+};
+
+template <int J, int I0, int I1, class ACC>
+__device__ __forceinline__ void gt_task(const GtCompact<ACC> *sCmp, const ACC *sphi, const int *sQ, const double *sw,
+                                        ACC *sT, const Coef &c, ACC mass, int ql, int B, int qp, int nqp)
+{
+    ACC out[J + 1][10];
+    if (qp < nqp) {
+        ACC n[10];
+        int q = sQ[qp];
+        basis_slots<ACC>(sphi, q & 15, (q >> 4) & 15, (q >> 8) & 15, B, n);
+        tangent_apply<Jet<3, ACC>, J, I0, I1, ACC>(sCmp[qp], c, n, mass, (ACC)sw[qp], out);
+    } else {
+#pragma unroll
+        for (int i = I0; i <= I1; i++)
+#pragma unroll
+            for (int s = 0; s < 10; s++) out[i][s] = (ACC)0;
+    }
+#pragma unroll
+    for (int i = I0; i <= I1; i++) {
+        ACC *col = sT + block_off(i) + (J - i) * 27 + B;
+#pragma unroll
+        for (int s = 0; s < 10; s++) col[(ql * 10 + s) * 192] = out[i][s];
+    }
+}
+
+template <class ACC, int QC>
+__device__ __forceinline__ void tangent_gt_body(const MeshArgs &m, const double *__restrict__ Uh, const double *__restrict__ coef,
+                                                double mass_d, const long long *__restrict__ rowptr,
+                                                const unsigned char *__restrict__ dmask, void *__restrict__ vals, int vals_f32,
+                                                const int ev[3])
+{
+    using G = GtTile<QC>;
+    using V = typename V4<ACC>::t;
+    constexpr int CMP_BYTES = 64 * (int)sizeof(GtCompact<ACC>);
+    constexpr int OP_BYTES0 = G::NTS * (int)sizeof(ACC);
+    constexpr int OP_BYTES = OP_BYTES0 > SF_SCRATCH * 8 ? OP_BYTES0 : SF_SCRATCH * 8;
+    constexpr int KE_BYTES = 81 * 81 * (int)sizeof(ACC);
+    constexpr int REGION = (CMP_BYTES + OP_BYTES) > KE_BYTES ? (CMP_BYTES + OP_BYTES) : KE_BYTES;
+    __shared__ __attribute__((aligned(16))) unsigned char sRegion[REGION];
+    __shared__ double sU[2 * 81];  // ubar, (u - u0)/2
+    __shared__ double sphi[3 * 4 * 9];
+    __shared__ ACC sphiA[3 * 4 * 9];
+    __shared__ double sw[64];
+    __shared__ int sQ[64];
+    __shared__ long long sRowBase[81];
+    __shared__ int sRowSx[81], sRowSy[81];
+    __shared__ unsigned char sMask[81];
+    GtCompact<ACC> *sCmp = reinterpret_cast<GtCompact<ACC> *>(sRegion);
+    ACC *sT = reinterpret_cast<ACC *>(sRegion + CMP_BYTES);
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const long long e = ((long long)ev[0] * m.ne[1] + ev[1]) * m.ne[2] + ev[2];
+    const int nq = m.nq, nqp = nq * nq * nq;
+    const ACC mass = (ACC)mass_d;
+
+    if (tid < 81) {
+        int A = tid / 3, i = tid % 3;
+        int a0 = ev[0] + A / 9, a1 = ev[1] + (A / 3) % 3, a2 = ev[2] + A % 3;
+        long long grow = 3 * node_of(m, a0, a1, a2) + i;
+        // u = slice 0 (t_{n+1}), u1 = slice 1 (t_n), u0 = slice 2 (t_{n-1})
+        const double u = Uh[grow], u1 = Uh[m.ndof + grow], u0 = Uh[2 * m.ndof + grow];
+        sU[tid] = (u0 + u1) / 2.0;
+        sU[81 + tid] = (u - u0) / 2.0;
+        int lo0 = max(0, a0 - 2), lo1 = max(0, a1 - 2), lo2 = max(0, a2 - 2);
+        int w1 = min(m.nb[1] - 1, a1 + 2) - lo1 + 1, w2 = min(m.nb[2] - 1, a2 + 2) - lo2 + 1;
+        sRowBase[tid] = rowptr[grow] + 3LL * (((long long)(ev[0] - lo0) * w1 + (ev[1] - lo1)) * w2 + (ev[2] - lo2));
+        sRowSx[tid] = 3 * w1 * w2;
+        sRowSy[tid] = 3 * w2;
+        sMask[tid] = dmask[grow];
+    } else if (tid >= 128 && tid < 192) {
+        int qp = tid - 128;
+        sQ[qp] = qp < nqp ? ((qp / (nq * nq)) | (((qp / nq) % nq) << 4) | ((qp % nq) << 8)) : 0;
+    }
+    fill_phi(m, ev, sphi, tid, 256);
+    __syncthreads();
+    const Coef c = load_coef(coef, e);
+
+    double *sScr = reinterpret_cast<double *>(sRegion + CMP_BYTES);
+    double ub[30], ud[30];
+    int qx = 0, qy = 0, qz = 0, qi = 0;
+    if (tid < nqp) {
+        int q = sQ[tid];
+        qx = q & 15; qy = (q >> 4) & 15; qz = (q >> 8) & 15; qi = qx * 16 + qy * 4 + qz;
+    }
+    interp_slots_sf(nq, sU, sphi, sScr, tid, 256);
+    if (tid < nqp) {
+#pragma unroll
+        for (int sl = 0; sl < 30; sl++) ub[sl] = sScr[324 + 864 + sl * 64 + qi];
+    }
+    __syncthreads();
+    interp_slots_sf(nq, sU + 81, sphi, sScr, tid, 256);
+    if (tid < nqp) {
+#pragma unroll
+        for (int sl = 0; sl < 30; sl++) ud[sl] = sScr[324 + 864 + sl * 64 + qi];
+        double w;
+        point_affine(m, ev, qx, qy, qz, ub, &w);
+        Jet<3> uj[30];
+#pragma unroll
+        for (int sl = 0; sl < 30; sl++) uj[sl] = Num<Jet<3>>::lin(ub[sl], ud[sl]);
+        Compact<Jet<3>> cp;
+        compact_of<Jet<3>>(uj, c, cp);
+        const double *src = reinterpret_cast<const double *>(&cp);
+        ACC *dst = reinterpret_cast<ACC *>(&sCmp[tid]);
+#pragma unroll
+        for (int f = 0; f < 144; f++) dst[f] = (ACC)src[f];
+        sw[tid] = w;
+    } else if (tid >= 128 && tid < 128 + 3 * 4 * 9) {
+        sphiA[tid - 128] = (ACC)sphi[tid - 128];
+    }
+    __syncthreads();
+    for (int idx = tid; idx < (G::KCP - G::KC) * 192; idx += 256) sT[G::KC * 192 + idx] = (ACC)0;
+
+    V acc[6];
+#pragma unroll
+    for (int t = 0; t < 6; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[t][r] = (ACC)0;
+    double accd[6][4];
+#pragma unroll
+    for (int t = 0; t < 6; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) accd[t][r] = 0.0;
+
+    const int A0 = lane & 15, A1 = A0 + 16, krow = lane >> 4;
+    const int ox0 = (A0 / 9) * 3, oy0 = ((A0 / 3) % 3) * 3, oz0 = (A0 % 3) * 3;
+    const int ox1 = (A1 < 27 ? A1 / 9 : 0) * 3, oy1 = (A1 < 27 ? (A1 / 3) % 3 : 0) * 3, oz1 = (A1 < 27 ? A1 % 3 : 0) * 3;
+    const ACC a1mask = A1 < 27 ? (ACC)1 : (ACC)0;
+
+    const int nchunk = (nqp + QC - 1) / QC;
+    for (int ch = 0; ch < nchunk; ch++) {
+        for (int t = lane; t < QC * 27; t += 64) {
+            int ql = t / 27, B = t % 27, qp = ch * QC + ql;
+            if (wave == 0) gt_task<2, 1, 2, ACC>(sCmp, sphiA, sQ, sw, sT, c, mass, ql, B, qp, nqp);
+            else if (wave == 1) gt_task<1, 0, 1, ACC>(sCmp, sphiA, sQ, sw, sT, c, mass, ql, B, qp, nqp);
+            else if (wave == 2) gt_task<0, 0, 0, ACC>(sCmp, sphiA, sQ, sw, sT, c, mass, ql, B, qp, nqp);
+            else gt_task<2, 0, 0, ACC>(sCmp, sphiA, sQ, sw, sT, c, mass, ql, B, qp, nqp);
+        }
+        __syncthreads();
+        const int ct0 = 3 * wave;
+        constexpr int NKS = G::KCP / 4;
+        ACC fa0[NKS], fa1[NKS], fb[NKS][3];
+#pragma unroll
+        for (int ks = 0; ks < NKS; ks++) {
+            const int k = 4 * ks + krow;
+            const int ql = k / 10, sl = k % 10, qp = ch * QC + ql;
+            const bool ok = k < G::KC && qp < nqp;
+            const int q = sQ[ok ? qp : 0];
+            const ACC *px = sphiA + (q & 15) * 9 + slot_d(sl, 0);
+            const ACC *py = sphiA + (4 + ((q >> 4) & 15)) * 9 + slot_d(sl, 1);
+            const ACC *pz = sphiA + (8 + ((q >> 8) & 15)) * 9 + slot_d(sl, 2);
+            const ACC okm = ok ? (ACC)1 : (ACC)0;
+            fa0[ks] = okm * (px[ox0] * py[oy0] * pz[oz0]);
+            fa1[ks] = (okm * a1mask) * (px[ox1] * py[oy1] * pz[oz1]);
+#pragma unroll
+            for (int t = 0; t < 3; t++) fb[ks][t] = sT[k * 192 + (ct0 + t) * 16 + (lane & 15)];
+        }
+#pragma unroll
+        for (int ks = 0; ks < NKS; ks++)
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+                acc[2 * t] = mfma16(fa0[ks], fb[ks][t], acc[2 * t]);
+                acc[2 * t + 1] = mfma16(fa1[ks], fb[ks][t], acc[2 * t + 1]);
+            }
+        if (sizeof(ACC) == 4) {
+#pragma unroll
+            for (int t = 0; t < 6; t++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    accd[t][r] += (double)acc[t][r];
+                    acc[t][r] = (ACC)0;
+                }
+        }
+        __syncthreads();
+    }
+    if (sizeof(ACC) == 4) {
+#pragma unroll
+        for (int t = 0; t < 6; t++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) acc[t][r] = (ACC)accd[t][r];
+    }
+
+    ACC *sKe = reinterpret_cast<ACC *>(sRegion);
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+        int ct = 3 * wave + t / 2, at = t & 1;
+        int i = ctile_block(ct);
+        int cl = ct * 16 + (lane & 15) - block_off(i);
+        if (cl >= (3 - i) * 27) continue;
+        int j = i + cl / 27, B = cl % 27;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            int A = at * 16 + acc_row((ACC)0, lane, r);
+            if (A >= 27) continue;
+            sKe[(3 * A + i) * 81 + 3 * B + j] = acc[t][r];
+            if (i != j) sKe[(3 * B + j) * 81 + 3 * A + i] = acc[t][r];
+        }
+    }
+    __syncthreads();
+    if (vals_f32)
+        scatter_rows<float, ACC, 256>(sKe, sRowBase, sRowSx, sRowSy, sMask, (float *)vals, tid);
+    else
+        scatter_rows<double, ACC, 256>(sKe, sRowBase, sRowSx, sRowSy, sMask, (double *)vals, tid);
+}
+
+// Gauss points per chunk of the getime tangent
+template <class ACC>
+struct GtQC { static constexpr int v = 2; };
+template <>
+struct GtQC<float> { static constexpr int v = 4; };
+template <class ACC>
+struct GtWaves { static constexpr int v = 1; };
+template <>
+struct GtWaves<float> { static constexpr int v = 2; };
+
+template <class ACC, int QC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GtWaves<ACC>::v))) void k_tangent_gt(
+    MeshArgs m, const double *__restrict__ Uh, const double *__restrict__ coef, double mass,
+    const long long *__restrict__ rowptr, const unsigned char *__restrict__ dmask, void *__restrict__ vals, int vals_f32,
+    int cx, int cy, int cz, int my, int mz)
+{
+    const int b = blockIdx.x;
+    const int ev[3] = {cx + 3 * (b / (mz * my)), cy + 3 * ((b / mz) % my), cz + 3 * (b % mz)};
+    tangent_gt_body<ACC, QC>(m, Uh, coef, mass, rowptr, dmask, vals, vals_f32, ev);
 }
 
 // diagonal of Dirichlet rows = 1 (MatZeroRowsColumns(..., 1.0, ...))
@@ -1382,23 +1450,6 @@ extern "C" int igap4h_residual_dev(igap4h_ctx *c, const double *d_Uh, double *d_
     return 0;
 }
 
-template <int PK, class ACC>
-static int launch_tangent(igap4h_ctx *c, const MeshArgs &m, const double *d_Uh, double mass, void *d_vals, int f32, hipStream_t st)
-{
-    for (int cx = 0; cx < 3; cx++)
-        for (int cy = 0; cy < 3; cy++)
-            for (int cz = 0; cz < 3; cz++) {
-                int mx = (m.ne[0] - cx + 2) / 3, my = (m.ne[1] - cy + 2) / 3, mz = (m.ne[2] - cz + 2) / 3;
-                long long nb = (long long)mx * my * mz;
-                if (nb <= 0) continue;
-                hipLaunchKernelGGL((k_tangent<PK, ACC>), dim3((unsigned)nb), dim3(TNT), 0, st, m, d_Uh,
-                                   (const double *)c->d_coef, mass, (const long long *)c->d_rowptr,
-                                   (const unsigned char *)c->d_mask, d_vals, f32, cx, cy, cz, my, mz);
-                HIPCHK(hipGetLastError());
-            }
-    return 0;
-}
-
 // Gauss points per chunk: 2 for FP64 (keeps two elements per CU), 4 for FP32.
 template <class ACC>
 struct GeQC { static constexpr int v = 2; };
@@ -1422,6 +1473,23 @@ static int launch_tangent_ge(igap4h_ctx *c, const MeshArgs &m, const double *d_U
     return 0;
 }
 
+template <class ACC>
+static int launch_tangent_gt(igap4h_ctx *c, const MeshArgs &m, const double *d_Uh, double mass, void *d_vals, int f32, hipStream_t st)
+{
+    for (int cx = 0; cx < 3; cx++)
+        for (int cy = 0; cy < 3; cy++)
+            for (int cz = 0; cz < 3; cz++) {
+                int mx = (m.ne[0] - cx + 2) / 3, my = (m.ne[1] - cy + 2) / 3, mz = (m.ne[2] - cz + 2) / 3;
+                long long nb = (long long)mx * my * mz;
+                if (nb <= 0) continue;
+                hipLaunchKernelGGL((k_tangent_gt<ACC, GtQC<ACC>::v>), dim3((unsigned)nb), dim3(256), 0, st, m, d_Uh,
+                                   (const double *)c->d_coef, mass, (const long long *)c->d_rowptr,
+                                   (const unsigned char *)c->d_mask, d_vals, f32, cx, cy, cz, my, mz);
+                HIPCHK(hipGetLastError());
+            }
+    return 0;
+}
+
 extern "C" int igap4h_tangent_dev(igap4h_ctx *c, const double *d_Uh, int prec, int vals_type, void *d_vals, void *stream)
 {
     if (!c || !d_Uh || !d_vals) return fail(IGAP4H_EINVAL, "NULL argument");
@@ -1439,8 +1507,8 @@ extern "C" int igap4h_tangent_dev(igap4h_ctx *c, const double *d_Uh, int prec, i
         rc = prec == IGAP4H_FP64 ? launch_tangent_ge<double>(c, m, d_Uh, d_vals, f32, st)
                                  : launch_tangent_ge<float>(c, m, d_Uh, d_vals, f32, st);
     else
-        rc = prec == IGAP4H_FP64 ? launch_tangent<1, double>(c, m, d_Uh, mass, d_vals, f32, st)
-                                 : launch_tangent<1, float>(c, m, d_Uh, mass, d_vals, f32, st);
+        rc = prec == IGAP4H_FP64 ? launch_tangent_gt<double>(c, m, d_Uh, mass, d_vals, f32, st)
+                                 : launch_tangent_gt<float>(c, m, d_Uh, mass, d_vals, f32, st);
     if (rc) return rc;
     int nd = (int)c->drows.size();
     if (nd > 0) {

@@ -39,52 +39,58 @@ namespace igap4h {
 #define IGAP4H_HD __host__ __device__ __forceinline__
 
 // ---------------------------------------------------------------- jets
-template <int N>
+// Truncated Taylor series in one variable, coefficients of type R (double;
+// float for the FP32 tangent of phys/getime).
+template <int N, class R = double>
 struct Jet {
-    double c[N];
+    R c[N];
+};
+template <class X>
+struct NoDeduce {
+    using t = X;
 };
 
-template <int N>
-IGAP4H_HD Jet<N> operator+(Jet<N> a, const Jet<N> &b)
+template <int N, class R>
+IGAP4H_HD Jet<N, R> operator+(Jet<N, R> a, const Jet<N, R> &b)
 {
 #pragma unroll
     for (int k = 0; k < N; k++) a.c[k] += b.c[k];
     return a;
 }
-template <int N>
-IGAP4H_HD Jet<N> operator-(Jet<N> a, const Jet<N> &b)
+template <int N, class R>
+IGAP4H_HD Jet<N, R> operator-(Jet<N, R> a, const Jet<N, R> &b)
 {
 #pragma unroll
     for (int k = 0; k < N; k++) a.c[k] -= b.c[k];
     return a;
 }
-template <int N>
-IGAP4H_HD Jet<N> operator*(Jet<N> a, double s)
+template <int N, class R>
+IGAP4H_HD Jet<N, R> operator*(Jet<N, R> a, typename NoDeduce<R>::t s)
 {
 #pragma unroll
     for (int k = 0; k < N; k++) a.c[k] *= s;
     return a;
 }
-template <int N>
-IGAP4H_HD Jet<N> operator*(double s, Jet<N> a)
+template <int N, class R>
+IGAP4H_HD Jet<N, R> operator*(typename NoDeduce<R>::t s, Jet<N, R> a)
 {
     return a * s;
 }
-template <int N>
-IGAP4H_HD Jet<N> operator*(const Jet<N> &a, const Jet<N> &b)
+template <int N, class R>
+IGAP4H_HD Jet<N, R> operator*(const Jet<N, R> &a, const Jet<N, R> &b)
 {
-    Jet<N> r;
+    Jet<N, R> r;
 #pragma unroll
     for (int k = 0; k < N; k++) {
-        double t = 0.0;
+        R t = R(0);
 #pragma unroll
         for (int i = 0; i <= k; i++) t += a.c[i] * b.c[k - i];
         r.c[k] = t;
     }
     return r;
 }
-template <int N>
-IGAP4H_HD Jet<N> &operator+=(Jet<N> &a, const Jet<N> &b)
+template <int N, class R>
+IGAP4H_HD Jet<N, R> &operator+=(Jet<N, R> &a, const Jet<N, R> &b)
 {
     a = a + b;
     return a;
@@ -97,20 +103,20 @@ struct Num<double> {
     static IGAP4H_HD double cst(double x) { return x; }
     static IGAP4H_HD double lin(double x0, double) { return x0; }
 };
-template <int N>
-struct Num<Jet<N>> {
-    static IGAP4H_HD Jet<N> cst(double x)
+template <int N, class R>
+struct Num<Jet<N, R>> {
+    static IGAP4H_HD Jet<N, R> cst(double x)
     {
-        Jet<N> r;
+        Jet<N, R> r;
 #pragma unroll
-        for (int k = 0; k < N; k++) r.c[k] = 0.0;
-        r.c[0] = x;
+        for (int k = 0; k < N; k++) r.c[k] = R(0);
+        r.c[0] = (R)x;
         return r;
     }
-    static IGAP4H_HD Jet<N> lin(double x0, double x1)
+    static IGAP4H_HD Jet<N, R> lin(double x0, double x1)
     {
-        Jet<N> r = cst(x0);
-        if (N > 1) r.c[1] = x1;
+        Jet<N, R> r = cst(x0);
+        if (N > 1) r.c[1] = (R)x1;
         return r;
     }
 };
@@ -118,13 +124,13 @@ struct Num<Jet<N>> {
 // (1/2)(c0/2 + c1/3 + c2/4): averaged tangent of a degree-2 series
 IGAP4H_HD double avg_tangent(double x) { return x; }
 IGAP4H_HD float avg_tangent(float x) { return x; }
-template <int N>
-IGAP4H_HD double avg_tangent(const Jet<N> &x)
+template <int N, class R>
+IGAP4H_HD R avg_tangent(const Jet<N, R> &x)
 {
-    double s = x.c[0] / 2.0;
-    if (N > 1) s += x.c[1] / 3.0;
-    if (N > 2) s += x.c[2] / 4.0;
-    return 0.5 * s;
+    R s = x.c[0] / R(2);
+    if (N > 1) s += x.c[1] / R(3);
+    if (N > 2) s += x.c[2] / R(4);
+    return R(0.5) * s;
 }
 // sum_k c_k/(k+1): averaged stress of a degree-3 series
 IGAP4H_HD double avg_residual(double x) { return x; }

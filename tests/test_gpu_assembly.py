@@ -131,6 +131,29 @@ def test_getime(oracle):
     assert rel(v32, v0) < TOL32, rel(v32, v0)
 
 
+def test_getime_colours(oracle):
+    """4^3 getime (several elements per colour launch), nq = 4: residual and
+    the three tangent forms against the oracle."""
+    say("getime 4^3")
+    pb = P.reference_example(4, physics=P.GETIME)
+    rng = np.random.default_rng(11)
+    n = pb.ndof
+    U0 = P.smooth_state(pb, amp=0.05, noise=0.0, nhist=1) + rng.uniform(-1, 1, n) * 0.01
+    U1 = U0 + rng.uniform(-1, 1, n) * 0.002
+    U = U1 + rng.uniform(-1, 1, n) * 0.002
+    Uh = np.concatenate([U, U1, U0])
+    app = App(pb)
+    app.U_hist[:] = Uh
+    assert rel(app.residual(U), oracle.residual(pb, Uh)) < TOL64
+    rowptr, col = app.pattern()
+    _, _, v0 = oracle.tangent(pb, Uh, rowptr, col)
+    assert rel(app.tangent(U), v0) < TOL64, rel(app.tangent(U), v0)
+    v32 = app.tangent(U, prec=L.IGAP4H_FP32)
+    assert rel(v32, v0) < TOL32, rel(v32, v0)
+    v32f = app.tangent(U, prec=L.IGAP4H_FP32, vals_type=L.IGAP4H_VALS_F32)
+    assert rel(v32f.astype(np.float64), v0) < TOL32
+
+
 def test_energy_gradient_on_gpu(oracle):
     """R = dPi/dU on interior rows, Pi from the device energy kernel."""
     import torch
