@@ -31,6 +31,10 @@ sys.path.insert(0, ROOT)
 # algorithmic flops of K_e per element and Gauss point: 9 dof blocks x
 # (2*10*10*27 + 2*27*10*27) = 179,820  (i.e. 11,508,480 per element at nq = 4)
 FLOP_KE_PER_QP = 9 * (2 * 10 * 10 * 27 + 2 * 27 * 10 * 27)
+# algorithmic flops of R_e per Gauss point: u-slot interpolation 1,620 +
+# projection 1,620 + basis products 540 + the pointwise stress 291 (ge,
+# counted on igap4h_physics.hpp's residual_grad: 121 adds, 170 multiplies)
+FLOP_RE_PER_QP = 1620 + 1620 + 540 + 291
 PEAK = {"f64": 78.6, "f32": 157.3}   # MI355X dense MFMA TFLOP/s
 PEAK_HBM = 8000.0                    # GB/s
 
@@ -38,7 +42,7 @@ PEAK_HBM = 8000.0                    # GB/s
 def profiled_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
     summaries (FETCH_SIZE and WRITE_SIZE runs, scaled by a plain copy)."""
-    for name in ("k_tangent_ge_64cube_traffic.txt", "k_spmv_64cube_traffic.txt"):
+    for name in ("k_tangent_ge_64cube_traffic.txt", "k_spmv_line_64cube_traffic.txt"):
         try:
             for line in open(os.path.join(ROOT, "profiles", name)):
                 if line.startswith("traffic_bytes_per_launch") and kernel in line:
@@ -163,7 +167,8 @@ def main():
         ms = wall * 1000.0 / args.asm_steps
         asm[name] = {"ms_per_assembly": ms, "elements_per_s": pb.n_elem / (ms / 1000.0), "residual_ms": r_ms,
                      "tangent_ms": k_ms,
-                     "tangent_tflops": pb.n_elem * FLOP_KE_PER_QP * pb.nquad ** 3 / (k_ms * 1e-3) / 1e12}
+                     "tangent_tflops": pb.n_elem * FLOP_KE_PER_QP * pb.nquad ** 3 / (k_ms * 1e-3) / 1e12,
+                     "residual_tflops": pb.n_elem * FLOP_RE_PER_QP * pb.nquad ** 3 / (r_ms * 1e-3) / 1e12}
         say("assembly", name, json.dumps(asm[name]))
 
     # ---- fine-level row pass (y = K x), the traversal every Krylov/multigrid pass makes
@@ -224,7 +229,7 @@ def main():
     main_sp = spmv.get("fp64") or spmv[wanted_asm[0]]
     main_nt = newton.get("fp64") or (newton[wanted[0]] if newton else None)
     tangent_kernel = "k_tangent_ge<double,2>" if "fp64" in asm else "k_tangent_ge<float,4>"
-    spmv_kernel = "k_spmv<double,0,0>" if "fp64" in spmv else "k_spmv<float,0,0>"
+    spmv_kernel = "k_spmv_line<double,0,0>" if "fp64" in spmv else "k_spmv_line<float,0,0>"
     if main_nt:
         value, ms_step = main_nt["elements_per_s"], main_nt["ms_per_newton_step"]
     else:
@@ -256,10 +261,21 @@ def main():
                              "unit": "TFLOP/s", "frac": main_asm["tangent_tflops"] / PEAK["f64"],
                              "traffic": profiled_traffic(tangent_kernel), "kernel": tangent_kernel + " (27 colour launches per tangent)",
                              "algorithmic": "11,508,480 flop per element (9 x (2*10*10*27 + 2*27*10*27) per Gauss point)"},
+        "roofline_residual": {"bound": "fp64 valu", "achieved": main_asm["residual_tflops"], "peak": PEAK["f64"],
+                              "unit": "TFLOP/s", "frac": main_asm["residual_tflops"] / PEAK["f64"],
+                              "kernel": "k_residual_elem<0> (sum-factorised; executes ~71k of the algorithmic flops)",
+                              "algorithmic": "%d flop per element (%d per Gauss point)" % (FLOP_RE_PER_QP * pb.nquad ** 3,
+                                                                                     FLOP_RE_PER_QP)},
         "newton": newton,
         "assembly": asm,
         "spmv": spmv,
     }
+    if "mixed" in asm:
+        line["roofline_tangent_f32"] = {"bound": "mfma", "achieved": asm["mixed"]["tangent_tflops"], "peak": PEAK["f32"],
+                                        "unit": "TFLOP/s", "frac": asm["mixed"]["tangent_tflops"] / PEAK["f32"],
+                                        "traffic": profiled_traffic("k_tangent_ge<float,4>"),
+                                        "kernel": "k_tangent_ge<float,4> (27 colour launches per tangent)",
+                                        "algorithmic": "11,508,480 flop per element"}
     if "mixed" in asm and "fp64" in asm:
         line["assembly"]["mixed"]["roofline_frac_f32"] = asm["mixed"]["tangent_tflops"] / PEAK["f32"]
         line["assembly"]["mixed"]["tangent_speedup_vs_fp64"] = asm["fp64"]["tangent_ms"] / asm["mixed"]["tangent_ms"]
