@@ -598,12 +598,28 @@ __device__ __forceinline__ void tangent_ge_body(const MeshArgs &m, const double 
         // T_ij, lane = (ql, B); the six (i <= j) blocks over the four waves:
         // wave 0 (i = 1, 2; j = 2), wave 1 (i = 0, 1; j = 1), wave 2
         // (i = 0; j = 0), wave 3 (i = 0; j = 2)
-        for (int t = lane; t < QC * 27; t += 64) {
-            int ql = t / 27, B = t % 27, qp = ch * QC + ql;
-            if (wave == 0) ge_task<2, 1, 2, ACC>(sCmp, sphiA, sQ, sw, sT, c, ql, B, qp, nqp);
-            else if (wave == 1) ge_task<1, 0, 1, ACC>(sCmp, sphiA, sQ, sw, sT, c, ql, B, qp, nqp);
-            else if (wave == 2) ge_task<0, 0, 0, ACC>(sCmp, sphiA, sQ, sw, sT, c, ql, B, qp, nqp);
-            else ge_task<2, 0, 0, ACC>(sCmp, sphiA, sQ, sw, sT, c, ql, B, qp, nqp);
+        if (QC * 27 > 64) {
+            // two lane-rounds of tasks: waves 0-1 take dof block j = 2 (rows
+            // 0-2, one preamble) for the first / second round, waves 2-3 take
+            // j = 1 and j = 0 (rows 0-1 and 0) for the first / second round
+            const int t = ((wave & 1) ? 64 : 0) + lane;
+            if (t < QC * 27) {
+                const int ql = t / 27, B = t % 27, qp = ch * QC + ql;
+                if (wave < 2) {
+                    ge_task<2, 0, 2, ACC>(sCmp, sphiA, sQ, sw, sT, c, ql, B, qp, nqp);
+                } else {
+                    ge_task<1, 0, 1, ACC>(sCmp, sphiA, sQ, sw, sT, c, ql, B, qp, nqp);
+                    ge_task<0, 0, 0, ACC>(sCmp, sphiA, sQ, sw, sT, c, ql, B, qp, nqp);
+                }
+            }
+        } else {
+            for (int t = lane; t < QC * 27; t += 64) {
+                int ql = t / 27, B = t % 27, qp = ch * QC + ql;
+                if (wave == 0) ge_task<2, 1, 2, ACC>(sCmp, sphiA, sQ, sw, sT, c, ql, B, qp, nqp);
+                else if (wave == 1) ge_task<1, 0, 1, ACC>(sCmp, sphiA, sQ, sw, sT, c, ql, B, qp, nqp);
+                else if (wave == 2) ge_task<0, 0, 0, ACC>(sCmp, sphiA, sQ, sw, sT, c, ql, B, qp, nqp);
+                else ge_task<2, 0, 0, ACC>(sCmp, sphiA, sQ, sw, sT, c, ql, B, qp, nqp);
+            }
         }
         __syncthreads();
         const int ct0 = 3 * wave;
@@ -840,12 +856,28 @@ __device__ __forceinline__ void tangent_gt_body(const MeshArgs &m, const double 
 
     const int nchunk = (nqp + QC - 1) / QC;
     for (int ch = 0; ch < nchunk; ch++) {
-        for (int t = lane; t < QC * 27; t += 64) {
-            int ql = t / 27, B = t % 27, qp = ch * QC + ql;
-            if (wave == 0) gt_task<2, 1, 2, ACC>(sCmp, sphiA, sQ, sw, sT, c, mass, ql, B, qp, nqp);
-            else if (wave == 1) gt_task<1, 0, 1, ACC>(sCmp, sphiA, sQ, sw, sT, c, mass, ql, B, qp, nqp);
-            else if (wave == 2) gt_task<0, 0, 0, ACC>(sCmp, sphiA, sQ, sw, sT, c, mass, ql, B, qp, nqp);
-            else gt_task<2, 0, 0, ACC>(sCmp, sphiA, sQ, sw, sT, c, mass, ql, B, qp, nqp);
+        if (QC * 27 > 64) {
+            // two lane-rounds of tasks: waves 0-1 take dof block j = 2 (rows
+            // 0-2, one preamble) for the first / second round, waves 2-3 take
+            // j = 1 and j = 0 (rows 0-1 and 0) for the first / second round
+            const int t = ((wave & 1) ? 64 : 0) + lane;
+            if (t < QC * 27) {
+                const int ql = t / 27, B = t % 27, qp = ch * QC + ql;
+                if (wave < 2) {
+                    gt_task<2, 0, 2, ACC>(sCmp, sphiA, sQ, sw, sT, c, mass, ql, B, qp, nqp);
+                } else {
+                    gt_task<1, 0, 1, ACC>(sCmp, sphiA, sQ, sw, sT, c, mass, ql, B, qp, nqp);
+                    gt_task<0, 0, 0, ACC>(sCmp, sphiA, sQ, sw, sT, c, mass, ql, B, qp, nqp);
+                }
+            }
+        } else {
+            for (int t = lane; t < QC * 27; t += 64) {
+                int ql = t / 27, B = t % 27, qp = ch * QC + ql;
+                if (wave == 0) gt_task<2, 1, 2, ACC>(sCmp, sphiA, sQ, sw, sT, c, mass, ql, B, qp, nqp);
+                else if (wave == 1) gt_task<1, 0, 1, ACC>(sCmp, sphiA, sQ, sw, sT, c, mass, ql, B, qp, nqp);
+                else if (wave == 2) gt_task<0, 0, 0, ACC>(sCmp, sphiA, sQ, sw, sT, c, mass, ql, B, qp, nqp);
+                else gt_task<2, 0, 0, ACC>(sCmp, sphiA, sQ, sw, sT, c, mass, ql, B, qp, nqp);
+            }
         }
         __syncthreads();
         const int ct0 = 3 * wave;
