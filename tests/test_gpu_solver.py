@@ -41,7 +41,10 @@ def csr(app, vals):
     return sp.csr_matrix((np.asarray(vals, np.float64), col, rowptr), shape=(app.ndof, app.ndof))
 
 
-@pytest.mark.parametrize("nelem", [(3, 4, 5), (6, 6, 6)])
+# (2, 3, 65): 67 nodes along z, two line segments of 34 + 33; (1, 2, 128):
+# 130 nodes, three segments of 44 + 44 + 42 (the 16-bit smoothing row pass
+# stages x and maps columns with the same line_stage / line_col)
+@pytest.mark.parametrize("nelem", [(3, 4, 5), (6, 6, 6), (2, 3, 65), (1, 2, 128)])
 @pytest.mark.parametrize("vt", [L.IGAP4H_VALS_F64, L.IGAP4H_VALS_F32])
 def test_spmv_matches_host_csr(torch_dev, nelem, vt):
     torch, dev = torch_dev
