@@ -213,6 +213,46 @@ __device__ __forceinline__ void interp_slots_sf(int nq, const double *sU, const 
     __syncthreads();
 }
 
+// As interp_slots_sf for one wave, with the last (x) stage evaluated by the
+// lane of each Gauss point into its own registers: scratch is V1 and V2 only
+// (1,188 doubles).  Ends without a barrier after the register stage.
+constexpr int SF2_SCRATCH = 324 + 864;
+__device__ __forceinline__ void interp_slots_sf2(int nq, const double *sU, const double *sphi, double *scr, int tid, int nthr,
+                                                 bool active, int qx, int qy, int qz, double u[30])
+{
+    double *V1 = scr, *V2 = scr + 324;
+    for (int idx = tid; idx < 324; idx += nthr) {
+        const int qz2 = idx & 3, dz = (idx >> 2) % 3, r = idx / 12;
+        if (qz2 < nq) {
+            const int ay = r % 3, ax = (r / 3) % 3, i = r / 9;
+            const double *z = sphi + (8 + qz2) * 9 + dz;
+            const double *uu = sU + 3 * (9 * ax + 3 * ay) + i;
+            V1[idx] = z[0] * uu[0] + z[3] * uu[3] + z[6] * uu[6];
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < 864; idx += nthr) {
+        const int qz2 = idx & 3, qy2 = (idx >> 2) & 3, c = (idx >> 4) % 6, r = idx / 96;
+        if (qy2 < nq && qz2 < nq) {
+            const int dy = (0x012010 >> (4 * c)) & 15, dz = (0x210100 >> (4 * c)) & 15;
+            const double *y = sphi + (4 + qy2) * 9 + dy;
+            const double *v = V1 + (r * 9 + dz) * 4 + qz2;
+            V2[idx] = y[0] * v[0] + y[3] * v[12] + y[6] * v[24];
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int sl = 0; sl < 30; sl++) {
+            const int i = sl / 10, sv = sl % 10;
+            const int dx = (int)((0x0001120010ULL >> (4 * sv)) & 15), c = (int)((0x5432102100ULL >> (4 * sv)) & 15);
+            const double *x = sphi + qx * 9 + dx;
+            const double *v = V2 + ((i * 3) * 6 + c) * 16 + qy * 4 + qz;
+            u[sl] = x[0] * v[0] + x[3] * v[96] + x[6] * v[192];
+        }
+    }
+}
+
 // periodic F0 terms of one point (snesiga_Residual.c:125-144) and its weight
 __device__ __forceinline__ void point_affine(const MeshArgs &m, const int e[3], int qx, int qy, int qz, double u[30],
                                              double *w)
@@ -242,10 +282,11 @@ __device__ __forceinline__ Coef load_coef(const double *coef, long long e)
 
 // ============================================================== residual
 // One wave per element.  The u-slots at the Gauss points come from a sum-
-// factorised interpolation (interp_slots_sf), the pointwise stress runs with
+// factorised interpolation (interp_slots_sf2), the pointwise stress runs with
 // lane = Gauss point, and the weak-form projection
 //   R_e[3A + i] = sum_q w_q sum_s N_{A,s}(q) r_{i,s}(q)   (snesiga_Residual.c:147-159)
-// is sum-factorised the other way: x contracted first, then y, then z.
+// is sum-factorised the other way (x contracted first, then y, then z), one
+// dof i at a time so that the scratch stays at 1,188 doubles.
 // Writes R_e[81] to Re; k_gather sums it per dof.
 template <int PK>
 __device__ __forceinline__ void residual_elem_body(const MeshArgs &m, const double *__restrict__ Uh,
@@ -254,7 +295,7 @@ __device__ __forceinline__ void residual_elem_body(const MeshArgs &m, const doub
     constexpr int NH = PhysT<PK>::NH;
     __shared__ double sU[NH * 81];
     __shared__ double sphi[3 * 4 * 9];
-    __shared__ double sScr[SF_SCRATCH];
+    __shared__ double sScr[SF2_SCRATCH];
     const int tid = threadIdx.x;
     int ev[3];
     ev[2] = (int)(e % m.ne[2]);
@@ -268,21 +309,18 @@ __device__ __forceinline__ void residual_elem_body(const MeshArgs &m, const doub
     fill_phi(m, ev, sphi, tid, 64);
     __syncthreads();
     const int nq = m.nq, nqp = nq * nq * nq;
+    const bool active = tid < nqp;
     const Coef c = load_coef(coef, e);
     const int qx = tid / (nq * nq), qy = (tid / nq) % nq, qz = tid % nq, qi = qx * 16 + qy * 4 + qz;
     double u[NH][30];
     for (int h = 0; h < NH; h++) {
-        interp_slots_sf(nq, sU + h * 81, sphi, sScr, tid, 64);
-        if (tid < nqp) {
-#pragma unroll
-            for (int sl = 0; sl < 30; sl++) u[h][sl] = sScr[324 + 864 + sl * 64 + qi];
-        }
+        interp_slots_sf2(nq, sU + h * 81, sphi, sScr, tid, 64, active, qx, qy, qz, u[h]);
         __syncthreads();
     }
     double r[30];
     for (int s = 0; s < 30; s++) r[s] = 0.0;
     double w = 0.0;
-    if (tid < nqp) {
+    if (active) {
         for (int h = 0; h < NH; h++) point_affine(m, ev, qx, qy, qz, u[h], &w);
         if (PK == 0) {
             residual_grad<double>(u[0], c, r);
@@ -299,59 +337,63 @@ __device__ __forceinline__ void residual_elem_body(const MeshArgs &m, const doub
                        (m.rho * (u[0][s] + u[NH - 1][s] - 2 * u[1 % NH][s])) / (m.dt * m.dt);
             }
         }
-        // weighted point values, [30 slots][qx*16 + qy*4 + qz]
+    }
+    double *rw = sScr, *P1 = sScr + 640, *P2 = sScr + 640 + 288;
+    for (int i = 0; i < 3; i++) {
+        // weighted point values of dof i, [10 slots][qx*16 + qy*4 + qz]
+        if (active) {
 #pragma unroll
-        for (int sl = 0; sl < 30; sl++) sScr[sl * 64 + qi] = w * r[sl];
-    }
-    __syncthreads();
-    double *rw = sScr, *P1 = sScr + 1920, *P2 = sScr + 1920 + 864;
-    // x: P1[i][ax][c][qy][qz] = sum_qx sum_{s: (dy,dz)(s) = c} X(qx, ax, dx(s)) rw[10i + s]
-    //    c = (dy,dz) in (00,10,01,20,11,02); slots of c with dx = 0, 1, 2:
-    //    (0,1,4) (2,5,-) (3,6,-) (7,-,-) (8,-,-) (9,-,-)
-    for (int idx = tid; idx < 864; idx += 64) {
-        const int qz2 = idx & 3, qy2 = (idx >> 2) & 3, cc = (idx >> 4) % 6, r2 = idx / 96; // r2 = i*3 + ax
-        if (qy2 < nq && qz2 < nq) {
-            const int i = r2 / 3, ax = r2 % 3;
-            const int s0 = 10 * i + ((0x987320 >> (4 * cc)) & 15), s1 = 10 * i + ((0x000651 >> (4 * cc)) & 15), s2 = 10 * i + 4;
-            const double m1 = cc < 3 ? 1.0 : 0.0, m2 = cc == 0 ? 1.0 : 0.0;
-            double t = 0.0;
-            for (int qx2 = 0; qx2 < nq; qx2++) {
-                const double *x = sphi + qx2 * 9 + ax * 3;
-                const int o = qx2 * 16 + qy2 * 4 + qz2;
-                t += x[0] * rw[s0 * 64 + o] + m1 * (x[1] * rw[s1 * 64 + o]) + m2 * (x[2] * rw[s2 * 64 + o]);
+            for (int sv = 0; sv < 10; sv++) rw[sv * 64 + qi] = w * r[10 * i + sv];
+        }
+        __syncthreads();
+        // x: P1[ax][c][qy][qz] = sum_qx sum_{s: (dy,dz)(s) = c} X(qx, ax, dx(s)) rw[s];
+        //    c = (dy,dz) in (00,10,01,20,11,02); slots of c with dx = 0, 1, 2:
+        //    (0,1,4) (2,5,-) (3,6,-) (7,-,-) (8,-,-) (9,-,-)
+        for (int idx = tid; idx < 288; idx += 64) {
+            const int qz2 = idx & 3, qy2 = (idx >> 2) & 3, cc = (idx >> 4) % 6, ax = idx / 96;
+            if (qy2 < nq && qz2 < nq) {
+                const int s0 = (0x987320 >> (4 * cc)) & 15, s1 = (0x000651 >> (4 * cc)) & 15;
+                const double m1 = cc < 3 ? 1.0 : 0.0, m2 = cc == 0 ? 1.0 : 0.0;
+                double t = 0.0;
+                for (int qx2 = 0; qx2 < nq; qx2++) {
+                    const double *x = sphi + qx2 * 9 + ax * 3;
+                    const int o = qx2 * 16 + qy2 * 4 + qz2;
+                    t += x[0] * rw[s0 * 64 + o] + m1 * (x[1] * rw[s1 * 64 + o]) + m2 * (x[2] * rw[4 * 64 + o]);
+                }
+                P1[idx] = t;
             }
-            P1[idx] = t;
         }
-    }
-    __syncthreads();
-    // y: P2[i][ax][ay][dz][qz] = sum_qy sum_{c: dz(c) = dz} Y(qy, ay, dy(c)) P1[i][ax][c][qy][qz];
-    //    combos of dz with dy = 0, 1, 2: (0,1,3) (2,4,-) (5,-,-)
-    for (int idx = tid; idx < 324; idx += 64) {
-        const int qz2 = idx & 3, dz = (idx >> 2) % 3, r3 = idx / 12; // r3 = (i*3 + ax)*3 + ay
-        if (qz2 < nq) {
-            const int ay = r3 % 3, r2 = r3 / 3;
-            const int c0 = (0x520 >> (4 * dz)) & 15, c1 = (0x041 >> (4 * dz)) & 15, c2 = 3;
-            const double m1 = dz < 2 ? 1.0 : 0.0, m2 = dz == 0 ? 1.0 : 0.0;
-            double t = 0.0;
-            for (int qy2 = 0; qy2 < nq; qy2++) {
-                const double *y = sphi + (4 + qy2) * 9 + ay * 3;
-                const double *p = P1 + r2 * 96 + qy2 * 4 + qz2;
-                t += y[0] * p[c0 * 16] + m1 * (y[1] * p[c1 * 16]) + m2 * (y[2] * p[c2 * 16]);
+        __syncthreads();
+        // y: P2[ax][ay][dz][qz] = sum_qy sum_{c: dz(c) = dz} Y(qy, ay, dy(c)) P1[ax][c][qy][qz];
+        //    combos of dz with dy = 0, 1, 2: (0,1,3) (2,4,-) (5,-,-)
+        for (int idx = tid; idx < 108; idx += 64) {
+            const int qz2 = idx & 3, dz = (idx >> 2) % 3, r3 = idx / 12;  // r3 = ax*3 + ay
+            if (qz2 < nq) {
+                const int ay = r3 % 3, ax = r3 / 3;
+                const int c0 = (0x520 >> (4 * dz)) & 15, c1 = (0x041 >> (4 * dz)) & 15, c2 = 3;
+                const double m1 = dz < 2 ? 1.0 : 0.0, m2 = dz == 0 ? 1.0 : 0.0;
+                double t = 0.0;
+                for (int qy2 = 0; qy2 < nq; qy2++) {
+                    const double *y = sphi + (4 + qy2) * 9 + ay * 3;
+                    const double *p = P1 + ax * 96 + qy2 * 4 + qz2;
+                    t += y[0] * p[c0 * 16] + m1 * (y[1] * p[c1 * 16]) + m2 * (y[2] * p[c2 * 16]);
+                }
+                P2[idx] = t;
             }
-            P2[idx] = t;
         }
-    }
-    __syncthreads();
-    // z: R_e[3A + i] = sum_qz sum_dz Z(qz, az, dz) P2[i][ax][ay][dz][qz]
-    for (int col = tid; col < 81; col += 64) {
-        const int A = col / 3, i = col % 3, ax = A / 9, ay = (A / 3) % 3, az = A % 3;
-        const double *p = P2 + ((i * 3 + ax) * 3 + ay) * 12;
-        double t = 0.0;
-        for (int qz2 = 0; qz2 < nq; qz2++) {
-            const double *z = sphi + (8 + qz2) * 9 + az * 3;
-            t += z[0] * p[qz2] + z[1] * p[4 + qz2] + z[2] * p[8 + qz2];
+        __syncthreads();
+        // z: R_e[3A + i] = sum_qz sum_dz Z(qz, az, dz) P2[ax][ay][dz][qz]
+        if (tid < 27) {
+            const int A = tid, ax = A / 9, ay = (A / 3) % 3, az = A % 3;
+            const double *p = P2 + (ax * 3 + ay) * 12;
+            double t = 0.0;
+            for (int qz2 = 0; qz2 < nq; qz2++) {
+                const double *z = sphi + (8 + qz2) * 9 + az * 3;
+                t += z[0] * p[qz2] + z[1] * p[4 + qz2] + z[2] * p[8 + qz2];
+            }
+            Re[e * 81 + 3 * A + i] = t;
         }
-        Re[e * 81 + col] = t;
+        __syncthreads();
     }
 }
 
