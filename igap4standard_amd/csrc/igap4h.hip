@@ -496,7 +496,6 @@ template <int QC>
 struct GeTile {
     static constexpr int KC = QC * 9;                 // contraction depth per chunk
     static constexpr int KCP = (KC + 3) / 4 * 4;      // padded to the MFMA k of 4
-    static constexpr int NKA = 0;                     // A operand formed in registers
     static constexpr int NTS = KCP * 192;             // T strip
 };
 
@@ -554,7 +553,7 @@ __device__ __forceinline__ void tangent_ge_body(const MeshArgs &m, const double 
     using V = typename V4<ACC>::t;
     // region: [compact SoA | T strip], reused for the element matrix
     constexpr int CMP_BYTES = NCMP * 64 * (int)sizeof(ACC);
-    constexpr int OP_BYTES0 = (G::NKA + G::NTS) * (int)sizeof(ACC);
+    constexpr int OP_BYTES0 = G::NTS * (int)sizeof(ACC);
     // the operand area doubles as the interpolation scratch of the prologue
     constexpr int OP_BYTES = OP_BYTES0 > SF_SCRATCH * 8 ? OP_BYTES0 : SF_SCRATCH * 8;
     constexpr int KE_BYTES = 81 * 81 * (int)sizeof(ACC);
@@ -569,8 +568,7 @@ __device__ __forceinline__ void tangent_ge_body(const MeshArgs &m, const double 
     __shared__ int sRowSx[81], sRowSy[81];
     __shared__ unsigned char sMask[81];
     ACC *sCmp = reinterpret_cast<ACC *>(sRegion);
-    ACC *sNk = reinterpret_cast<ACC *>(sRegion + CMP_BYTES);
-    ACC *sT = sNk + G::NKA;
+    ACC *sT = reinterpret_cast<ACC *>(sRegion + CMP_BYTES);
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const long long e = ((long long)ev[0] * m.ne[1] + ev[1]) * m.ne[2] + ev[2];
@@ -757,7 +755,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(ACC)
 // slot carries the mass term rho/dt^2 + c/(2 dt) (phys/getime/tangent.c:4127).
 // ubar and (u - u0)/2 are interpolated directly (interpolation is linear);
 // the periodic F0 terms belong to ubar only.
-using JetA3f = Jet<3, float>;
 template <class ACC>
 using GtCompact = Compact<Jet<3, ACC>>;
 
