@@ -44,6 +44,8 @@ struct MeshArgs {
     int ne[3];
     int nb[3];
     const double *knot[3];
+    const double *phi;  // 1D basis tables, [(phioff[d] + e_d) * 4 + q][9]
+    int phioff[3];
     int nq;
     double cq[4], wq[4];
     double pp[9];     // par_periodic (F0)
@@ -90,24 +92,47 @@ __device__ __forceinline__ f4 mfma16(float a, float b, f4 c)
 __device__ __forceinline__ int acc_row(double, int l, int r) { return (l >> 4) + 4 * r; }
 __device__ __forceinline__ int acc_row(float, int l, int r) { return 4 * (l >> 4) + r; }
 
+// 1D basis values of element e_d of direction d at Gauss point q: o[a*3 + k],
+// k = value, first, second derivative of function a (eval_Bspline.c:9-93)
+__device__ __forceinline__ void phi_1d(const MeshArgs &m, int d, int ed, int q, double *o)
+{
+    const double *k = m.knot[d] + ed;
+    double X0 = k[2], X1 = k[3];
+    double xi = m.cq[q];
+    double x = (-xi + 1.0) * X0 + xi * X1;
+    double v[3], d1[3], d2[3];
+    bspline1d(k, x, v, d1, d2);
+    for (int a = 0; a < 3; a++) {
+        o[a * 3 + 0] = v[a];
+        o[a * 3 + 1] = d1[a];
+        o[a * 3 + 2] = d2[a];
+    }
+}
+
+// the tables of every element and direction, one thread per (d, e_d, q)
+__global__ void k_phi_table(MeshArgs m, double *tab)
+{
+    const int n = (m.ne[0] + m.ne[1] + m.ne[2]) * 4;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int row = t / 4, q = t % 4;
+    const int d = row < m.ne[0] ? 0 : (row < m.ne[0] + m.ne[1] ? 1 : 2);
+    const int ed = row - m.phioff[d];
+    double o[9];
+    if (q < m.nq) phi_1d(m, d, ed, q, o);
+    else
+        for (int j = 0; j < 9; j++) o[j] = 0.0;
+    for (int j = 0; j < 9; j++) tab[(size_t)t * 9 + j] = o[j];
+}
+
 // 1D basis tables of one element: phi[(d*4 + q)*9 + a*3 + k], k = value,
-// first, second derivative of function a at Gauss point q in direction d.
+// first, second derivative of function a at Gauss point q in direction d,
+// copied from the context's tables
 __device__ __forceinline__ void fill_phi(const MeshArgs &m, const int e[3], double *sphi, int tid, int nthr)
 {
-    for (int idx = tid; idx < 3 * m.nq; idx += nthr) {
-        int d = idx / m.nq, q = idx % m.nq;
-        const double *k = m.knot[d] + e[d];
-        double X0 = k[2], X1 = k[3];
-        double xi = m.cq[q];
-        double x = (-xi + 1.0) * X0 + xi * X1;
-        double v[3], d1[3], d2[3];
-        bspline1d(k, x, v, d1, d2);
-        double *o = sphi + (d * 4 + q) * 9;
-        for (int a = 0; a < 3; a++) {
-            o[a * 3 + 0] = v[a];
-            o[a * 3 + 1] = d1[a];
-            o[a * 3 + 2] = d2[a];
-        }
+    for (int idx = tid; idx < 3 * 4 * 9; idx += nthr) {
+        const int d = idx / 36, r = idx % 36;
+        sphi[idx] = m.phi[(size_t)(m.phioff[d] + e[d]) * 36 + r];
     }
 }
 
@@ -1198,6 +1223,10 @@ static MeshArgs mesh_args(const igap4h_ctx *c)
         m.nb[d] = c->mesh.nb[d];
         m.knot[d] = c->d_knot[d];
     }
+    m.phi = c->d_phi;
+    m.phioff[0] = 0;
+    m.phioff[1] = m.ne[0];
+    m.phioff[2] = m.ne[0] + m.ne[1];
     m.nq = c->nquad;
     for (int q = 0; q < 4; q++) {
         m.cq[q] = q < c->nquad ? c->cq[q] : 0.0;
@@ -1218,6 +1247,7 @@ static void ctx_free(igap4h_ctx *c)
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(c->device);
     for (int d = 0; d < 3; d++) (void)hipFree(c->d_knot[d]);
+    (void)hipFree(c->d_phi);
     (void)hipFree(c->d_coef);
     (void)hipFree(c->d_rowptr);
     (void)hipFree(c->d_col);
@@ -1391,6 +1421,13 @@ extern "C" int igap4h_create(const IGA *app, const igap4h_opts *opts, igap4h_ctx
     ALLOC(c->d_drows, c->drows.size());
     if (!c->drows.empty()) CPY(c->d_drows, c->drows.data(), c->drows.size() * sizeof(int32_t));
     ALLOC(c->d_Re, (size_t)ne * 81);
+    // 1D basis tables of every element (the nquad points used)
+    ALLOC(c->d_phi, (size_t)(m.ne[0] + m.ne[1] + m.ne[2]) * 36);
+    {
+        MeshArgs mt = mesh_args(c);
+        const int nt = (m.ne[0] + m.ne[1] + m.ne[2]) * 4;
+        hipLaunchKernelGGL(k_phi_table, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, 0, mt, c->d_phi);
+    }
     // CSR pattern on the device
     MeshArgs ma = mesh_args(c);
     {

@@ -32,6 +32,7 @@ struct igap4h_ctx {
     std::vector<int32_t> drows;
     // device
     double *d_knot[3] = {nullptr, nullptr, nullptr};
+    double *d_phi = nullptr;     // 1D basis tables [d][element][Gauss point][9], built at create
     double *d_coef = nullptr;
     long long *d_rowptr = nullptr;
     int *d_col = nullptr;
