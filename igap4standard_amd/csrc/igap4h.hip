@@ -242,13 +242,16 @@ __device__ __forceinline__ void interp_slots_sf(int nq, const double *sU, const 
 // lane of each Gauss point into its own registers: scratch is V1 and V2 only
 // (1,188 doubles).  Ends without a barrier after the register stage.
 constexpr int SF2_SCRATCH = 324 + 864;
-__device__ __forceinline__ void interp_slots_sf2(int nq, const double *sU, const double *sphi, double *scr, int tid, int nthr,
+template <int NTHR>
+__device__ __forceinline__ void interp_slots_sf2(int nq, const double *sU, const double *sphi, double *scr, int tid,
                                                  bool active, int qx, int qy, int qz, double u[30])
 {
     double *V1 = scr, *V2 = scr + 324;
-    for (int idx = tid; idx < 324; idx += nthr) {
+#pragma unroll
+    for (int it = 0; it < (324 + NTHR - 1) / NTHR; it++) {
+        const int idx = tid + it * NTHR;
         const int qz2 = idx & 3, dz = (idx >> 2) % 3, r = idx / 12;
-        if (qz2 < nq) {
+        if (idx < 324 && qz2 < nq) {
             const int ay = r % 3, ax = (r / 3) % 3, i = r / 9;
             const double *z = sphi + (8 + qz2) * 9 + dz;
             const double *uu = sU + 3 * (9 * ax + 3 * ay) + i;
@@ -256,9 +259,11 @@ __device__ __forceinline__ void interp_slots_sf2(int nq, const double *sU, const
         }
     }
     __syncthreads();
-    for (int idx = tid; idx < 864; idx += nthr) {
+#pragma unroll
+    for (int it = 0; it < (864 + NTHR - 1) / NTHR; it++) {
+        const int idx = tid + it * NTHR;
         const int qz2 = idx & 3, qy2 = (idx >> 2) & 3, c = (idx >> 4) % 6, r = idx / 96;
-        if (qy2 < nq && qz2 < nq) {
+        if (idx < 864 && qy2 < nq && qz2 < nq) {
             const int dy = (0x012010 >> (4 * c)) & 15, dz = (0x210100 >> (4 * c)) & 15;
             const double *y = sphi + (4 + qy2) * 9 + dy;
             const double *v = V1 + (r * 9 + dz) * 4 + qz2;
@@ -339,7 +344,7 @@ __device__ __forceinline__ void residual_elem_body(const MeshArgs &m, const doub
     const int qx = tid / (nq * nq), qy = (tid / nq) % nq, qz = tid % nq, qi = qx * 16 + qy * 4 + qz;
     double u[NH][30];
     for (int h = 0; h < NH; h++) {
-        interp_slots_sf2(nq, sU + h * 81, sphi, sScr, tid, 64, active, qx, qy, qz, u[h]);
+        interp_slots_sf2<64>(nq, sU + h * 81, sphi, sScr, tid, active, qx, qy, qz, u[h]);
         __syncthreads();
     }
     double r[30];
@@ -374,13 +379,17 @@ __device__ __forceinline__ void residual_elem_body(const MeshArgs &m, const doub
         // x: P1[ax][c][qy][qz] = sum_qx sum_{s: (dy,dz)(s) = c} X(qx, ax, dx(s)) rw[s];
         //    c = (dy,dz) in (00,10,01,20,11,02); slots of c with dx = 0, 1, 2:
         //    (0,1,4) (2,5,-) (3,6,-) (7,-,-) (8,-,-) (9,-,-)
-        for (int idx = tid; idx < 288; idx += 64) {
+#pragma unroll
+        for (int it = 0; it < 5; it++) {
+            const int idx = tid + 64 * it;
             const int qz2 = idx & 3, qy2 = (idx >> 2) & 3, cc = (idx >> 4) % 6, ax = idx / 96;
-            if (qy2 < nq && qz2 < nq) {
+            if (idx < 288 && qy2 < nq && qz2 < nq) {
                 const int s0 = (0x987320 >> (4 * cc)) & 15, s1 = (0x000651 >> (4 * cc)) & 15;
                 const double m1 = cc < 3 ? 1.0 : 0.0, m2 = cc == 0 ? 1.0 : 0.0;
                 double t = 0.0;
-                for (int qx2 = 0; qx2 < nq; qx2++) {
+#pragma unroll
+                for (int qx2 = 0; qx2 < 4; qx2++) {
+                    if (qx2 >= nq) break;
                     const double *x = sphi + qx2 * 9 + ax * 3;
                     const int o = qx2 * 16 + qy2 * 4 + qz2;
                     t += x[0] * rw[s0 * 64 + o] + m1 * (x[1] * rw[s1 * 64 + o]) + m2 * (x[2] * rw[4 * 64 + o]);
@@ -391,14 +400,18 @@ __device__ __forceinline__ void residual_elem_body(const MeshArgs &m, const doub
         __syncthreads();
         // y: P2[ax][ay][dz][qz] = sum_qy sum_{c: dz(c) = dz} Y(qy, ay, dy(c)) P1[ax][c][qy][qz];
         //    combos of dz with dy = 0, 1, 2: (0,1,3) (2,4,-) (5,-,-)
-        for (int idx = tid; idx < 108; idx += 64) {
+#pragma unroll
+        for (int it = 0; it < 2; it++) {
+            const int idx = tid + 64 * it;
             const int qz2 = idx & 3, dz = (idx >> 2) % 3, r3 = idx / 12;  // r3 = ax*3 + ay
-            if (qz2 < nq) {
+            if (idx < 108 && qz2 < nq) {
                 const int ay = r3 % 3, ax = r3 / 3;
                 const int c0 = (0x520 >> (4 * dz)) & 15, c1 = (0x041 >> (4 * dz)) & 15, c2 = 3;
                 const double m1 = dz < 2 ? 1.0 : 0.0, m2 = dz == 0 ? 1.0 : 0.0;
                 double t = 0.0;
-                for (int qy2 = 0; qy2 < nq; qy2++) {
+#pragma unroll
+                for (int qy2 = 0; qy2 < 4; qy2++) {
+                    if (qy2 >= nq) break;
                     const double *y = sphi + (4 + qy2) * 9 + ay * 3;
                     const double *p = P1 + ax * 96 + qy2 * 4 + qz2;
                     t += y[0] * p[c0 * 16] + m1 * (y[1] * p[c1 * 16]) + m2 * (y[2] * p[c2 * 16]);
@@ -412,7 +425,9 @@ __device__ __forceinline__ void residual_elem_body(const MeshArgs &m, const doub
             const int A = tid, ax = A / 9, ay = (A / 3) % 3, az = A % 3;
             const double *p = P2 + (ax * 3 + ay) * 12;
             double t = 0.0;
-            for (int qz2 = 0; qz2 < nq; qz2++) {
+#pragma unroll
+            for (int qz2 = 0; qz2 < 4; qz2++) {
+                if (qz2 >= nq) break;
                 const double *z = sphi + (8 + qz2) * 9 + az * 3;
                 t += z[0] * p[qz2] + z[1] * p[4 + qz2] + z[2] * p[8 + qz2];
             }
