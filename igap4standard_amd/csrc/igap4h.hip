@@ -201,13 +201,15 @@ __device__ __forceinline__ void interp_slots(const MeshArgs &m, const double *sU
 // c = (dy,dz) in (00,10,01,20,11,02), out [30 slots][qx*16 + qy*4 + qz] 1920.
 // Ends with a barrier.  Periodic F0 terms and weights are left to the caller.
 constexpr int SF_SCRATCH = 324 + 864 + 1920;
-__device__ __forceinline__ void interp_slots_sf(int nq, const double *sU, const double *sphi, double *scr, int tid,
-                                                int nthr)
+template <int NTHR>
+__device__ __forceinline__ void interp_slots_sf(int nq, const double *sU, const double *sphi, double *scr, int tid)
 {
     double *V1 = scr, *V2 = scr + 324, *uo = scr + 324 + 864;
-    for (int idx = tid; idx < 324; idx += nthr) {
+#pragma unroll
+    for (int it = 0; it < (324 + NTHR - 1) / NTHR; it++) {
+        const int idx = tid + it * NTHR;
         const int qz = idx & 3, dz = (idx >> 2) % 3, r = idx / 12; // r = (i*3 + ax)*3 + ay
-        if (qz < nq) {
+        if (idx < 324 && qz < nq) {
             const int ay = r % 3, ax = (r / 3) % 3, i = r / 9;
             const double *z = sphi + (8 + qz) * 9 + dz;
             const double *u = sU + 3 * (9 * ax + 3 * ay) + i;
@@ -215,9 +217,11 @@ __device__ __forceinline__ void interp_slots_sf(int nq, const double *sU, const 
         }
     }
     __syncthreads();
-    for (int idx = tid; idx < 864; idx += nthr) {
+#pragma unroll
+    for (int it = 0; it < (864 + NTHR - 1) / NTHR; it++) {
+        const int idx = tid + it * NTHR;
         const int qz = idx & 3, qy = (idx >> 2) & 3, c = (idx >> 4) % 6, r = idx / 96; // r = i*3 + ax
-        if (qy < nq && qz < nq) {
+        if (idx < 864 && qy < nq && qz < nq) {
             const int dy = (0x012010 >> (4 * c)) & 15, dz = (0x210100 >> (4 * c)) & 15;
             const double *y = sphi + (4 + qy) * 9 + dy;
             const double *v = V1 + (r * 9 + dz) * 4 + qz; // ay stride 12
@@ -225,10 +229,12 @@ __device__ __forceinline__ void interp_slots_sf(int nq, const double *sU, const 
         }
     }
     __syncthreads();
-    for (int idx = tid; idx < 1920; idx += nthr) {
+#pragma unroll
+    for (int it = 0; it < (1920 + NTHR - 1) / NTHR; it++) {
+        const int idx = tid + it * NTHR;
         const int q = idx & 63, sl = idx >> 6, i = sl / 10, sv = sl % 10;
         const int qx = q >> 4, qy = (q >> 2) & 3, qz = q & 3;
-        if (qx < nq && qy < nq && qz < nq) {
+        if (idx < 1920 && qx < nq && qy < nq && qz < nq) {
             const int dx = (int)((0x0001120010ULL >> (4 * sv)) & 15), c = (int)((0x5432102100ULL >> (4 * sv)) & 15);
             const double *x = sphi + qx * 9 + dx;
             const double *v = V2 + ((i * 3) * 6 + c) * 16 + qy * 4 + qz; // ax stride 96
@@ -634,7 +640,7 @@ __device__ __forceinline__ void tangent_ge_body(const MeshArgs &m, const double 
     const Coef c = load_coef(coef, e);
 
     double *sScr = reinterpret_cast<double *>(sRegion + CMP_BYTES);
-    interp_slots_sf(nq, sU, sphi, sScr, tid, 256);
+    interp_slots_sf<256>(nq, sU, sphi, sScr, tid);
     if (tid < nqp) {
         int q = sQ[tid];
         const int qx = q & 15, qy = (q >> 4) & 15, qz = (q >> 8) & 15, qi = qx * 16 + qy * 4 + qz;
@@ -889,13 +895,13 @@ __device__ __forceinline__ void tangent_gt_body(const MeshArgs &m, const double 
         int q = sQ[tid];
         qx = q & 15; qy = (q >> 4) & 15; qz = (q >> 8) & 15; qi = qx * 16 + qy * 4 + qz;
     }
-    interp_slots_sf(nq, sU, sphi, sScr, tid, 256);
+    interp_slots_sf<256>(nq, sU, sphi, sScr, tid);
     if (tid < nqp) {
 #pragma unroll
         for (int sl = 0; sl < 30; sl++) ub[sl] = sScr[324 + 864 + sl * 64 + qi];
     }
     __syncthreads();
-    interp_slots_sf(nq, sU + 81, sphi, sScr, tid, 256);
+    interp_slots_sf<256>(nq, sU + 81, sphi, sScr, tid);
     if (tid < nqp) {
 #pragma unroll
         for (int sl = 0; sl < 30; sl++) ud[sl] = sScr[324 + 864 + sl * 64 + qi];
