@@ -545,21 +545,14 @@ struct GeTile {
     static constexpr int NTS = KCP * 192;             // T strip
 };
 
-// compact data of one Gauss point, structure of arrays (stride 64), rounded
-// once to the tangent's arithmetic type ACC
+// compact data of one Gauss point (array of structures, read in place by the
+// T phase), rounded once to the tangent's arithmetic type ACC
 template <class ACC>
-__device__ __forceinline__ void store_compact_soa(ACC *sC, int qp, const Compact<double> &cp)
+__device__ __forceinline__ void store_compact_aos(ACC *sC, int qp, const Compact<double> &cp)
 {
     const double *v = reinterpret_cast<const double *>(&cp);
 #pragma unroll
-    for (int f = 0; f < NCMP; f++) sC[f * 64 + qp] = (ACC)v[f];
-}
-template <class ACC>
-__device__ __forceinline__ void load_compact_soa(const ACC *sC, int qp, Compact<ACC> &cp)
-{
-    ACC *v = reinterpret_cast<ACC *>(&cp);
-#pragma unroll
-    for (int f = 0; f < NCMP; f++) v[f] = sC[f * 64 + qp];
+    for (int f = 0; f < NCMP; f++) sC[qp * NCMP + f] = (ACC)v[f];
 }
 
 // T_ij(q, B) for dof block j = J and rows i of [I0, I1], in the arithmetic
@@ -573,8 +566,7 @@ __device__ __forceinline__ void ge_task(const ACC *sCmp, const ACC *sphi, const 
         ACC n[10];
         int q = sQ[qp];
         basis_slots<ACC>(sphi, q & 15, (q >> 4) & 15, (q >> 8) & 15, B, n);
-        Compact<ACC> cp;
-        load_compact_soa<ACC>(sCmp, qp, cp);
+        const Compact<ACC> &cp = reinterpret_cast<const Compact<ACC> *>(sCmp)[qp];  // read from LDS at use
         tangent_apply<ACC, J, I0, I1, ACC>(cp, c, n, (ACC)0, (ACC)sw[qp], out);
     } else {
 #pragma unroll
@@ -650,7 +642,7 @@ __device__ __forceinline__ void tangent_ge_body(const MeshArgs &m, const double 
         point_affine(m, ev, qx, qy, qz, u, &w);
         Compact<double> cp;
         compact_of<double>(u, c, cp);
-        store_compact_soa<ACC>(sCmp, tid, cp);
+        store_compact_aos<ACC>(sCmp, tid, cp);
         sw[tid] = w;
     } else if (tid >= 128 && tid < 128 + 3 * 4 * 9) {
         sphiA[tid - 128] = (ACC)sphi[tid - 128];
