@@ -526,11 +526,21 @@ __global__ __launch_bounds__(256) void k_mdot(const double *__restrict__ V, long
         double acc[JT];
 #pragma unroll
         for (int t = 0; t < JT; t++) acc[t] = 0.0;
-        for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-            double wi = w[i];
+        // four grid-stride steps per pass, all their loads issued together
+        const long long stride = (long long)gridDim.x * blockDim.x;
+        for (long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += 4 * stride) {
+            double wi[4], vt[4][JT];
 #pragma unroll
-            for (int t = 0; t < JT; t++)
-                if (j0 + t < k) acc[t] += V[(long long)(j0 + t) * ld + i] * wi;
+            for (int u = 0; u < 4; u++) {
+                const long long i = i0 + u * stride;
+                wi[u] = i < n ? w[i] : 0.0;
+#pragma unroll
+                for (int t = 0; t < JT; t++) vt[u][t] = (i < n && j0 + t < k) ? V[(long long)(j0 + t) * ld + i] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int t = 0; t < JT; t++) acc[t] += vt[u][t] * wi[u];
         }
 #pragma unroll
         for (int t = 0; t < JT; t++) {
@@ -552,7 +562,15 @@ __global__ __launch_bounds__(256) void k_mgemv(const double *__restrict__ V, lon
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         double s = 0.0;
-        for (int j = 0; j < k; j++) s += V[(long long)j * ld + i] * h[j];
+        // eight basis vectors per step, their loads issued together
+        for (int j0 = 0; j0 < k; j0 += 8) {
+            double v[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) v[t] = j0 + t < k ? V[(long long)(j0 + t) * ld + i] : 0.0;
+#pragma unroll
+            for (int t = 0; t < 8; t++)
+                if (j0 + t < k) s += v[t] * h[j0 + t];
+        }
         w[i] += alpha * s;
     }
 }
