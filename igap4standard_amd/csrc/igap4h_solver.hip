@@ -235,12 +235,22 @@ __global__ __launch_bounds__(LW * 64) void k_spmv_line(SpArgs s, LineGeom g, con
 }
 
 // ---------------------------------------------------------------- 16-bit smoothing copies
-// Node n's three rows sit back to back from noff[n], each padded to an even
-// length Lp = L + (L & 1) with a zero, so a lane reads two values per 4-byte
-// load.  bfloat16 = the upper half of the FP32 bit pattern, rounded to nearest
+// The copy is interleaved for one node per lane: nodes are taken in packs of
+// 64 consecutive indices, and word u (8 values of 16 bits) of node 64 p + l
+// sits at v4[(p * NU16 + u) * 64 + l], so a wave's load of word u is 1 KiB
+// contiguous.  The words come in 5 phases of NUP, one per neighbour plane
+// d0 = -2..2; in phase d0, value v = 9 o + 3 i + j (v < 225) is
+// K[(n, i), (n + d, j)] with d = (d0, o / 5 - 2, o % 5 - 2) over the full
+// stencil; entries outside the mesh and the phase's last 7 values are zero.
+// bfloat16 = the upper half of the FP32 bit pattern, rounded to nearest
 // even.  binary16 (F16 = 1): row r is divided by rs[r] = 2^e with
 // max_k |K_rk| < 2^e first, exactly, so every entry lies in (-1, 1).
-__host__ __device__ __forceinline__ int even_len(int L) { return L + (L & 1); }
+constexpr int NUP = 29;          // words per phase: ceil(225 / 8)
+constexpr int NU16 = 5 * NUP;    // words per node
+constexpr int CH_NT = 256;       // k_cheb16 threads (nodes) per block
+
+// x slab of one phase of k_cheb16, in doubles: 3 components x (CH_NT + 4 nb2 + 4) nodes
+inline size_t cheb16_lds_bytes(int nb2) { return (size_t)3 * (CH_NT + 4 * nb2 + 4) * sizeof(double); }
 
 __device__ __forceinline__ unsigned short f32_to_bf16(float f)
 {
@@ -255,104 +265,154 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
-template <class VT, int F16>
-__global__ __launch_bounds__(256) void k_pack_16(SpArgs s, const VT *__restrict__ vals, const long long *__restrict__ noff,
-                                                 unsigned short *__restrict__ out, double *__restrict__ rs)
+// rs[3 n + r] = 2^e > max_k |K_(3n+r),k| (1 for a zero row); one wave per node
+template <class VT>
+__global__ __launch_bounds__(256) void k_rowscale_16(SpArgs s, const VT *__restrict__ vals, double *__restrict__ rs)
 {
     const int lane = threadIdx.x & 63;
     const long long node = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (node >= s.nnode) return;
     int a[3], lo[3], w[3];
     node_box(s, node, a, lo, w);
-    const int L = 3 * w[0] * w[1] * w[2], Lp = even_len(L);
-    const long long base = s.rowptr[3 * node], ob = noff[node];
+    const int L = 3 * w[0] * w[1] * w[2];
+    const long long base = s.rowptr[3 * node];
     for (int r = 0; r < 3; r++) {
-        const VT *row = vals + base + (long long)r * L;
-        double inv = 1.0;
-        if (F16) {
-            double m = 0.0;
-            for (int k = lane; k < L; k += 64) m = fmax(m, fabs((double)row[k]));
-            m = wave_max(m);
-            int e = 0;
-            (void)frexp(m, &e);  // m < 2^e
-            const double sc = m > 0.0 ? ldexp(1.0, e) : 1.0;
-            inv = 1.0 / sc;  // exact (power of two)
-            if (lane == 0) rs[3 * node + r] = sc;
-        }
-        for (int k = lane; k < Lp; k += 64) {
-            unsigned short h = 0;
-            if (k < L) {
-                if (F16) {
-                    _Float16 v = (_Float16)((double)row[k] * inv);
-                    h = __builtin_bit_cast(unsigned short, v);
-                } else {
-                    h = f32_to_bf16((float)row[k]);
-                }
-            }
-            out[ob + (long long)r * Lp + k] = h;
-        }
+        double m = 0.0;
+        for (int k = lane; k < L; k += 64) m = fmax(m, fabs((double)vals[base + (long long)r * L + k]));
+        m = wave_max(m);
+        int e = 0;
+        (void)frexp(m, &e);  // m < 2^e
+        if (lane == 0) rs[3 * node + r] = m > 0.0 ? ldexp(1.0, e) : 1.0;
     }
 }
 
-__device__ __forceinline__ float lo16(unsigned u, int f16)
+// one thread per (node, word): gather the word's 8 values from the AIJ rows
+template <class VT, int F16>
+__global__ __launch_bounds__(256) void k_pack_16(SpArgs s, const VT *__restrict__ vals, const double *__restrict__ rs,
+                                                 uint4 *__restrict__ v4, long long nthreads)
 {
-    return f16 ? (float)__builtin_bit_cast(_Float16, (unsigned short)(u & 0xFFFFu)) : __uint_as_float(u << 16);
-}
-__device__ __forceinline__ float hi16(unsigned u, int f16)
-{
-    return f16 ? (float)__builtin_bit_cast(_Float16, (unsigned short)(u >> 16)) : __uint_as_float(u & 0xFFFF0000u);
-}
-
-// y = K~ x (RES = 0) or y = b - K~ x (RES = 1) on a 16-bit copy, line-staged
-// x as k_spmv_line, two columns per lane and 4-byte load
-template <int RES, int F16>
-__global__ __launch_bounds__(LW * 64) void k_spmv_16(SpArgs s, LineGeom g, const unsigned *__restrict__ v32,
-                                                 const long long *__restrict__ noff, const double *__restrict__ rs,
-                                                 const double *__restrict__ x, const double *__restrict__ b,
-                                                 double *__restrict__ y)
-{
-    extern __shared__ double sx[];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    int a0, a1, z0, z1;
-    line_stage(s, g, xcd_remap(blockIdx.x, gridDim.x), x, sx, a0, a1, z0, z1);
-    const int zb = z0 - 2, SW3 = 3 * (g.seglen + 4);
-    for (int z = z0 + wv; z < z1; z += LW) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nthreads) return;
+    const int l = (int)(t & 63);
+    const long long q = t >> 6, p = q / NU16;
+    const int u = (int)(q - p * NU16), ph = u / NUP;
+    const long long node = 64 * p + l;
+    unsigned short h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (node < s.nnode) {
         int a[3], lo[3], w[3];
-        line_box(s, a0, a1, z, a, lo, w);
-        const long long node = ((long long)a0 * s.nb[1] + a1) * s.nb[2] + z;
-        const int L = 3 * w[0] * w[1] * w[2], Lp = even_len(L);
-        const float r2 = 1.0f / (float)w[2], r1 = 1.0f / (float)w[1];
-        const long long ob = noff[node] >> 1;  // in 4-byte words
-        double xa[3], xb[3];
-        unsigned u0[3], u1[3], u2[3];
-#pragma unroll
-        for (int it = 0; it < 3; it++) {
-            const int p = lane + 64 * it, k = 2 * p;
-            xa[it] = xb[it] = 0.0;
-            u0[it] = u1[it] = u2[it] = 0u;
-            if (k < L) {
-                u0[it] = v32[ob + p];
-                u1[it] = v32[ob + (Lp >> 1) + p];
-                u2[it] = v32[ob + Lp + p];
-                xa[it] = sx[line_col(a, lo, w, r1, r2, zb, SW3, k)];
-                if (k + 1 < L) xb[it] = sx[line_col(a, lo, w, r1, r2, zb, SW3, k + 1)];
+        node_box(s, node, a, lo, w);
+        const int L = 3 * w[0] * w[1] * w[2];
+        const long long base = s.rowptr[3 * node];
+        for (int e = 0; e < 8; e++) {
+            const int v = 8 * (u - ph * NUP) + e;
+            if (v >= 225) break;
+            const int o = v / 9, i = (v % 9) / 3, j = v % 3;
+            const int b0 = a[0] + ph - 2 - lo[0], b1 = a[1] + o / 5 - 2 - lo[1], b2 = a[2] + o % 5 - 2 - lo[2];
+            if (b0 < 0 || b0 >= w[0] || b1 < 0 || b1 >= w[1] || b2 < 0 || b2 >= w[2]) continue;
+            const double x = (double)vals[base + (long long)i * L + 3 * ((b0 * w[1] + b1) * w[2] + b2) + j];
+            if (F16) {
+                _Float16 f = (_Float16)(x / rs[3 * node + i]);  // exact division by a power of two
+                h[e] = __builtin_bit_cast(unsigned short, f);
+            } else {
+                h[e] = f32_to_bf16((float)x);
             }
         }
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    }
+    v4[t] = make_uint4(h[0] | (unsigned)h[1] << 16, h[2] | (unsigned)h[3] << 16, h[4] | (unsigned)h[5] << 16,
+                       h[6] | (unsigned)h[7] << 16);
+}
+
+template <int F16>
+__device__ __forceinline__ float half_at(const uint4 &w, int e)
+{
+    const unsigned u = (e >> 1) == 0 ? w.x : (e >> 1) == 1 ? w.y : (e >> 1) == 2 ? w.z : w.w;
+    if (F16) return (float)__builtin_bit_cast(_Float16, (unsigned short)((e & 1) ? u >> 16 : u & 0xFFFFu));
+    return __uint_as_float((e & 1) ? u & 0xFFFF0000u : u << 16);
+}
+
+// acc[i] += the node's 3 x 3 blocks at in-plane offsets O0 .. O0 + NO - 1 of
+// one phase times x from the slab; w points at the group's first word (a
+// stream read once: non-temporal), sx at the lane's node in the slab.
+template <int NO, int O0, int F16>
+__device__ __forceinline__ void sm16_group(const uint4 *__restrict__ w, const double *sx, int Z, int SL, double acc[3])
+{
+    typedef unsigned u4v __attribute__((ext_vector_type(4)));
+    constexpr int NW = (9 * NO + 7) / 8;
+    uint4 wd[NW];
 #pragma unroll
-        for (int it = 0; it < 3; it++) {
-            s0 += (double)lo16(u0[it], F16) * xa[it] + (double)hi16(u0[it], F16) * xb[it];
-            s1 += (double)lo16(u1[it], F16) * xa[it] + (double)hi16(u1[it], F16) * xb[it];
-            s2 += (double)lo16(u2[it], F16) * xa[it] + (double)hi16(u2[it], F16) * xb[it];
+    for (int u = 0; u < NW; u++) {
+        const u4v t4 = __builtin_nontemporal_load((const u4v *)(w + u * 64));
+        wd[u] = make_uint4(t4.x, t4.y, t4.z, t4.w);
+    }
+    double xv[NO][3];
+#pragma unroll
+    for (int k = 0; k < NO; k++) {
+        const int o = O0 + k;
+        const double *xs = sx + (o / 5) * Z + (o % 5);
+#pragma unroll
+        for (int j = 0; j < 3; j++) xv[k][j] = xs[j * SL];
+    }
+#pragma unroll
+    for (int v = 0; v < 9 * NO; v++) acc[(v % 9) / 3] += (double)half_at<F16>(wd[v >> 3], v & 7) * xv[v / 9][v % 3];
+}
+
+// Fine-level Chebyshev pass on the 16-bit copy, one lane per node (rows 3n..3n+2):
+//   MODE 1: r = b - K~ x, d = c2 Dinv r                       (start from x)
+//   MODE 2: r -= K~ din, x += din, and if newd: dout = c1 din + c2 Dinv r
+// For each phase d0 the block stages the x entries of nodes
+// [n0 + d0 P - 2 Z - 2, n0 + CH_NT + d0 P + 2 Z + 2) (P = nb1 nb2, Z = nb2, n0 the
+// block's first node) in LDS, one array per component; nodes outside the
+// mesh stage zeros.  Blocks take XCD-contiguous node ranges.
+template <int MODE, int F16>
+__global__ __launch_bounds__(CH_NT) void k_cheb16(SpArgs s, const uint4 *__restrict__ v4, const double *__restrict__ rs,
+                                                  const double *__restrict__ Dinv, const double *__restrict__ xin,
+                                                  const double *__restrict__ b, double *__restrict__ r, double *__restrict__ x,
+                                                  double *__restrict__ dout, double c1, double c2, int newd)
+{
+    extern __shared__ double sx[];
+    const long long n0 = xcd_remap(blockIdx.x, gridDim.x) * CH_NT;
+    const long long t = n0 + threadIdx.x;
+    const int l = threadIdx.x & 63;
+    const long long p = t >> 6;
+    const int Z = s.nb[2], P = s.nb[1] * s.nb[2], SL = CH_NT + 4 * Z + 4;
+    const long long nx = 3 * s.nnode;
+    const uint4 *w = v4 + p * NU16 * 64 + l;
+    double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int ph = 0; ph < 5; ph++) {
+        __syncthreads();
+        const long long g0 = 3 * (n0 + (long long)(ph - 2) * P - 2 * Z - 2);
+        for (int k = threadIdx.x; k < 3 * SL; k += CH_NT) {
+            const long long gi = g0 + k;
+            const int m = k / 3;
+            sx[(k - 3 * m) * SL + m] = gi >= 0 && gi < nx ? xin[gi] : 0.0;
         }
-        s0 = wave_sum(s0);
-        s1 = wave_sum(s1);
-        s2 = wave_sum(s2);
-        if (lane == 0) {
-            double acc[3] = {s0, s1, s2};
-            if (F16)
-                for (int i = 0; i < 3; i++) acc[i] *= rs[3 * node + i];
-            for (int i = 0; i < 3; i++) y[3 * node + i] = RES ? b[3 * node + i] - acc[i] : acc[i];
+        __syncthreads();
+        const uint4 *wp = w + ph * NUP * 64;
+        const double *xs = sx + threadIdx.x;
+        sm16_group<8, 0, F16>(wp, xs, Z, SL, acc);
+        sm16_group<8, 8, F16>(wp + 9 * 64, xs, Z, SL, acc);
+        sm16_group<8, 16, F16>(wp + 18 * 64, xs, Z, SL, acc);
+        sm16_group<1, 24, F16>(wp + 27 * 64, xs, Z, SL, acc);
+    }
+    if (t >= s.nnode) return;
+    if (F16)
+        for (int i = 0; i < 3; i++) acc[i] *= rs[3 * t + i];
+    double q[3];
+    for (int i = 0; i < 3; i++) {
+        if (MODE == 1) {
+            q[i] = b[3 * t + i] - acc[i];
+        } else {
+            q[i] = r[3 * t + i] - acc[i];
+            x[3 * t + i] += xin[3 * t + i];
+        }
+        r[3 * t + i] = q[i];
+    }
+    if (MODE == 1 || newd) {
+        const double *B = Dinv + 9 * t;
+        for (int i = 0; i < 3; i++) {
+            const double dq = B[3 * i] * q[0] + B[3 * i + 1] * q[1] + B[3 * i + 2] * q[2];
+            dout[3 * t + i] = MODE == 1 ? c2 * dq : c1 * xin[3 * t + i] + c2 * dq;
         }
     }
 }
@@ -1108,9 +1168,7 @@ struct igap4h::Solver {
     int key_f32 = -1, key_pre = -1, key_post = -1, key_smoother = -1;
     int key_svt = -1;
     const void *pc_vals = nullptr;  // matrix the preconditioner was last set up on
-    unsigned short *sm16 = nullptr; // bfloat16 copy of the fine matrix for smoothing (padded rows)
-    size_t sm16_n = 0;
-    long long *noff = nullptr;      // node offsets into sm16
+    uint4 *sm16 = nullptr;          // 16-bit copy of the fine matrix for smoothing (node-interleaved)
     double *rs16 = nullptr;         // row scales of the binary16 copy
     bool jac_valid = false;  // s->vals holds a tangent (of jac_key) for lagged Newton steps
     long long jac_key = -1;
@@ -1143,7 +1201,6 @@ void igap4h::solver_free(igap4h_ctx *c)
     if (!s) return;
     (void)hipFree(s->vals);
     (void)hipFree(s->sm16);
-    (void)hipFree(s->noff);
     (void)hipFree(s->rs16);
     for (double *p : {s->Dblk, s->Dinv, s->V, s->r, s->z, s->p, s->q, s->w, s->t, s->Y, s->U0, s->R, s->partial,
                       s->pspmv, s->hd, s->sc})
@@ -1394,56 +1451,51 @@ struct Op {
     int smoother = 0;             // 0 block Gauss-Seidel, 1 Chebyshev (degree nu_pre / nu_post)
     int svt = IGAP4H_SMOOTH_SAME; // matrix of the fine-level Chebyshev passes
 
-    // fine-level row pass of the smoother: the Krylov matrix or its 16-bit copy
-    int spmv_s(const double *x, double *y, const double *b = nullptr) const
+    // fine-level Chebyshev pass on the 16-bit copy (k_cheb16): MODE 1 starts
+    // from x (r = b - K~ x, d = c2 Dinv r), MODE 2 is one step on din
+    template <int MODE>
+    int cheb16(const double *xin, const double *b, double *r, double *x, double *dout, double c1, double c2, int newd) const
     {
-        if (svt == IGAP4H_SMOOTH_SAME) return spmv(x, y, b);
-        const LineGeom lg = line_geom(sp.nb);
-        const unsigned g = (unsigned)((long long)sp.nb[0] * sp.nb[1] * lg.nseg);
-        const size_t sh = line_lds_bytes(lg);
-        const unsigned *v32 = (const unsigned *)s->sm16;
-        const long long *no = s->noff;
-        const double *rs = s->rs16;
-        if (svt == IGAP4H_SMOOTH_F16) {
-            if (b) hipLaunchKernelGGL((k_spmv_16<1, 1>), dim3(g), dim3(LW * 64), sh, st, sp, lg, v32, no, rs, x, b, y);
-            else hipLaunchKernelGGL((k_spmv_16<0, 1>), dim3(g), dim3(LW * 64), sh, st, sp, lg, v32, no, rs, x, b, y);
-        } else {
-            if (b) hipLaunchKernelGGL((k_spmv_16<1, 0>), dim3(g), dim3(LW * 64), sh, st, sp, lg, v32, no, rs, x, b, y);
-            else hipLaunchKernelGGL((k_spmv_16<0, 0>), dim3(g), dim3(LW * 64), sh, st, sp, lg, v32, no, rs, x, b, y);
-        }
+        const unsigned g = (unsigned)((sp.nnode + CH_NT - 1) / CH_NT);
+        const size_t sh = cheb16_lds_bytes(sp.nb[2]);
+        if (svt == IGAP4H_SMOOTH_F16)
+            hipLaunchKernelGGL((k_cheb16<MODE, 1>), dim3(g), dim3(CH_NT), sh, st, sp, (const uint4 *)s->sm16, (const double *)s->rs16,
+                               (const double *)s->Dinv, xin, b, r, x, dout, c1, c2, newd);
+        else
+            hipLaunchKernelGGL((k_cheb16<MODE, 0>), dim3(g), dim3(CH_NT), sh, st, sp, (const uint4 *)s->sm16, (const double *)s->rs16,
+                               (const double *)s->Dinv, xin, b, r, x, dout, c1, c2, newd);
         HIPCHK(hipGetLastError());
         return 0;
     }
-    // the 16-bit copy of the current matrix (node offsets computed once)
+    // the 16-bit copy of the current matrix; packs of 64 nodes, their number a
+    // multiple of CH_NT / 64 so that k_cheb16's blocks stay inside the copy
     int pack_16() const
     {
-        if (!s->noff) {
-            std::vector<long long> off((size_t)sp.nnode + 1);
-            off[0] = 0;
-            for (long long n = 0; n < sp.nnode; n++) {
-                int a[3] = {(int)(n / ((long long)sp.nb[2] * sp.nb[1])), (int)((n / sp.nb[2]) % sp.nb[1]), (int)(n % sp.nb[2])};
-                int L = 3;
-                for (int d = 0; d < 3; d++) L *= std::min(sp.nb[d] - 1, a[d] + 2) - std::max(0, a[d] - 2) + 1;
-                off[n + 1] = off[n] + 3LL * even_len(L);
+        const long long npk = (sp.nnode + CH_NT - 1) / CH_NT * (CH_NT / 64);
+        if (!s->sm16) {
+            if (cheb16_lds_bytes(sp.nb[2]) > 65536)
+                return fail(IGAP4H_EINVAL, "16-bit smoothing copy: %d nodes in direction 2 (limit 617)", sp.nb[2]);
+            const size_t bytes = (size_t)npk * NU16 * 64 * sizeof(uint4);
+            if (hipMalloc(&s->sm16, bytes) != hipSuccess) {
+                s->sm16 = nullptr;
+                return fail(IGAP4H_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
             }
             int rc;
-            if ((rc = dev_alloc(&s->noff, off.size()))) return rc;
-            HIPCHK(hipMemcpy(s->noff, off.data(), off.size() * sizeof(long long), hipMemcpyHostToDevice));
-            s->sm16_n = (size_t)off.back();
-            if (hipMalloc(&s->sm16, s->sm16_n * sizeof(unsigned short)) != hipSuccess) {
-                s->sm16 = nullptr;
-                return fail(IGAP4H_ENOMEM, "hipMalloc of %zu bytes failed", s->sm16_n * sizeof(unsigned short));
-            }
             if ((rc = dev_alloc(&s->rs16, 3 * (size_t)sp.nnode))) return rc;
         }
-        unsigned g = grid_for(sp.nnode, 4);
-        const long long *no = s->noff;
+        const long long nt = npk * NU16 * 64;
+        const dim3 gr(grid_for(sp.nnode, 4)), gp(grid_for(nt, 256));
         if (svt == IGAP4H_SMOOTH_F16) {
-            if (f32) hipLaunchKernelGGL((k_pack_16<float, 1>), dim3(g), dim3(256), 0, st, sp, (const float *)vals, no, s->sm16, s->rs16);
-            else hipLaunchKernelGGL((k_pack_16<double, 1>), dim3(g), dim3(256), 0, st, sp, (const double *)vals, no, s->sm16, s->rs16);
+            if (f32) {
+                hipLaunchKernelGGL((k_rowscale_16<float>), gr, dim3(256), 0, st, sp, (const float *)vals, s->rs16);
+                hipLaunchKernelGGL((k_pack_16<float, 1>), gp, dim3(256), 0, st, sp, (const float *)vals, (const double *)s->rs16, s->sm16, nt);
+            } else {
+                hipLaunchKernelGGL((k_rowscale_16<double>), gr, dim3(256), 0, st, sp, (const double *)vals, s->rs16);
+                hipLaunchKernelGGL((k_pack_16<double, 1>), gp, dim3(256), 0, st, sp, (const double *)vals, (const double *)s->rs16, s->sm16, nt);
+            }
         } else {
-            if (f32) hipLaunchKernelGGL((k_pack_16<float, 0>), dim3(g), dim3(256), 0, st, sp, (const float *)vals, no, s->sm16, s->rs16);
-            else hipLaunchKernelGGL((k_pack_16<double, 0>), dim3(g), dim3(256), 0, st, sp, (const double *)vals, no, s->sm16, s->rs16);
+            if (f32) hipLaunchKernelGGL((k_pack_16<float, 0>), gp, dim3(256), 0, st, sp, (const float *)vals, (const double *)s->rs16, s->sm16, nt);
+            else hipLaunchKernelGGL((k_pack_16<double, 0>), gp, dim3(256), 0, st, sp, (const double *)vals, (const double *)s->rs16, s->sm16, nt);
         }
         HIPCHK(hipGetLastError());
         return 0;
@@ -1604,8 +1656,10 @@ struct Op {
         if (from_zero) {
             hipLaunchKernelGGL(k_cheb_zero, gn, dim3(256), 0, st, L.nnode, Dinv, b, r, d0, x, 1.0 / theta);
             HIPCHK(hipGetLastError());
+        } else if (l == 0 && svt != IGAP4H_SMOOTH_SAME) {
+            if ((rc = cheb16<1>(x, b, r, nullptr, d0, 0.0, 1.0 / theta, 1))) return rc;
         } else if (l == 0) {  // fine level: the row pass is the SpMV kernel
-            if ((rc = spmv_s(x, r, b))) return rc;
+            if ((rc = spmv(x, r, b))) return rc;
             hipLaunchKernelGGL(k_cheb_dfromr, gn, dim3(256), 0, st, L.nnode, Dinv, (const double *)r, d0, 1.0 / theta);
             HIPCHK(hipGetLastError());
         } else {
@@ -1619,8 +1673,10 @@ struct Op {
                 break;
             }
             const double rn = 1.0 / (2.0 * sigma - rho);
-            if (l == 0) {
-                if ((rc = spmv_s(d0, L.x))) return rc;
+            if (l == 0 && svt != IGAP4H_SMOOTH_SAME) {
+                if ((rc = cheb16<2>(d0, nullptr, r, x, d1, rn * rho, 2.0 * rn / delta, last ? 0 : 1))) return rc;
+            } else if (l == 0) {
+                if ((rc = spmv(d0, L.x))) return rc;
                 hipLaunchKernelGGL(k_cheb_update, gn, dim3(256), 0, st, L.nnode, Dinv, (const double *)L.x, (const double *)d0, d1, r, x,
                                    rn * rho, 2.0 * rn / delta, last ? 0 : 1);
                 HIPCHK(hipGetLastError());
