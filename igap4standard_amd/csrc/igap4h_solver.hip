@@ -357,6 +357,7 @@ __device__ __forceinline__ void sm16_group(const uint4 *__restrict__ w, const do
 }
 
 // Fine-level Chebyshev pass on the 16-bit copy, one lane per node (rows 3n..3n+2):
+//   MODE 0: d = c2 Dinv K~ x                                  (power iteration)
 //   MODE 1: r = b - K~ x, d = c2 Dinv r                       (start from x)
 //   MODE 2: r -= K~ din, x += din, and if newd: dout = c1 din + c2 Dinv r
 // For each phase d0 the block stages the x entries of nodes
@@ -400,6 +401,10 @@ __global__ __launch_bounds__(CH_NT) void k_cheb16(SpArgs s, const uint4 *__restr
         for (int i = 0; i < 3; i++) acc[i] *= rs[3 * t + i];
     double q[3];
     for (int i = 0; i < 3; i++) {
+        if (MODE == 0) {
+            q[i] = acc[i];
+            continue;
+        }
         if (MODE == 1) {
             q[i] = b[3 * t + i] - acc[i];
         } else {
@@ -408,11 +413,11 @@ __global__ __launch_bounds__(CH_NT) void k_cheb16(SpArgs s, const uint4 *__restr
         }
         r[3 * t + i] = q[i];
     }
-    if (MODE == 1 || newd) {
+    if (MODE != 2 || newd) {
         const double *B = Dinv + 9 * t;
         for (int i = 0; i < 3; i++) {
             const double dq = B[3 * i] * q[0] + B[3 * i + 1] * q[1] + B[3 * i + 2] * q[2];
-            dout[3 * t + i] = MODE == 1 ? c2 * dq : c1 * xin[3 * t + i] + c2 * dq;
+            dout[3 * t + i] = MODE == 2 ? c1 * xin[3 * t + i] + c2 * dq : c2 * dq;
         }
     }
 }
@@ -978,8 +983,9 @@ __global__ void k_block_diag_sm(long long nnode, const SMT *__restrict__ sm, dou
 
 // One direction d of the Galerkin product: out = (R_d (x) I) in (R_d (x) I)^T
 // with R_d = P_d^T, the other directions unchanged.  in: stencil storage on
-// grid nbi (IN_CSR: the assembled AIJ values of the fine level); out:
-// stencil storage on nbo (nbo[d] coarse).  One thread per (node, offset).
+// grid nbi (IN_CSR: the assembled AIJ values of the fine level, with dmask
+// bit i of node n set when dof 3n + i is a Dirichlet dof); out: stencil
+// storage on nbo (nbo[d] coarse).  One thread per (node, offset).
 template <int IN_CSR, class VT>
 __global__ __launch_bounds__(256) void k_galerkin(int d, int3 nbi, int3 nbo, const long long *__restrict__ rowptr,
                                                   const VT *__restrict__ in, Xfer x, SMT *__restrict__ out,
@@ -1025,11 +1031,12 @@ __global__ __launch_bounds__(256) void k_galerkin(int d, int3 nbi, int3 nbo, con
                     // Dirichlet rows and columns (identity in K) are left out of the
                     // coarse operators: the coarse problem sees them as removed dofs
                     const long long ncl = ((long long)(Ai[0] + Di[0]) * nbi_[1] + (Ai[1] + Di[1])) * nbi_[2] + (Ai[2] + Di[2]);
+                    const int mi = dmask ? dmask[nin] : 0, mj = dmask ? dmask[ncl] : 0;
 #pragma unroll
                     for (int i = 0; i < 3; i++)
 #pragma unroll
                         for (int j = 0; j < 3; j++)
-                            if (!dmask || !(dmask[3 * nin + i] | dmask[3 * ncl + j])) acc[3 * i + j] += w * (double)in[pos + i * L + j];
+                            if (!(((mi >> i) | (mj >> j)) & 1)) acc[3 * i + j] += w * (double)in[pos + i * L + j];
                 } else {
                     const int oi = ((Di[0] + 2) * 5 + (Di[1] + 2)) * 5 + (Di[2] + 2);
                     const VT *v = in + nin * 1125 + oi * 3;
@@ -1046,6 +1053,12 @@ __global__ __launch_bounds__(256) void k_galerkin(int d, int3 nbi, int3 nbo, con
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) po[i * 375 + j] = (SMT)acc[3 * i + j];
+}
+
+__global__ void k_node_mask(long long nnode, const unsigned char *__restrict__ dmask, unsigned char *__restrict__ nm)
+{
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < nnode) nm[n] = (unsigned char)((dmask[3 * n] ? 1 : 0) | (dmask[3 * n + 1] ? 2 : 0) | (dmask[3 * n + 2] ? 4 : 0));
 }
 
 // bc = P^T rf (restriction), one thread per coarse node
@@ -1170,6 +1183,7 @@ struct igap4h::Solver {
     const void *pc_vals = nullptr;  // matrix the preconditioner was last set up on
     uint4 *sm16 = nullptr;          // 16-bit copy of the fine matrix for smoothing (node-interleaved)
     double *rs16 = nullptr;         // row scales of the binary16 copy
+    unsigned char *nmask = nullptr;
     bool jac_valid = false;  // s->vals holds a tangent (of jac_key) for lagged Newton steps
     long long jac_key = -1;
     long long jac_age = 0;
@@ -1201,6 +1215,7 @@ void igap4h::solver_free(igap4h_ctx *c)
     if (!s) return;
     (void)hipFree(s->vals);
     (void)hipFree(s->sm16);
+    (void)hipFree(s->nmask);
     (void)hipFree(s->rs16);
     for (double *p : {s->Dblk, s->Dinv, s->V, s->r, s->z, s->p, s->q, s->w, s->t, s->Y, s->U0, s->R, s->partial,
                       s->pspmv, s->hd, s->sc})
@@ -1451,8 +1466,8 @@ struct Op {
     int smoother = 0;             // 0 block Gauss-Seidel, 1 Chebyshev (degree nu_pre / nu_post)
     int svt = IGAP4H_SMOOTH_SAME; // matrix of the fine-level Chebyshev passes
 
-    // fine-level Chebyshev pass on the 16-bit copy (k_cheb16): MODE 1 starts
-    // from x (r = b - K~ x, d = c2 Dinv r), MODE 2 is one step on din
+    // fine-level pass on the 16-bit copy (k_cheb16): MODE 0 is d = c2 Dinv K~ x,
+    // MODE 1 starts Chebyshev from x (r = b - K~ x, d = c2 Dinv r), MODE 2 is one step on din
     template <int MODE>
     int cheb16(const double *xin, const double *b, double *r, double *x, double *dout, double c1, double c2, int newd) const
     {
@@ -1566,7 +1581,12 @@ struct Op {
             int3 g3 = make_int3(C.nb[0], C.nb[1], C.nb[2]);
             auto nthr = [](int3 g) { return grid_for((long long)g.x * g.y * g.z * 125, 256); };
             if (l == 0) {
-                const unsigned char *dm = c->d_mask;
+                if (!s->nmask && hipMalloc(&s->nmask, (size_t)sp.nnode) != hipSuccess) {
+                    s->nmask = nullptr;
+                    return fail(IGAP4H_ENOMEM, "hipMalloc of %lld bytes failed", sp.nnode);
+                }
+                hipLaunchKernelGGL(k_node_mask, dim3(grid_for(sp.nnode, 256)), dim3(256), 0, st, sp.nnode, (const unsigned char *)c->d_mask, s->nmask);
+                const unsigned char *dm = s->nmask;
                 if (f32) hipLaunchKernelGGL((k_galerkin<1, float>), dim3(nthr(g1)), dim3(256), 0, st, 0, g0, g1, sp.rowptr, (const float *)vals, F.T.d[0], F.stage[0], dm);
                 else hipLaunchKernelGGL((k_galerkin<1, double>), dim3(nthr(g1)), dim3(256), 0, st, 0, g0, g1, sp.rowptr, (const double *)vals, F.T.d[0], F.stage[0], dm);
             } else {
@@ -1617,7 +1637,8 @@ struct Op {
         HIPCHK(hipGetLastError());
         return finish(s->partial, 1, NRB, out_dev, true);
     }
-    // largest eigenvalue of Dinv K on level l by power iteration
+    // largest eigenvalue of Dinv K on level l by power iteration (on the
+    // fine level with a 16-bit smoothing copy K~: of Dinv K~)
     int estimate_lmax(size_t l) const
     {
         MGLevel &L = s->mg[l];
@@ -1630,7 +1651,11 @@ struct Op {
         if ((rc = norm_n(v, n, s->sc + 12))) return rc;
         hipLaunchKernelGGL(k_scale_inv, dim3(vgrid(n)), dim3(256), 0, st, n, (const double *)(s->sc + 12), v);
         for (int it = 0; it < CHEB_POWER_ITS; it++) {
-            LEVEL_LAUNCH(l, k_dinv_k, Dinv, (const double *)v, w);
+            if (l == 0 && svt != IGAP4H_SMOOTH_SAME) {  // the smoother's operator: Dinv times the 16-bit copy
+                if ((rc = cheb16<0>(v, nullptr, nullptr, nullptr, w, 0.0, 1.0, 1))) return rc;
+            } else {
+                LEVEL_LAUNCH(l, k_dinv_k, Dinv, (const double *)v, w);
+            }
             if ((rc = norm_n(w, n, s->sc + 12))) return rc;
             hipLaunchKernelGGL(k_scale_inv, dim3(vgrid(n)), dim3(256), 0, st, n, (const double *)(s->sc + 12), w);
             HIPCHK(hipGetLastError());
