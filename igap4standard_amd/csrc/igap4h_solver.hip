@@ -265,61 +265,61 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
-// rs[3 n + r] = 2^e > max_k |K_(3n+r),k| (1 for a zero row); one wave per node
-template <class VT>
-__global__ __launch_bounds__(256) void k_rowscale_16(SpArgs s, const VT *__restrict__ vals, double *__restrict__ rs)
-{
-    const int lane = threadIdx.x & 63;
-    const long long node = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (node >= s.nnode) return;
-    int a[3], lo[3], w[3];
-    node_box(s, node, a, lo, w);
-    const int L = 3 * w[0] * w[1] * w[2];
-    const long long base = s.rowptr[3 * node];
-    for (int r = 0; r < 3; r++) {
-        double m = 0.0;
-        for (int k = lane; k < L; k += 64) m = fmax(m, fabs((double)vals[base + (long long)r * L + k]));
-        m = wave_max(m);
-        int e = 0;
-        (void)frexp(m, &e);  // m < 2^e
-        if (lane == 0) rs[3 * node + r] = m > 0.0 ? ldexp(1.0, e) : 1.0;
-    }
-}
-
-// one thread per (node, word): gather the word's 8 values from the AIJ rows
+// One wave per node, PK_NODES = 8 consecutive nodes per block: each word u
+// of the 8 nodes is one 128-byte line of the copy.  The waves read their
+// node's three AIJ rows coalesced and place every value in the block's
+// lines in LDS (zero-filled first); the lines are then written out whole.
+// F16: rs[3 n + r] = 2^e > max_k |K_(3n+r),k| (1 for a zero row) from a
+// wave reduction first.
+constexpr int PK_NODES = 8;
 template <class VT, int F16>
-__global__ __launch_bounds__(256) void k_pack_16(SpArgs s, const VT *__restrict__ vals, const double *__restrict__ rs,
-                                                 uint4 *__restrict__ v4, long long nthreads)
+__global__ __launch_bounds__(64 * PK_NODES) void k_pack_16(SpArgs s, const VT *__restrict__ vals, double *__restrict__ rs,
+                                                           uint4 *__restrict__ v4)
 {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nthreads) return;
-    const int l = (int)(t & 63);
-    const long long q = t >> 6, p = q / NU16;
-    const int u = (int)(q - p * NU16), ph = u / NUP;
-    const long long node = 64 * p + l;
-    unsigned short h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (node < s.nnode) {
+    __shared__ uint4 sl[NU16 * PK_NODES];  // [u][node in block]
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const long long n0 = (long long)blockIdx.x * PK_NODES, node = n0 + wv;
+    for (int k = threadIdx.x; k < NU16 * PK_NODES; k += blockDim.x) sl[k] = make_uint4(0u, 0u, 0u, 0u);
+    __syncthreads();
+    unsigned short *po = (unsigned short *)sl + wv * 8;  // half e of word u of this node: po[u * 64 + e]
+    for (int r = 0; r < 3 && node < s.nnode; r++) {
         int a[3], lo[3], w[3];
         node_box(s, node, a, lo, w);
         const int L = 3 * w[0] * w[1] * w[2];
         const long long base = s.rowptr[3 * node];
-        for (int e = 0; e < 8; e++) {
-            const int v = 8 * (u - ph * NUP) + e;
-            if (v >= 225) break;
-            const int o = v / 9, i = (v % 9) / 3, j = v % 3;
-            const int b0 = a[0] + ph - 2 - lo[0], b1 = a[1] + o / 5 - 2 - lo[1], b2 = a[2] + o % 5 - 2 - lo[2];
-            if (b0 < 0 || b0 >= w[0] || b1 < 0 || b1 >= w[1] || b2 < 0 || b2 >= w[2]) continue;
-            const double x = (double)vals[base + (long long)i * L + 3 * ((b0 * w[1] + b1) * w[2] + b2) + j];
+        const VT *row = vals + base + (long long)r * L;
+        double inv = 1.0;
+        if (F16) {
+            double m = 0.0;
+            for (int k = lane; k < L; k += 64) m = fmax(m, fabs((double)row[k]));
+            m = wave_max(m);
+            int e = 0;
+            (void)frexp(m, &e);  // m < 2^e
+            const double sc = m > 0.0 ? ldexp(1.0, e) : 1.0;
+            inv = 1.0 / sc;  // exact (power of two)
+            if (lane == 0) rs[3 * node + r] = sc;
+        }
+        for (int k = lane; k < L; k += 64) {
+            const int t = k / 3, j = k - 3 * t, t2 = t / w[2], o2 = t - t2 * w[2], o0 = t2 / w[1], o1 = t2 - o0 * w[1];
+            const int ph = lo[0] + o0 - a[0] + 2, ol = (lo[1] + o1 - a[1] + 2) * 5 + (lo[2] + o2 - a[2] + 2);
+            const int v = 9 * ol + 3 * r + j;
+            unsigned short h;
             if (F16) {
-                _Float16 f = (_Float16)(x / rs[3 * node + i]);  // exact division by a power of two
-                h[e] = __builtin_bit_cast(unsigned short, f);
+                _Float16 f = (_Float16)((double)row[k] * inv);
+                h = __builtin_bit_cast(unsigned short, f);
             } else {
-                h[e] = f32_to_bf16((float)x);
+                h = f32_to_bf16((float)row[k]);
             }
+            po[(ph * NUP + (v >> 3)) * 64 + (v & 7)] = h;
         }
     }
-    v4[t] = make_uint4(h[0] | (unsigned)h[1] << 16, h[2] | (unsigned)h[3] << 16, h[4] | (unsigned)h[5] << 16,
-                       h[6] | (unsigned)h[7] << 16);
+    __syncthreads();
+    // node n0 + q's word u -> v4[(p * NU16 + u) * 64 + l], l = (n0 + q) mod 64 (n0 is a multiple of 8)
+    const long long pb = (n0 >> 6) * NU16 * 64 + (n0 & 63);
+    for (int k = threadIdx.x; k < NU16 * PK_NODES; k += blockDim.x) {
+        const int u = k / PK_NODES, q = k - u * PK_NODES;
+        v4[pb + (long long)u * 64 + q] = sl[k];
+    }
 }
 
 template <int F16>
@@ -1067,8 +1067,12 @@ __global__ void k_node_mask(long long nnode, const unsigned char *__restrict__ d
 // MGW^3 = 64 fine nodes of its support, then a wave sum (dmask: the fine
 // Dirichlet dofs contribute nothing)
 static_assert(MGW * MGW * MGW == 64, "one lane per fine node of a coarse support");
+// With Dc: also the coarse level's Chebyshev start from zero (rc = bc, xc = 0,
+// dc = inv_theta Dc bc) that k_cheb_zero would make.
 __global__ __launch_bounds__(256) void k_restrict(Xfer3 x, const double *__restrict__ rf, double *__restrict__ bc,
-                                                  const unsigned char *__restrict__ dmask)
+                                                  const unsigned char *__restrict__ dmask, const double *__restrict__ Dc,
+                                                  double *__restrict__ rc, double *__restrict__ xc, double *__restrict__ dc,
+                                                  double inv_theta)
 {
     const long long nc = (long long)x.nbc[0] * x.nbc[1] * x.nbc[2];
     const long long C = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -1094,15 +1098,24 @@ __global__ __launch_bounds__(256) void k_restrict(Xfer3 x, const double *__restr
     s1 = wave_sum(s1);
     s2 = wave_sum(s2);
     if (lane == 0) {
-        bc[3 * C] = s0;
-        bc[3 * C + 1] = s1;
-        bc[3 * C + 2] = s2;
+        const double q[3] = {s0, s1, s2};
+        for (int i = 0; i < 3; i++) bc[3 * C + i] = q[i];
+        if (Dc) {
+            const double *B = Dc + 9 * C;
+            for (int i = 0; i < 3; i++) {
+                rc[3 * C + i] = q[i];
+                xc[3 * C + i] = 0.0;
+                dc[3 * C + i] = inv_theta * (B[3 * i] * q[0] + B[3 * i + 1] * q[1] + B[3 * i + 2] * q[2]);
+            }
+        }
     }
 }
 
-// xf += P xc (prolongation), one thread per fine node
-// (dmask: no correction on the fine Dirichlet dofs)
-__global__ void k_prolong_add(Xfer3 x, const double *__restrict__ xc, double *__restrict__ xf, const unsigned char *__restrict__ dmask)
+// xf += P (xc + dc) (prolongation; dc, if given, is the coarse smoother's last
+// update, not yet added to xc), one thread per fine node (dmask: no
+// correction on the fine Dirichlet dofs)
+__global__ void k_prolong_add(Xfer3 x, const double *__restrict__ xc, const double *__restrict__ dc, double *__restrict__ xf,
+                              const unsigned char *__restrict__ dmask)
 {
     const long long nf = (long long)x.nbf[0] * x.nbf[1] * x.nbf[2];
     const long long F = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1121,9 +1134,12 @@ __global__ void k_prolong_add(Xfer3 x, const double *__restrict__ xc, double *__
                 double w = w1 * x.d[2].f2c_w[a[2] * MGW + k2];
                 if (w == 0.0) continue;
                 long long C = (((long long)(l0 + k0)) * x.nbc[1] + (l1 + k1)) * x.nbc[2] + (l2 + k2);
-                s0 += w * xc[3 * C];
-                s1 += w * xc[3 * C + 1];
-                s2 += w * xc[3 * C + 2];
+                double v[3] = {xc[3 * C], xc[3 * C + 1], xc[3 * C + 2]};
+                if (dc)
+                    for (int i = 0; i < 3; i++) v[i] += dc[3 * C + i];
+                s0 += w * v[0];
+                s1 += w * v[1];
+                s2 += w * v[2];
             }
         }
     }
@@ -1498,19 +1514,20 @@ struct Op {
             int rc;
             if ((rc = dev_alloc(&s->rs16, 3 * (size_t)sp.nnode))) return rc;
         }
-        const long long nt = npk * NU16 * 64;
-        const dim3 gr(grid_for(sp.nnode, 4)), gp(grid_for(nt, 256));
+        // every block writes its 8 nodes' words whole (nodes past nnode as
+        // zeros); the packs past the last block are zeroed once
+        const long long nblk = grid_for(sp.nnode, PK_NODES), done = nblk * PK_NODES / 64 * NU16 * 64 * 2;
+        const long long nw = npk * NU16 * 64 * 2;  // the copy in 8-byte words
+        if (done < nw)
+            hipLaunchKernelGGL(k_zero, dim3(vgrid(nw - done)), dim3(256), 0, st, (double *)s->sm16 + done, nw - done);
+        const dim3 gp((unsigned)nblk), bp(64 * PK_NODES);
+        uint4 *o16 = s->sm16;
         if (svt == IGAP4H_SMOOTH_F16) {
-            if (f32) {
-                hipLaunchKernelGGL((k_rowscale_16<float>), gr, dim3(256), 0, st, sp, (const float *)vals, s->rs16);
-                hipLaunchKernelGGL((k_pack_16<float, 1>), gp, dim3(256), 0, st, sp, (const float *)vals, (const double *)s->rs16, s->sm16, nt);
-            } else {
-                hipLaunchKernelGGL((k_rowscale_16<double>), gr, dim3(256), 0, st, sp, (const double *)vals, s->rs16);
-                hipLaunchKernelGGL((k_pack_16<double, 1>), gp, dim3(256), 0, st, sp, (const double *)vals, (const double *)s->rs16, s->sm16, nt);
-            }
+            if (f32) hipLaunchKernelGGL((k_pack_16<float, 1>), gp, bp, 0, st, sp, (const float *)vals, s->rs16, o16);
+            else hipLaunchKernelGGL((k_pack_16<double, 1>), gp, bp, 0, st, sp, (const double *)vals, s->rs16, o16);
         } else {
-            if (f32) hipLaunchKernelGGL((k_pack_16<float, 0>), gp, dim3(256), 0, st, sp, (const float *)vals, (const double *)s->rs16, s->sm16, nt);
-            else hipLaunchKernelGGL((k_pack_16<double, 0>), gp, dim3(256), 0, st, sp, (const double *)vals, (const double *)s->rs16, s->sm16, nt);
+            if (f32) hipLaunchKernelGGL((k_pack_16<float, 0>), gp, bp, 0, st, sp, (const float *)vals, s->rs16, o16);
+            else hipLaunchKernelGGL((k_pack_16<double, 0>), gp, bp, 0, st, sp, (const double *)vals, s->rs16, o16);
         }
         HIPCHK(hipGetLastError());
         return 0;
@@ -1667,20 +1684,25 @@ struct Op {
         return 0;
     }
     // Chebyshev smoothing of degree m on level l for K x = b (from x = 0 if
-    // from_zero); with want_res, L.r = b - K x on return
-    int cheb(size_t l, const double *b, double *x, int m, bool from_zero, bool want_res) const
+    // from_zero; started: that start was made by the restriction); with
+    // want_res, L.r = b - K x on return.  With pending, the last update
+    // x += d is left to the caller: *pending = d.
+    int cheb(size_t l, const double *b, double *x, int m, bool from_zero, bool want_res, bool started = false,
+             const double **pending = nullptr) const
     {
         const MGLevel &L = s->mg[l];
         const double *Dinv = l ? L.Dinv : s->Dinv;
         const double bmax = 1.1 * L.lmax, amin = bmax / CHEB_RATIO;
-        const double theta = 0.5 * (bmax + amin), delta = 0.5 * (bmax - amin), sigma = theta / delta;
+        const double theta = cheb_theta(l), delta = 0.5 * (bmax - amin), sigma = theta / delta;
         double rho = 1.0 / sigma;
         double *r = L.r, *d0 = L.d[0], *d1 = L.d[1];
         const dim3 gn(grid_for(L.nnode, 256));
         int rc;
         if (from_zero) {
-            hipLaunchKernelGGL(k_cheb_zero, gn, dim3(256), 0, st, L.nnode, Dinv, b, r, d0, x, 1.0 / theta);
-            HIPCHK(hipGetLastError());
+            if (!started) {
+                hipLaunchKernelGGL(k_cheb_zero, gn, dim3(256), 0, st, L.nnode, Dinv, b, r, d0, x, 1.0 / theta);
+                HIPCHK(hipGetLastError());
+            }
         } else if (l == 0 && svt != IGAP4H_SMOOTH_SAME) {
             if ((rc = cheb16<1>(x, b, r, nullptr, d0, 0.0, 1.0 / theta, 1))) return rc;
         } else if (l == 0) {  // fine level: the row pass is the SpMV kernel
@@ -1692,6 +1714,10 @@ struct Op {
         }
         for (int k = 1; k <= m; k++) {
             const bool last = k == m;
+            if (last && !want_res && pending) {
+                *pending = d0;
+                break;
+            }
             if (last && !want_res) {
                 hipLaunchKernelGGL(k_waxpy, dim3(vgrid(L.ndof)), dim3(256), 0, st, L.ndof, 1.0, (const double *)x, (const double *)d0, x);
                 HIPCHK(hipGetLastError());
@@ -1713,8 +1739,14 @@ struct Op {
         }
         return 0;
     }
-    // x = V-cycle(b) on level l (x overwritten)
-    int vcycle(size_t l, const double *b, double *x) const
+    // Chebyshev's theta = (bmax + amin) / 2 on level l
+    double cheb_theta(size_t l) const
+    {
+        const double bmax = 1.1 * s->mg[l].lmax, amin = bmax / CHEB_RATIO;
+        return 0.5 * (bmax + amin);
+    }
+    // x = V-cycle(b) on level l (x overwritten); started and pending as in cheb
+    int vcycle(size_t l, const double *b, double *x, bool started = false, const double **pending = nullptr) const
     {
         const MGLevel &L = s->mg[l];
         const long long n = L.ndof;
@@ -1735,18 +1767,27 @@ struct Op {
         if (smoother == 1) {
             const double *res = b;
             if (nu_pre > 0) {
-                if ((rc = cheb(l, b, x, nu_pre, true, true))) return rc;
+                if ((rc = cheb(l, b, x, nu_pre, true, true, started))) return rc;
                 res = L.r;
             } else {
                 hipLaunchKernelGGL(k_zero, dim3(vgrid(n)), dim3(256), 0, st, x, n);
                 HIPCHK(hipGetLastError());
             }
-            hipLaunchKernelGGL(k_restrict, dim3(grid_for(C.nnode, 4)), dim3(256), 0, st, L.T, res, C.b, l ? (const unsigned char *)nullptr : (const unsigned char *)c->d_mask);
+            // the restriction also makes the coarse level's Chebyshev start
+            // (unless that level is the coarsest), and the prolongation adds
+            // the coarse smoother's last update
+            const bool cstart = nu_pre > 0 && l + 2 < s->mg.size();
+            const double cinv = cstart ? 1.0 / cheb_theta(l + 1) : 0.0;
+            hipLaunchKernelGGL(k_restrict, dim3(grid_for(C.nnode, 4)), dim3(256), 0, st, L.T, res, C.b,
+                               l ? (const unsigned char *)nullptr : (const unsigned char *)c->d_mask,
+                               cstart ? (const double *)C.Dinv : nullptr, C.r, C.x, C.d[0], cinv);
             HIPCHK(hipGetLastError());
-            if ((rc = vcycle(l + 1, C.b, C.x))) return rc;
-            hipLaunchKernelGGL(k_prolong_add, dim3(grid_for(L.nnode, 256)), dim3(256), 0, st, L.T, (const double *)C.x, x, l ? (const unsigned char *)nullptr : (const unsigned char *)c->d_mask);
+            const double *cpend = nullptr;
+            if ((rc = vcycle(l + 1, C.b, C.x, cstart, &cpend))) return rc;
+            hipLaunchKernelGGL(k_prolong_add, dim3(grid_for(L.nnode, 256)), dim3(256), 0, st, L.T, (const double *)C.x, cpend, x,
+                               l ? (const unsigned char *)nullptr : (const unsigned char *)c->d_mask);
             HIPCHK(hipGetLastError());
-            if (nu_post > 0) return cheb(l, b, x, nu_post, false, false);
+            if (nu_post > 0) return cheb(l, b, x, nu_post, false, false, false, pending);
             return 0;
         }
         hipLaunchKernelGGL(k_zero, dim3(vgrid(n)), dim3(256), 0, st, x, n);
@@ -1763,10 +1804,13 @@ struct Op {
             }
             res = L.r;
         }
-        hipLaunchKernelGGL(k_restrict, dim3(grid_for(C.nnode, 4)), dim3(256), 0, st, L.T, res, C.b, l ? (const unsigned char *)nullptr : (const unsigned char *)c->d_mask);
+        hipLaunchKernelGGL(k_restrict, dim3(grid_for(C.nnode, 4)), dim3(256), 0, st, L.T, res, C.b,
+                           l ? (const unsigned char *)nullptr : (const unsigned char *)c->d_mask, (const double *)nullptr,
+                           (double *)nullptr, (double *)nullptr, (double *)nullptr, 0.0);
         HIPCHK(hipGetLastError());
         if ((rc = vcycle(l + 1, C.b, C.x))) return rc;
-        hipLaunchKernelGGL(k_prolong_add, dim3(grid_for(L.nnode, 256)), dim3(256), 0, st, L.T, (const double *)C.x, x, l ? (const unsigned char *)nullptr : (const unsigned char *)c->d_mask);
+        hipLaunchKernelGGL(k_prolong_add, dim3(grid_for(L.nnode, 256)), dim3(256), 0, st, L.T, (const double *)C.x, (const double *)nullptr, x,
+                           l ? (const unsigned char *)nullptr : (const unsigned char *)c->d_mask);
         HIPCHK(hipGetLastError());
         for (int k = 0; k < nu_post; k++)
             if ((rc = gs_sweep(l, true, b, x))) return rc;
