@@ -313,7 +313,9 @@ typedef struct {
                           the preconditioner is set up (half the bytes of FP32; the
                           Krylov matrix, residual and Newton update are unchanged);
                           IGAP4H_SMOOTH_F16: a binary16 copy of the rows scaled by a
-                          power of two each (entries in (-1, 1), 2^-11 rounding)   */
+                          power of two each (entries in (-1, 1), 2^-11 rounding).
+                          The 16-bit copies need at most 617 nodes in direction 2
+                          (else the solve returns IGAP4H_EINVAL)                   */
 } igap4h_newton_opts;
 
 typedef struct {
