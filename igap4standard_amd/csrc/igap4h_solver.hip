@@ -170,7 +170,9 @@ __device__ __forceinline__ void line_box(const SpArgs &s, int a0, int a1, int z,
 }
 
 // y = K x (RES = 0) or y = b - K x (RES = 1) on the AIJ values; with DOT,
-// partial[block] = sum over the block's rows of x . y (CG's p.Kp)
+// partial[block] = sum over the block's rows of x . y (CG's p.Kp).  The values
+// are read once per pass (non-temporal loads, so they do not evict the x
+// entries other blocks stage from L2).
 template <class VT, int RES, int DOT>
 __global__ __launch_bounds__(LW * 64) void k_spmv_line(SpArgs s, LineGeom g, const VT *__restrict__ vals, const double *__restrict__ x,
                                                    const double *__restrict__ b, double *__restrict__ y,
@@ -198,9 +200,9 @@ __global__ __launch_bounds__(LW * 64) void k_spmv_line(SpArgs s, LineGeom g, con
             xv[it] = 0.0;
             v0[it] = v1[it] = v2[it] = (VT)0;
             if (k < L) {
-                v0[it] = vals[base + k];
-                v1[it] = vals[base + L + k];
-                v2[it] = vals[base + 2 * L + k];
+                v0[it] = __builtin_nontemporal_load(vals + base + k);
+                v1[it] = __builtin_nontemporal_load(vals + base + L + k);
+                v2[it] = __builtin_nontemporal_load(vals + base + 2 * L + k);
                 xv[it] = sx[line_col(a, lo, w, r1, r2, zb, SW3, k)];
             }
         }
