@@ -279,7 +279,7 @@ typedef struct {
     int vals_type;     /* matrix values used by the Krylov solve (F64/F32)     */
     int ksp_type;      /* IGAP4H_KSP_GMRES (reference, snes_init.c:32) or _CG (the
                           alternative left commented at snes_init.c:40-43)      */
-    int restart;       /* GMRES restart length                                  */
+    int restart;       /* GMRES restart length (default 50)                     */
     double ksp_rtol;   /* linear stop: ||b - K x|| < ksp_rtol ||b|| (PETSc 1e-5) */
     int ksp_max_it;    /* linear iterations per Newton step                    */
     int ksp_guess;     /* 1: start from the previous Newton direction
@@ -296,8 +296,9 @@ typedef struct {
                           every Newton increment vanishes there and the zeroed columns
                           of the tangent (snesiga_Tangent.c:270) lose nothing; 0: start
                           from U as given (reference)                                */
-    int mg_pre, mg_post;/* IGAP4H_PC_MG: Gauss-Seidel sweeps before / after the coarse
-                          correction on each level (0..4 each, sum >= 1)           */
+    int mg_pre, mg_post;/* IGAP4H_PC_MG: smoothing sweeps (Gauss-Seidel) or Chebyshev
+                          degree before / after the coarse correction on each level
+                          (0..4 each, sum >= 1; default 3 / 3)                     */
     int mg_graph;      /* IGAP4H_PC_MG: 1 replays the V-cycle from a captured HIP graph */
     int mg_smoother;   /* IGAP4H_PC_MG: IGAP4H_MG_GS (block Gauss-Seidel, mg_pre/mg_post
                           sweeps) or IGAP4H_MG_CHEBYSHEV (block-Jacobi Chebyshev of
