@@ -317,6 +317,10 @@ typedef struct {
                           power of two each (entries in (-1, 1), 2^-11 rounding).
                           The 16-bit copies need at most 617 nodes in direction 2
                           (else the solve returns IGAP4H_EINVAL)                   */
+    int mg_coarse_deg; /* IGAP4H_MG_CHEBYSHEV: Chebyshev degree before / after the coarse
+                          correction on the levels below the finest, at most
+                          mg_pre / mg_post (0..4; 0: mg_pre / mg_post as on the
+                          finest level; default 1)                                 */
 } igap4h_newton_opts;
 
 typedef struct {

@@ -113,26 +113,34 @@ def test_multigrid_mesh_independent(torch_dev, n):
     db = torch.tensor(b, device=dev)
     dx = torch.zeros_like(db)
     its_of = {}
-    for smoother, deg, svals in ((L.IGAP4H_MG_GS, 1, L.IGAP4H_SMOOTH_SAME), (L.IGAP4H_MG_CHEBYSHEV, 2, L.IGAP4H_SMOOTH_SAME),
-                                 (L.IGAP4H_MG_CHEBYSHEV, 2, L.IGAP4H_SMOOTH_BF16),
-                                 (L.IGAP4H_MG_CHEBYSHEV, 2, L.IGAP4H_SMOOTH_F16)):
+    cheb = L.IGAP4H_MG_CHEBYSHEV
+    for smoother, deg, svals, cdeg in ((L.IGAP4H_MG_GS, 1, L.IGAP4H_SMOOTH_SAME, 1), (cheb, 2, L.IGAP4H_SMOOTH_SAME, 1),
+                                       (cheb, 2, L.IGAP4H_SMOOTH_BF16, 1), (cheb, 2, L.IGAP4H_SMOOTH_F16, 1),
+                                       (cheb, 3, L.IGAP4H_SMOOTH_SAME, 0), (cheb, 3, L.IGAP4H_SMOOTH_SAME, 2),
+                                       (cheb, 3, L.IGAP4H_SMOOTH_F16, 1)):
         dx.zero_()
         its, relres = app.linear_solve_dev(dv.data_ptr(), L.IGAP4H_VALS_F64, db.data_ptr(), dx.data_ptr(),
                                            pc=L.IGAP4H_PC_MG, restart=60, ksp_rtol=1e-8, mg_smoother=smoother,
-                                           mg_pre=deg, mg_post=deg, mg_smooth_vals=svals)
+                                           mg_pre=deg, mg_post=deg, mg_smooth_vals=svals, mg_coarse_deg=cdeg)
         torch.cuda.synchronize()
         x = dx.cpu().numpy()
         true_rel = np.linalg.norm(b - K @ x) / np.linalg.norm(b)
-        say("  smoother", smoother, "smoothing matrix", svals, "its", its, "true", true_rel)
+        say("  smoother", smoother, "degree", deg, "coarse", cdeg, "smoothing matrix", svals, "its", its, "true", true_rel)
         assert true_rel < 2e-8
         assert its <= 80, its
-        its_of[(smoother, svals)] = its
+        its_of[(smoother, deg, svals, cdeg)] = its
     # the 16-bit copies (relative rounding 2^-9 bfloat16, 2^-12 binary16)
     # smooth about as well as the matrix itself: a wrong row pass would show
     # as many more iterations
-    cheb = its_of[(L.IGAP4H_MG_CHEBYSHEV, L.IGAP4H_SMOOTH_SAME)]
-    assert its_of[(L.IGAP4H_MG_CHEBYSHEV, L.IGAP4H_SMOOTH_BF16)] <= cheb + 3
-    assert its_of[(L.IGAP4H_MG_CHEBYSHEV, L.IGAP4H_SMOOTH_F16)] <= cheb + 2
+    c2 = its_of[(cheb, 2, L.IGAP4H_SMOOTH_SAME, 1)]
+    assert its_of[(cheb, 2, L.IGAP4H_SMOOTH_BF16, 1)] <= c2 + 3
+    assert its_of[(cheb, 2, L.IGAP4H_SMOOTH_F16, 1)] <= c2 + 2
+    # degree 3 smooths at least as well as 2; a lower degree below the finest
+    # level (the default 1) costs few iterations
+    c3 = its_of[(cheb, 3, L.IGAP4H_SMOOTH_SAME, 0)]
+    assert c3 <= c2
+    assert its_of[(cheb, 3, L.IGAP4H_SMOOTH_SAME, 2)] <= c3 + 2 + c3 // 5
+    assert its_of[(cheb, 3, L.IGAP4H_SMOOTH_F16, 1)] <= c3 + 3 + c3 // 5
 
 
 def test_linear_solve_deterministic(torch_dev):
@@ -251,3 +259,5 @@ def test_newton_options_rejected(torch_dev):
         app.newton_solve_dev(Uh.data_ptr(), pc=7)
     with pytest.raises(L.IgapError):
         app.newton_solve_dev(Uh.data_ptr(), mg_smooth_vals=3)
+    with pytest.raises(L.IgapError):
+        app.newton_solve_dev(Uh.data_ptr(), mg_coarse_deg=5)
