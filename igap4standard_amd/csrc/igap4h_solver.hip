@@ -1489,7 +1489,7 @@ static bool dense_inverse(std::vector<double> &A, int n, std::vector<double> &in
 namespace {
 
 constexpr long long MG_WG_NODES = 512;  // levels up to this size smooth in one work-group
-constexpr double CHEB_RATIO = 20.0;     // Chebyshev interval [lmax' / CHEB_RATIO, lmax'], lmax' = 1.1 lmax
+constexpr double CHEB_RATIO = 100.0;    // Chebyshev interval [lmax' / CHEB_RATIO, lmax'], lmax' = 1.1 lmax
 constexpr int CHEB_POWER_ITS = 12;      // power iterations for lmax
 
 // launch a one-wave-per-node level kernel KERNEL<SM, VT> on level l of Op *this
