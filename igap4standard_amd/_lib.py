@@ -73,7 +73,8 @@ class NewtonOpts(ctypes.Structure):
                 ("stagnation", ctypes.c_int), ("dirichlet_lift", ctypes.c_int),
                 ("mg_pre", ctypes.c_int), ("mg_post", ctypes.c_int), ("mg_graph", ctypes.c_int),
                 ("mg_smoother", ctypes.c_int), ("lag_jacobian", ctypes.c_int), ("lag_persist", ctypes.c_int),
-                ("mg_smooth_vals", ctypes.c_int), ("mg_coarse_deg", ctypes.c_int)]
+                ("mg_smooth_vals", ctypes.c_int), ("mg_coarse_deg", ctypes.c_int),
+                ("mg_cheb_ratio", ctypes.c_double)]
 
 
 class NewtonInfo(ctypes.Structure):

@@ -1231,6 +1231,7 @@ struct igap4h::Solver {
     const void *key_vals = nullptr;
     int key_f32 = -1, key_pre = -1, key_post = -1, key_smoother = -1;
     int key_svt = -1, key_cdeg = -1;
+    double key_cratio = -1.0;
     const void *pc_vals = nullptr;  // matrix the preconditioner was last set up on
     uint4 *sm16 = nullptr;          // 16-bit copy of the fine matrix for smoothing (node-interleaved)
     double *rs16 = nullptr;         // row scales of the binary16 copy
@@ -1489,7 +1490,6 @@ static bool dense_inverse(std::vector<double> &A, int n, std::vector<double> &in
 namespace {
 
 constexpr long long MG_WG_NODES = 512;  // levels up to this size smooth in one work-group
-constexpr double CHEB_RATIO = 100.0;    // Chebyshev interval [lmax' / CHEB_RATIO, lmax'], lmax' = 1.1 lmax
 constexpr int CHEB_POWER_ITS = 12;      // power iterations for lmax
 
 // launch a one-wave-per-node level kernel KERNEL<SM, VT> on level l of Op *this
@@ -1517,6 +1517,7 @@ struct Op {
     int smoother = 0;             // 0 block Gauss-Seidel, 1 Chebyshev (degree nu_pre / nu_post)
     int svt = IGAP4H_SMOOTH_SAME; // matrix of the fine-level Chebyshev passes
     int cdeg = 0;                 // Chebyshev degree cap below the finest level (0: none)
+    double cratio = 200.0;        // Chebyshev interval [lmax' / cratio, lmax'], lmax' = 1.1 lmax
 
     // smoothing degree (Chebyshev) or sweeps (Gauss-Seidel) nu on level l
     int deg(size_t l, int nu) const { return l && smoother == 1 && cdeg > 0 ? std::min(nu, cdeg) : nu; }
@@ -1731,7 +1732,7 @@ struct Op {
     {
         const MGLevel &L = s->mg[l];
         const double *Dinv = l ? L.Dinv : s->Dinv;
-        const double bmax = 1.1 * L.lmax, amin = bmax / CHEB_RATIO;
+        const double bmax = 1.1 * L.lmax, amin = bmax / cratio;
         const double theta = cheb_theta(l), delta = 0.5 * (bmax - amin), sigma = theta / delta;
         double rho = 1.0 / sigma;
         double *r = L.r, *d0 = L.d[0], *d1 = L.d[1];
@@ -1781,7 +1782,7 @@ struct Op {
     // Chebyshev's theta = (bmax + amin) / 2 on level l
     double cheb_theta(size_t l) const
     {
-        const double bmax = 1.1 * s->mg[l].lmax, amin = bmax / CHEB_RATIO;
+        const double bmax = 1.1 * s->mg[l].lmax, amin = bmax / cratio;
         return 0.5 * (bmax + amin);
     }
     // x = V-cycle(b) on level l (x overwritten); started and pending as in cheb
@@ -1862,7 +1863,7 @@ struct Op {
         if (z != s->z) return vcycle(0, r, z);
         const long long n = 3 * sp.nnode;
         if (!s->mg_exec || s->key_vals != vals || s->key_f32 != f32 || s->key_pre != nu_pre || s->key_post != nu_post ||
-            s->key_smoother != smoother || s->key_svt != svt || s->key_cdeg != cdeg) {
+            s->key_smoother != smoother || s->key_svt != svt || s->key_cdeg != cdeg || s->key_cratio != cratio) {
             if (s->mg_exec) {
                 HIPCHK(hipGraphExecDestroy(s->mg_exec));
                 s->mg_exec = nullptr;
@@ -1894,6 +1895,7 @@ struct Op {
             s->key_smoother = smoother;
             s->key_svt = svt;
             s->key_cdeg = cdeg;
+            s->key_cratio = cratio;
         }
         HIPCHK(hipMemcpyAsync(s->mg_in, r, n * sizeof(double), hipMemcpyDeviceToDevice, st));
         HIPCHK(hipGraphLaunch(s->mg_exec, st));
@@ -2157,12 +2159,14 @@ int check_opts(const igap4h_newton_opts *o)
     if (o->ksp_type == IGAP4H_KSP_GMRES && (o->restart < 1 || o->restart > 4096))
         return fail(IGAP4H_EINVAL, "GMRES restart %d outside 1..4096", o->restart);
     if (o->ksp_max_it < 1) return fail(IGAP4H_EINVAL, "ksp_max_it must be >= 1");
-    if (o->pc == IGAP4H_PC_MG && (o->mg_pre < 0 || o->mg_pre > 4 || o->mg_post < 0 || o->mg_post > 4 || o->mg_pre + o->mg_post < 1))
-        return fail(IGAP4H_EINVAL, "multigrid sweeps pre %d / post %d outside 0..4 (sum >= 1)", o->mg_pre, o->mg_post);
+    if (o->pc == IGAP4H_PC_MG && (o->mg_pre < 0 || o->mg_pre > 8 || o->mg_post < 0 || o->mg_post > 8 || o->mg_pre + o->mg_post < 1))
+        return fail(IGAP4H_EINVAL, "multigrid sweeps pre %d / post %d outside 0..8 (sum >= 1)", o->mg_pre, o->mg_post);
     if (o->mg_smoother != IGAP4H_MG_GS && o->mg_smoother != IGAP4H_MG_CHEBYSHEV) return fail(IGAP4H_EINVAL, "bad mg_smoother %d", o->mg_smoother);
     if (o->mg_smooth_vals != IGAP4H_SMOOTH_SAME && o->mg_smooth_vals != IGAP4H_SMOOTH_BF16 && o->mg_smooth_vals != IGAP4H_SMOOTH_F16)
         return fail(IGAP4H_EINVAL, "bad mg_smooth_vals %d", o->mg_smooth_vals);
-    if (o->mg_coarse_deg < 0 || o->mg_coarse_deg > 4) return fail(IGAP4H_EINVAL, "mg_coarse_deg %d outside 0..4", o->mg_coarse_deg);
+    if (o->mg_coarse_deg < 0 || o->mg_coarse_deg > 8) return fail(IGAP4H_EINVAL, "mg_coarse_deg %d outside 0..8", o->mg_coarse_deg);
+    if (o->pc == IGAP4H_PC_MG && o->mg_smoother == IGAP4H_MG_CHEBYSHEV && !(o->mg_cheb_ratio > 1.0 && o->mg_cheb_ratio <= 1e6))
+        return fail(IGAP4H_EINVAL, "mg_cheb_ratio %g outside (1, 1e6]", o->mg_cheb_ratio);
     if (o->vals_type != IGAP4H_VALS_F64 && o->vals_type != IGAP4H_VALS_F32) return fail(IGAP4H_EINVAL, "bad vals_type %d", o->vals_type);
     if (o->tangent_prec != IGAP4H_FP64 && o->tangent_prec != IGAP4H_FP32) return fail(IGAP4H_EINVAL, "bad tangent_prec %d", o->tangent_prec);
     return 0;
@@ -2198,14 +2202,15 @@ extern "C" int igap4h_newton_default_opts(igap4h_newton_opts *o)
     o->pc = IGAP4H_PC_MG;
     o->line_search = 1;               // snes_init.c:21-22
     o->stagnation = 1;                // snes_solve.c:30
-    o->mg_pre = 3;                    // Chebyshev degree 3 before and after the coarse correction
-    o->mg_post = 3;                   // (64^3: 26 % fewer GMRES iterations than degree 2)
+    o->mg_pre = 5;                    // Chebyshev degree 5 before and after the coarse correction
+    o->mg_post = 5;                   // on the finest level (64^3: fastest of degrees 2..6)
     o->mg_graph = 1;
     o->mg_smoother = IGAP4H_MG_CHEBYSHEV;
     o->lag_jacobian = 1;
     o->lag_persist = 0;
     o->mg_smooth_vals = IGAP4H_SMOOTH_SAME;
     o->mg_coarse_deg = 1;             // 64^3: 2 % more GMRES iterations than degree 3, 3-8 % less time
+    o->mg_cheb_ratio = 200.0;         // with degree 5: 64^3 fastest of 20..400
     return 0;
 }
 
@@ -2233,6 +2238,7 @@ extern "C" int igap4h_linear_solve_dev(igap4h_ctx *c, const void *d_vals, int va
     A.smoother = o->mg_smoother;
     A.svt = o->mg_smooth_vals;
     A.cdeg = o->mg_coarse_deg;
+    A.cratio = o->mg_cheb_ratio;
     if ((rc = A.setup_pc())) return rc;
     return linear_solve(A, d_b, d_x, o, its, relres);
 }
@@ -2257,6 +2263,7 @@ extern "C" int igap4h_newton_solve_dev(igap4h_ctx *c, double *d_Uh, const igap4h
     A.smoother = o->mg_smoother;
     A.svt = o->mg_smooth_vals;
     A.cdeg = o->mg_coarse_deg;
+    A.cratio = o->mg_cheb_ratio;
     double *U = d_Uh;  // slice 0: current iterate
 
     auto t_all = clk::now();

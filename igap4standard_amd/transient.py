@@ -32,6 +32,10 @@ class TimeStepper:
         self.n = n
         self.Uh = torch.zeros(3 * n, device=self.dev, dtype=torch.float64)
         kw = dict(atol=1e-10, rtol=0.0)   # the reference's absolute tolerance (src/snes_init.c:18)
+        # the mass term rho/dt^2 makes these systems well conditioned (~20 GMRES
+        # iterations per Newton step): a lighter smoother than the static default
+        # (48^3: 147 ms per step against 163 with degree 5 over [lmax/200, lmax])
+        kw.update(mg_pre=2, mg_post=2, mg_cheb_ratio=20.0)
         if mixed:
             kw.update(tangent_prec=L.IGAP4H_FP32, vals_type=L.IGAP4H_VALS_F32)
         kw.update(newton_kw)

@@ -298,7 +298,7 @@ typedef struct {
                           from U as given (reference)                                */
     int mg_pre, mg_post;/* IGAP4H_PC_MG: smoothing sweeps (Gauss-Seidel) or Chebyshev
                           degree before / after the coarse correction on each level
-                          (0..4 each, sum >= 1; default 3 / 3)                     */
+                          (0..8 each, sum >= 1; default 5 / 5)                     */
     int mg_graph;      /* IGAP4H_PC_MG: 1 replays the V-cycle from a captured HIP graph */
     int mg_smoother;   /* IGAP4H_PC_MG: IGAP4H_MG_GS (block Gauss-Seidel, mg_pre/mg_post
                           sweeps) or IGAP4H_MG_CHEBYSHEV (block-Jacobi Chebyshev of
@@ -319,8 +319,11 @@ typedef struct {
                           (else the solve returns IGAP4H_EINVAL)                   */
     int mg_coarse_deg; /* IGAP4H_MG_CHEBYSHEV: Chebyshev degree before / after the coarse
                           correction on the levels below the finest, at most
-                          mg_pre / mg_post (0..4; 0: mg_pre / mg_post as on the
+                          mg_pre / mg_post (0..8; 0: mg_pre / mg_post as on the
                           finest level; default 1)                                 */
+    double mg_cheb_ratio;/* IGAP4H_MG_CHEBYSHEV: the smoothing interval is
+                          [1.1 lmax / mg_cheb_ratio, 1.1 lmax] (lmax of Dinv K by power
+                          iteration); > 1, default 200                              */
 } igap4h_newton_opts;
 
 typedef struct {

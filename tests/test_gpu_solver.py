@@ -260,4 +260,6 @@ def test_newton_options_rejected(torch_dev):
     with pytest.raises(L.IgapError):
         app.newton_solve_dev(Uh.data_ptr(), mg_smooth_vals=3)
     with pytest.raises(L.IgapError):
-        app.newton_solve_dev(Uh.data_ptr(), mg_coarse_deg=5)
+        app.newton_solve_dev(Uh.data_ptr(), mg_coarse_deg=9)
+    with pytest.raises(L.IgapError):
+        app.newton_solve_dev(Uh.data_ptr(), mg_cheb_ratio=1.0)
