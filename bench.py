@@ -285,7 +285,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "phys/ge Newton solve to ||R||/||R0||<1e-10 (residual, tangent, GMRES(50) + "
-                               "multigrid, line search), %d^3 quadratic C1 B-spline elements, nq=4, clamp/stretch "
+                               "multigrid V-cycle with block-Jacobi Chebyshev smoothing (degree 5 on the finest "
+                               "level, 1 below, interval [lmax/200, lmax]), line search), %d^3 quadratic C1 "
+                               "B-spline elements, nq=4, clamp/stretch "
                                "a=0.01, Lame +-10%% per element; value = elements x Newton steps / solve time; "
                                "ms_per_step = per Newton step; steps = timed solves" % n,
                    "nelem": pb.n_elem, "ndof": app.ndof, "nnz": app.nnz, "nquad": 4, "parallelism": "single GPU"},
