@@ -226,6 +226,26 @@ def test_newton_16bit_smoothing_same_solution(torch_dev, oracle, svals):
     assert i16["ksp_its"] <= 1.1 * i32["ksp_its"] + 5
 
 
+def test_16bit_smoothing_mesh_limit(torch_dev):
+    """The 16-bit smoothing pass stages x slabs of 3 (260 + 4 nb2) doubles in
+    LDS (64 KB at most): a mesh with more than 617 nodes in direction 2 is
+    refused when the copy is set up, before any kernel uses it, and the
+    context stays usable."""
+    torch, dev = torch_dev
+    pb = P.stretch_problem((1, 1, 616), a=0.01)
+    app = App(pb)
+    U = P.affine_guess(pb)
+    Uh = torch.tensor(U, device=dev)
+    kw = dict(rtol=1e-8, atol=0.0, dirichlet_lift=1, max_newton=1, ksp_max_it=50,
+              tangent_prec=L.IGAP4H_FP32, vals_type=L.IGAP4H_VALS_F32)
+    for svals in (L.IGAP4H_SMOOTH_F16, L.IGAP4H_SMOOTH_BF16):
+        with pytest.raises(L.IgapError):
+            app.newton_solve_dev(Uh.data_ptr(), mg_smooth_vals=svals, **kw)
+        torch.cuda.synchronize()
+    R = app.residual(U)
+    assert np.all(np.isfinite(R))
+
+
 def test_newton_getime_step(torch_dev, oracle):
     """One implicit getime step on 3^3, all faces clamped: R(U) = 0 with the
     history slices held fixed.  The history is zero on the clamped dofs (as
