@@ -942,7 +942,8 @@ __global__ __launch_bounds__(256) void k_cheb_step(SpArgs s, const VT *__restric
     }
 }
 
-// fine level: the row pass is k_spmv_line / k_spmv_16; these apply the per-node updates
+// fine level with the Krylov matrix: the row pass is k_spmv_line, these apply
+// the per-node updates (the 16-bit copy's k_cheb16 fuses them)
 // d = Dinv r / theta
 __global__ void k_cheb_dfromr(long long nnode, const double *__restrict__ Dinv, const double *__restrict__ r,
                               double *__restrict__ d, double inv_theta)
