@@ -974,6 +974,16 @@ __global__ void k_cheb_update(long long nnode, const double *__restrict__ Dinv, 
     }
 }
 
+// out = Dinv in per node (thread per node)
+__global__ void k_apply_dinv(long long nnode, const double *__restrict__ Dinv, const double *__restrict__ in, double *__restrict__ out)
+{
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nnode) return;
+    const double *B = Dinv + 9 * n;
+    const double q0 = in[3 * n], q1 = in[3 * n + 1], q2 = in[3 * n + 2];
+    for (int i = 0; i < 3; i++) out[3 * n + i] = B[3 * i] * q0 + B[3 * i + 1] * q1 + B[3 * i + 2] * q2;
+}
+
 // y = Dinv K x (power iteration for the largest eigenvalue of Dinv K)
 template <int SM, class VT>
 __global__ __launch_bounds__(256) void k_dinv_k(SpArgs s, const VT *__restrict__ vals, const double *__restrict__ Dinv,
@@ -1221,6 +1231,8 @@ struct MGLevel {
     bool dense = false;
     double *d[2] = {nullptr, nullptr};    // Chebyshev directions
     double lmax = 0.0;                    // largest eigenvalue estimate of Dinv K
+    double *ev = nullptr;                 // its eigenvector estimate, the next set-up's start
+    bool ev_ok = false;                   // ev holds one from this solve
 };
 }  // namespace
 
@@ -1279,7 +1291,7 @@ void igap4h::solver_free(igap4h_ctx *c)
     if (s->cap) (void)hipStreamDestroy(s->cap);
     (void)hipFree(s->mg_in);
     for (MGLevel &L : s->mg) {
-        for (double *p : {L.Dblk, L.Dinv, L.x, L.b, L.r, L.xfer_w, L.Ainv, L.d[0], L.d[1]}) (void)hipFree(p);
+        for (double *p : {L.Dblk, L.Dinv, L.x, L.b, L.r, L.xfer_w, L.Ainv, L.d[0], L.d[1], L.ev}) (void)hipFree(p);
         for (SMT *p : {L.sm, L.stage[0], L.stage[1]}) (void)hipFree(p);
         (void)hipFree(L.xfer_i);
     }
@@ -1366,7 +1378,8 @@ static int mg_build(igap4h_ctx *c, Solver *s)
     s->mg.push_back(L0);
     int rc;
     if ((rc = dev_alloc(&s->mg[0].r, (size_t)L0.ndof)) || (rc = dev_alloc(&s->mg[0].d[0], (size_t)L0.ndof)) ||
-        (rc = dev_alloc(&s->mg[0].d[1], (size_t)L0.ndof)) || (rc = dev_alloc(&s->mg[0].x, (size_t)L0.ndof)))
+        (rc = dev_alloc(&s->mg[0].d[1], (size_t)L0.ndof)) || (rc = dev_alloc(&s->mg[0].x, (size_t)L0.ndof)) ||
+        (rc = dev_alloc(&s->mg[0].ev, (size_t)L0.ndof)))
         return rc;  // mg[0].x: fine-level product buffer (the level's solution is the caller's z)
     while (s->mg.size() < 12) {
         MGLevel &F = s->mg.back();
@@ -1431,7 +1444,8 @@ static int mg_build(igap4h_ctx *c, Solver *s)
         if ((rc = dev_alloc(&C.sm, (size_t)C.nnode * 1125)) || (rc = dev_alloc(&C.Dblk, 9 * (size_t)C.nnode)) ||
             (rc = dev_alloc(&C.Dinv, 9 * (size_t)C.nnode)) || (rc = dev_alloc(&C.x, (size_t)C.ndof)) ||
             (rc = dev_alloc(&C.b, (size_t)C.ndof)) || (rc = dev_alloc(&C.r, (size_t)C.ndof)) ||
-            (rc = dev_alloc(&C.d[0], (size_t)C.ndof)) || (rc = dev_alloc(&C.d[1], (size_t)C.ndof)))
+            (rc = dev_alloc(&C.d[0], (size_t)C.ndof)) || (rc = dev_alloc(&C.d[1], (size_t)C.ndof)) ||
+            (rc = dev_alloc(&C.ev, (size_t)C.ndof)))
             return rc;
         for (int d = 0; d < 3; d++) kf[d] = kc[d];
         s->mg.push_back(C);
@@ -1491,7 +1505,8 @@ static bool dense_inverse(std::vector<double> &A, int n, std::vector<double> &in
 namespace {
 
 constexpr long long MG_WG_NODES = 512;  // levels up to this size smooth in one work-group
-constexpr int CHEB_POWER_ITS = 12;      // power iterations for lmax
+constexpr int CHEB_POWER_ITS = 12;      // power iterations for lmax from a pseudo-random start
+constexpr int CHEB_POWER_WARM = 4;      // from the previous Newton step's eigenvector
 
 // launch a one-wave-per-node level kernel KERNEL<SM, VT> on level l of Op *this
 #define LEVEL_LAUNCH(l, KERNEL, ...)                                                                                   \
@@ -1704,13 +1719,22 @@ struct Op {
         double *v = L.d[0], *w = L.d[1];
         const long long n = L.ndof;
         int rc;
-        hipLaunchKernelGGL(k_hash_vec, dim3(vgrid(n)), dim3(256), 0, st, n, v);
-        HIPCHK(hipGetLastError());
+        if (L.ev_ok) {  // a later Newton step of the same solve: start from the last estimate
+            HIPCHK(hipMemcpyAsync(v, L.ev, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        } else {
+            hipLaunchKernelGGL(k_hash_vec, dim3(vgrid(n)), dim3(256), 0, st, n, v);
+            HIPCHK(hipGetLastError());
+        }
         if ((rc = norm_n(v, n, s->sc + 12))) return rc;
         hipLaunchKernelGGL(k_scale_inv, dim3(vgrid(n)), dim3(256), 0, st, n, (const double *)(s->sc + 12), v);
-        for (int it = 0; it < CHEB_POWER_ITS; it++) {
+        const int its = L.ev_ok ? CHEB_POWER_WARM : CHEB_POWER_ITS;
+        for (int it = 0; it < its; it++) {
             if (l == 0 && svt != IGAP4H_SMOOTH_SAME) {  // the smoother's operator: Dinv times the 16-bit copy
                 if ((rc = cheb16<0>(v, nullptr, nullptr, nullptr, w, 0.0, 1.0, 1))) return rc;
+            } else if (l == 0) {  // the line pass, then Dinv per node
+                if ((rc = spmv(v, L.x))) return rc;
+                hipLaunchKernelGGL(k_apply_dinv, dim3(grid_for(L.nnode, 256)), dim3(256), 0, st, L.nnode, Dinv, (const double *)L.x, w);
+                HIPCHK(hipGetLastError());
             } else {
                 LEVEL_LAUNCH(l, k_dinv_k, Dinv, (const double *)v, w);
             }
@@ -1719,6 +1743,8 @@ struct Op {
             HIPCHK(hipGetLastError());
             std::swap(v, w);
         }
+        HIPCHK(hipMemcpyAsync(L.ev, v, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        L.ev_ok = true;
         if ((rc = read_scalars(12, 1))) return rc;
         L.lmax = s->hp[12];
         if (!(L.lmax > 0.0) || !std::isfinite(L.lmax)) return fail(IGAP4H_EINVAL, "multigrid: eigenvalue estimate %g on level %zu", L.lmax, l);
@@ -2233,6 +2259,7 @@ extern "C" int igap4h_linear_solve_dev(igap4h_ctx *c, const void *d_vals, int va
     HIPCHK(hipSetDevice(c->device));
     if ((rc = solver_ws(c, o->ksp_type == IGAP4H_KSP_GMRES ? o->restart : 1, 0))) return rc;
     Op A{c, c->solver, sp_args(c), d_vals, vals_type == IGAP4H_VALS_F32, o->pc, (hipStream_t)stream};
+    for (MGLevel &L : c->solver->mg) L.ev_ok = false;  // warm starts only within one solve
     A.nu_pre = o->mg_pre;
     A.nu_post = o->mg_post;
     A.mg_graph = o->mg_graph;
@@ -2258,6 +2285,7 @@ extern "C" int igap4h_newton_solve_dev(igap4h_ctx *c, double *d_Uh, const igap4h
     hipStream_t st = (hipStream_t)stream;
     const long long n = s->ndof;
     Op A{c, s, sp_args(c), s->vals, f32, o->pc, st};
+    for (MGLevel &L : s->mg) L.ev_ok = false;  // warm starts only within one solve (repeatable solves)
     A.nu_pre = o->mg_pre;
     A.nu_post = o->mg_post;
     A.mg_graph = o->mg_graph;
