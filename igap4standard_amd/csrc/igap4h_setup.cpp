@@ -272,7 +272,8 @@ std::vector<double> coarsen_knots(const std::vector<double> &kf, int nef)
     std::vector<double> kc;
     kc.push_back(kf[0]);
     kc.push_back(kf[1]);
-    for (int e = 0; e <= nef; e += 2) kc.push_back(kf[2 + e]);  // element boundaries 0, 2, ..., nef
+    for (int e = 0; e < nef; e += 2) kc.push_back(kf[2 + e]);  // element boundaries 0, 2, ...
+    kc.push_back(kf[2 + nef]);                                  // and the last one
     kc.push_back(kf[nef + 3]);
     kc.push_back(kf[nef + 4]);
     return kc;

@@ -47,8 +47,9 @@ std::vector<double> neumann_load(const Mesh &m, int nquad, igap4h_neumann_fn hoo
 // (A36, A37, A38, e0, e345, le^2) from par_mat (phys/ge/residual.c:45-47).
 void coef_from_par(const double *par, double coef[6]);
 
-// Dyadic coarsening of an open quadratic knot vector: kf has nef + 5 knots;
-// kc keeps every other element boundary (nef even), nef / 2 + 5 knots.
+// Coarsening of an open quadratic knot vector: kf has nef + 5 knots; kc keeps
+// every other element boundary and the last one (nef odd: the last coarse
+// element is one fine element), (nef + 1) / 2 + 5 knots; kc is nested in kf.
 std::vector<double> coarsen_knots(const std::vector<double> &kf, int nef);
 
 // Knot-insertion prolongation between nested quadratic spline spaces:

@@ -1357,10 +1357,11 @@ static SpArgs sp_args(const igap4h_ctx *c)
 }
 
 // ====================================================================== multigrid set-up
-// Levels by dyadic coarsening of the knot vectors while every direction has
-// an even element count and the level has more than MG_DENSE dofs; the
-// coarsest level is solved with a dense inverse when it has at most MG_DENSE
-// dofs, else by symmetric Gauss-Seidel sweeps.
+// Levels by coarsening the knot vectors (every other element boundary kept,
+// and the last one: n elements become (n + 1) / 2; a direction with one
+// element stays) while the level has more than MG_DENSE dofs; the coarsest
+// level is solved with a dense inverse when it has at most MG_DENSE dofs,
+// else by symmetric Gauss-Seidel sweeps.
 constexpr long long MG_DENSE = 400;
 
 static int mg_build(igap4h_ctx *c, Solver *s)
@@ -1383,15 +1384,15 @@ static int mg_build(igap4h_ctx *c, Solver *s)
         return rc;  // mg[0].x: fine-level product buffer (the level's solution is the caller's z)
     while (s->mg.size() < 12) {
         MGLevel &F = s->mg.back();
-        bool can = F.ndof > MG_DENSE;
-        for (int d = 0; d < 3; d++) can = can && F.ne[d] % 2 == 0 && F.ne[d] >= 2;
-        if (!can) break;
+        bool any = false;
+        for (int d = 0; d < 3; d++) any = any || F.ne[d] >= 2;
+        if (F.ndof <= MG_DENSE || !any) break;
         MGLevel C;
         std::vector<double> kc[3], P[3];
         for (int d = 0; d < 3; d++) {
-            C.ne[d] = F.ne[d] / 2;
+            C.ne[d] = F.ne[d] >= 2 ? (F.ne[d] + 1) / 2 : F.ne[d];
             C.nb[d] = C.ne[d] + 2;
-            kc[d] = coarsen_knots(kf[d], F.ne[d]);
+            kc[d] = F.ne[d] >= 2 ? coarsen_knots(kf[d], F.ne[d]) : kf[d];
             if (prolongation_1d(kf[d], F.ne[d], kc[d], C.ne[d], P[d]))
                 return fail(IGAP4H_EINVAL, "multigrid: knot vectors of direction %d are not nested", d);
         }

@@ -97,11 +97,12 @@ def test_linear_solve(torch_dev, ksp, pc, restart):
     assert its2 <= 1
 
 
-@pytest.mark.parametrize("n", [8, 16, (16, 8, 12)])
+@pytest.mark.parametrize("n", [8, 16, (16, 8, 12), (9, 11, 13), 15])
 def test_multigrid_mesh_independent(torch_dev, n):
     """GMRES + one V-cycle (Gauss-Seidel or Chebyshev smoothing): iteration
-    count bounded independently of h
-    (block Jacobi needs thousands at 16^3)."""
+    count bounded independently of h (block Jacobi needs thousands at 16^3),
+    odd element counts included (their coarse levels end in a one-element
+    span)."""
     torch, dev = torch_dev
     say("multigrid", n)
     pb = P.stretch_problem(n, a=0.01, noise=0.1)
