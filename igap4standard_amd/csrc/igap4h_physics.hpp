@@ -184,14 +184,18 @@ struct Strain {
 template <class T>
 IGAP4H_HD void strain_of(const T *u, const Coef &c, Strain<T> &st)
 {
+    #pragma unroll
     for (int a = 0; a < 3; a++) {
+        #pragma unroll
         for (int J = 0; J < 3; J++) {
             st.F[a][J] = u[10 * a + 1 + J];
             if (a == J) st.F[a][J] = st.F[a][J] + Num<T>::cst(1.0);
         }
+        #pragma unroll
         for (int h = 0; h < 6; h++) st.H[a][h] = u[10 * a + 4 + h];
     }
     const int PI_[6] = {0, 1, 2, 0, 0, 1}, PJ_[6] = {0, 1, 2, 1, 2, 2};
+    #pragma unroll
     for (int p = 0; p < 6; p++) {
         T t = st.F[0][PI_[p]] * st.F[0][PJ_[p]] + st.F[1][PI_[p]] * st.F[1][PJ_[p]] + st.F[2][PI_[p]] * st.F[2][PJ_[p]];
         if (p < 3) t = t - Num<T>::cst(1.0);
@@ -201,11 +205,14 @@ IGAP4H_HD void strain_of(const T *u, const Coef &c, Strain<T> &st)
     st.h2 = (st.eps[0] - st.eps[1]) * IGAP4H_S2;
     st.h3 = (st.eps[0] + st.eps[1] - st.eps[2] * 2.0) * IGAP4H_S6;
     st.q = st.h2 * st.h2 + st.h3 * st.h3;
+    #pragma unroll
     for (int K = 0; K < 3; K++) {
         T g[3];
+        #pragma unroll
         for (int I = 0; I < 3; I++)
             g[I] = st.F[0][I] * st.H[0][hidx(I, K)] + st.F[1][I] * st.H[1][hidx(I, K)] + st.F[2][I] * st.H[2][hidx(I, K)];
         T m = (g[0] + g[1] + g[2]) * (1.0 / 3.0);
+        #pragma unroll
         for (int I = 0; I < 3; I++) st.tau[I][K] = (g[I] - m) * (2.0 * c.lsq);
     }
 }
@@ -237,10 +244,13 @@ IGAP4H_HD void residual_grad(const T *u, const Coef &c, T r[30])
     stress_of(st, c, S);
     // full symmetric S
     T Sm[3][3] = {{S[0], S[3], S[4]}, {S[3], S[1], S[5]}, {S[4], S[5], S[2]}};
+    #pragma unroll
     for (int a = 0; a < 3; a++) {
         r[10 * a] = Num<T>::cst(0.0);
+        #pragma unroll
         for (int J = 0; J < 3; J++) {
             T t = st.F[a][0] * Sm[0][J] + st.F[a][1] * Sm[1][J] + st.F[a][2] * Sm[2][J];
+            #pragma unroll
             for (int K = 0; K < 3; K++) t = t + st.tau[J][K] * st.H[a][hidx(J, K)];
             r[10 * a + 1 + J] = t;
         }
