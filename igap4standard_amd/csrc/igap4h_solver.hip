@@ -1941,6 +1941,9 @@ struct Op {
             int rc = mg_build(c, s);
             if (rc) return rc;
             if (smoother == 1 && svt != IGAP4H_SMOOTH_SAME && (rc = pack_16())) return rc;
+            // the V-cycle graph holds this set-up's Chebyshev coefficients (from
+            // lmax) as kernel arguments: capture it again
+            s->key_vals = nullptr;
             return mg_setup();
         }
         return 0;

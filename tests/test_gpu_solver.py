@@ -161,6 +161,35 @@ def test_linear_solve_deterministic(torch_dev):
     assert np.array_equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("svals", [L.IGAP4H_SMOOTH_SAME, L.IGAP4H_SMOOTH_F16])
+def test_preconditioner_follows_new_values(torch_dev, svals):
+    """New matrix values in the same buffer (as between Newton steps): the
+    multigrid set-up, including the Chebyshev coefficients of the replayed
+    V-cycle graph, follows them — the second solve equals a solve of the new
+    values in a fresh context bit for bit."""
+    torch, dev = torch_dev
+    pb = P.stretch_problem(8, a=0.01, noise=0.1)
+    kw = dict(ksp_rtol=1e-8, mg_smooth_vals=svals)
+    b = np.random.default_rng(8).standard_normal(pb.ndof)
+    db = torch.tensor(b, device=dev)
+    app = App(pb)
+    v1 = app.tangent(P.smooth_state(pb, amp=1e-3, noise=0.0))
+    v2 = app.tangent(P.smooth_state(pb, amp=3e-2, noise=1e-3, seed=7))
+    dv = torch.tensor(v1, device=dev)
+    dx = torch.zeros_like(db)
+    app.linear_solve_dev(dv.data_ptr(), L.IGAP4H_VALS_F64, db.data_ptr(), dx.data_ptr(), **kw)
+    dv.copy_(torch.tensor(v2, device=dev))
+    dx.zero_()
+    its2, _ = app.linear_solve_dev(dv.data_ptr(), L.IGAP4H_VALS_F64, db.data_ptr(), dx.data_ptr(), **kw)
+    torch.cuda.synchronize()
+    x2 = dx.cpu().numpy()
+    app2 = App(pb)
+    dx.zero_()
+    its2b, _ = app2.linear_solve_dev(dv.data_ptr(), L.IGAP4H_VALS_F64, db.data_ptr(), dx.data_ptr(), **kw)
+    torch.cuda.synchronize()
+    assert its2 == its2b and np.array_equal(x2, dx.cpu().numpy())
+
+
 def newton_gpu(app, pb, U0, torch, dev, **kw):
     Uh = torch.tensor(U0, device=dev)
     info = app.newton_solve_dev(Uh.data_ptr(), **kw)
