@@ -5,7 +5,7 @@ u_x = 0.01 X at x = 1, per-element Lame fields with seeded +-10 % noise
 
 A step is one full Newton solve on the GPU from the affine guess u = 0.01 X
 (Dirichlet rows set to their values) to ||R|| / ||R0|| < 1e-10: per Newton
-iteration the FP64 residual, the tangent on the AIJ pattern, GMRES(50) with a
+iteration the FP64 residual, the tangent on the AIJ pattern, GMRES(300) with a
 multigrid V-cycle and the backtracking line search, all with inputs resident
 in HBM.  Two forms: plain FP64 (the primary line) and mixed (FP32-MFMA
 tangent with FP64 accumulation over Gauss-point chunks, FP32 matrix values
@@ -284,7 +284,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "phys/ge Newton solve to ||R||/||R0||<1e-10 (residual, tangent, GMRES(50) + "
+        "config": {"workload": "phys/ge Newton solve to ||R||/||R0||<1e-10 (residual, tangent, GMRES(300) + "
                                "multigrid V-cycle with block-Jacobi Chebyshev smoothing (degree 5 on the finest "
                                "level, 1 below, interval [lmax/200, lmax]), line search), %d^3 quadratic C1 "
                                "B-spline elements, nq=4, clamp/stretch "

@@ -2225,8 +2225,10 @@ extern "C" int igap4h_newton_default_opts(igap4h_newton_opts *o)
     o->tangent_prec = IGAP4H_FP64;
     o->vals_type = IGAP4H_VALS_F64;
     o->ksp_type = IGAP4H_KSP_GMRES;  // snes_init.c:32
-    o->restart = 50;                  // the reference's 10000 keeps every Krylov vector; 50 is the
-                                      // fastest measured (64^3: 30 and 80 slower)
+    o->restart = 300;                 // the reference's 10000 keeps every Krylov vector; at 64^3 no
+                                      // linear solve reaches 50 iterations, but the first Newton
+                                      // step's (indefinite) system on finer meshes needs hundreds:
+                                      // 96^3 1,852 iterations with restart 50, 525 with 300
     o->ksp_rtol = 1e-5;               // PETSc default (snes_init.c:26 keeps it)
     o->ksp_max_it = 20000;
     o->ksp_guess = 1;                 // snes_init.c:25

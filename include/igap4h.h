@@ -279,7 +279,8 @@ typedef struct {
     int vals_type;     /* matrix values used by the Krylov solve (F64/F32)     */
     int ksp_type;      /* IGAP4H_KSP_GMRES (reference, snes_init.c:32) or _CG (the
                           alternative left commented at snes_init.c:40-43)      */
-    int restart;       /* GMRES restart length (default 50)                     */
+    int restart;       /* GMRES restart length (default 300; (restart + 1) Krylov
+                          vectors are kept: 2.1 GB at 64^3)                      */
     double ksp_rtol;   /* linear stop: ||b - K x|| < ksp_rtol ||b|| (PETSc 1e-5) */
     int ksp_max_it;    /* linear iterations per Newton step                    */
     int ksp_guess;     /* 1: start from the previous Newton direction
