@@ -1516,7 +1516,10 @@ extern "C" int igap4h_update_bc(igap4h_ctx *c, const BC *bc)
     for (int i = 0; i < 9; i++) m.par_dirichlet[i] = bc->par_dirichlet[i];
     m.par_neumann = bc->par_neumann;
     HIPCHK(hipSetDevice(c->device));
+    // kernels still queued on any stream may read the arrays replaced below
+    HIPCHK(hipDeviceSynchronize());
     solver_invalidate_jacobian(c);
+    c->bc_gen++;
     const int64_t ndof = m.ndof();
     std::vector<int32_t> rows = dirichlet_rows(m);
     std::vector<unsigned char> mask;
@@ -1550,7 +1553,13 @@ extern "C" int igap4h_set_coef(igap4h_ctx *c, const double *elem_coef)
 {
     if (!c) return fail(IGAP4H_EINVAL, "NULL context");
     HIPCHK(hipSetDevice(c->device));
-    return upload_coef(c, elem_coef);
+    // queued kernels may still read the coefficients being replaced
+    HIPCHK(hipDeviceSynchronize());
+    int rc = upload_coef(c, elem_coef);
+    if (rc) return rc;
+    // a lagged tangent (and its preconditioner) belongs to the old coefficients
+    solver_invalidate_jacobian(c);
+    return 0;
 }
 
 extern "C" int igap4h_residual_dev(igap4h_ctx *c, const double *d_Uh, double *d_R, void *stream)
@@ -1836,7 +1845,43 @@ struct igap4h_batch {
     int qc = 0;                             // chosen variant
     hipStream_t cap = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::vector<igap4h_ctx *> ctx;          // the instances' contexts
+    std::vector<int> bc_gen;                // their bc_gen when h_inst was filled
 };
+
+// the fields of an instance that igap4h_update_bc may change (it can
+// reallocate the Dirichlet row list and the Neumann load)
+static void batch_bc_fields(BatchInst &I, const igap4h_ctx *c)
+{
+    I.mask = c->d_mask;
+    I.g = c->d_g;
+    I.fneu = c->d_fneu;
+    I.drows = c->d_drows;
+    I.ndrows = (int)c->drows.size();
+}
+
+static void batch_drop_graph(igap4h_batch *b);
+
+// Re-read the boundary data of instances whose context changed them since
+// the descriptors were uploaded; the captured graph is dropped because the
+// Dirichlet launch is sized by the largest row count.
+static int batch_refresh(igap4h_batch *b)
+{
+    bool changed = false;
+    for (int i = 0; i < b->n; i++)
+        if (b->ctx[i]->bc_gen != b->bc_gen[i]) {
+            batch_bc_fields(b->h_inst[i], b->ctx[i]);
+            b->bc_gen[i] = b->ctx[i]->bc_gen;
+            changed = true;
+        }
+    if (!changed) return 0;
+    b->maxnd = 0;
+    for (int i = 0; i < b->n; i++) b->maxnd = std::max(b->maxnd, b->h_inst[i].ndrows);
+    batch_drop_graph(b);
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(b->d_inst, b->h_inst.data(), sizeof(BatchInst) * b->n, hipMemcpyHostToDevice));
+    return 0;
+}
 
 template <class ACC, int QC>
 static int batch_tangent_launches(igap4h_batch *b, hipStream_t st)
@@ -1945,16 +1990,14 @@ extern "C" int igap4h_batch_create(igap4h_ctx *const *ctxs, int n, const double 
         I.Uh = d_U_hist[i];
         I.coef = c->d_coef;
         I.rowptr = c->d_rowptr;
-        I.mask = c->d_mask;
-        I.g = c->d_g;
-        I.fneu = c->d_fneu;
-        I.drows = c->d_drows;
-        I.ndrows = (int)c->drows.size();
+        batch_bc_fields(I, c);
         I.Re = c->d_Re;
         I.R = d_R[i];
         I.vals = d_vals[i];
         I.nnz = c->nnz;
         b->h_inst.push_back(I);
+        b->ctx.push_back(c);
+        b->bc_gen.push_back(c->bc_gen);
         b->eoff[i + 1] = b->eoff[i] + c->mesh.nelem();
         for (int col = 0; col < 27; col++) {
             int cx = col / 9, cy = (col / 3) % 3, cz = col % 3;
@@ -2010,6 +2053,8 @@ extern "C" int igap4h_batch_tune(igap4h_batch *b, int reps, int *best_qc, double
     if (!b) return fail(IGAP4H_EINVAL, "NULL batch");
     if (reps < 1) reps = 3;
     HIPCHK(hipSetDevice(b->device));
+    int rc0 = batch_refresh(b);
+    if (rc0) return rc0;
     const int *cand = kBatchQC[b->prec == IGAP4H_FP32];
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
@@ -2051,6 +2096,8 @@ extern "C" int igap4h_batch_assemble(igap4h_batch *b, void *stream)
 {
     if (!b) return fail(IGAP4H_EINVAL, "NULL batch");
     HIPCHK(hipSetDevice(b->device));
+    int rc0 = batch_refresh(b);
+    if (rc0) return rc0;
     if (!b->exec) {
         HIPCHK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeRelaxed));
         int rc = batch_sequence(b, b->cap);

@@ -49,6 +49,9 @@ struct igap4h_ctx {
     double *U_hist_app = nullptr;
     igap4h_neumann_fn neumann = nullptr;  // traction hook given at create
     igap4h::Solver *solver = nullptr;  // Krylov work space, created on first use
+    // incremented whenever the Dirichlet/Neumann device data change (their
+    // arrays may be reallocated), so that users of copies (batches) refresh
+    int bc_gen = 0;
 };
 
 namespace igap4h {

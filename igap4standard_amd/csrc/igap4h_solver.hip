@@ -1246,6 +1246,10 @@ struct igap4h::Solver {
     int key_svt = -1, key_cdeg = -1;
     double key_cratio = -1.0;
     const void *pc_vals = nullptr;  // matrix the preconditioner was last set up on
+    // options the preconditioner was last set up with (pc, smoother, smoothing
+    // values, coarse degree, pre, post; Chebyshev ratio)
+    int pc_key[6] = {-1, -1, -1, -1, -1, -1};
+    double pc_key_ratio = -1.0;
     uint4 *sm16 = nullptr;          // 16-bit copy of the fine matrix for smoothing (node-interleaved)
     double *rs16 = nullptr;         // row scales of the binary16 copy
     unsigned char *nmask = nullptr;
@@ -1258,7 +1262,8 @@ struct igap4h::Solver {
     size_t vals_bytes = 0;
     void *vals = nullptr;  // tangent values used by the Newton solve
     double *Dblk = nullptr, *Dinv = nullptr;
-    double *V = nullptr;   // (restart + 1) x ndof
+    double *V = nullptr;   // Krylov basis: vcap x ndof, grown on demand up to restart + 1
+    int vcap = 0;
     double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *w = nullptr, *t = nullptr;
     double *Y = nullptr, *U0 = nullptr, *R = nullptr;
     double *partial = nullptr;  // (restart + 2) x NRB
@@ -1327,8 +1332,13 @@ static int solver_ws(igap4h_ctx *c, int restart, size_t vals_bytes)
         if (s->hh) (void)hipHostFree(s->hh);
         s->V = s->partial = s->hd = s->hh = nullptr;
         s->restart = 0;
+        s->vcap = 0;
         int rc;
-        if ((rc = dev_alloc(&s->V, (size_t)(restart + 1) * s->ndof))) return rc;
+        // the basis starts small (most solves need few vectors) and grows in
+        // basis_reserve
+        const int v0 = std::min(restart + 1, 32);
+        if ((rc = dev_alloc(&s->V, (size_t)v0 * s->ndof))) return rc;
+        s->vcap = v0;
         if ((rc = dev_alloc(&s->partial, (size_t)(restart + 2) * NRB))) return rc;
         if ((rc = dev_alloc(&s->hd, 3 * (size_t)(restart + 1)))) return rc;
         if (hipHostMalloc((void **)&s->hh, 3 * (size_t)(restart + 1) * sizeof(double)) != hipSuccess) {
@@ -1344,6 +1354,23 @@ static int solver_ws(igap4h_ctx *c, int restart, size_t vals_bytes)
         if (hipMalloc(&s->vals, vals_bytes) != hipSuccess) return fail(IGAP4H_ENOMEM, "hipMalloc of %zu bytes failed", vals_bytes);
         s->vals_bytes = vals_bytes;
     }
+    return 0;
+}
+
+// make room for nvec Krylov vectors (at most restart + 1): doubles the
+// allocation and moves the vectors already built, in stream order
+static int basis_reserve(Solver *s, int nvec, hipStream_t st)
+{
+    if (nvec <= s->vcap) return 0;
+    const int cap = std::min(s->restart + 1, std::max(nvec, 2 * s->vcap));
+    double *V = nullptr;
+    int rc = dev_alloc(&V, (size_t)cap * s->ndof);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(V, s->V, (size_t)s->vcap * s->ndof * sizeof(double), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    (void)hipFree(s->V);
+    s->V = V;
+    s->vcap = cap;
     return 0;
 }
 
@@ -1929,9 +1956,19 @@ struct Op {
         HIPCHK(hipGraphLaunch(s->mg_exec, st));
         return 0;
     }
+    bool pc_key_matches() const
+    {
+        const int k[6] = {pc, smoother, svt, cdeg, nu_pre, nu_post};
+        for (int i = 0; i < 6; i++)
+            if (s->pc_key[i] != k[i]) return false;
+        return s->pc_key_ratio == cratio;
+    }
     int setup_pc() const
     {
         s->pc_vals = vals;
+        const int k[6] = {pc, smoother, svt, cdeg, nu_pre, nu_post};
+        for (int i = 0; i < 6; i++) s->pc_key[i] = k[i];
+        s->pc_key_ratio = cratio;
         if (pc == IGAP4H_PC_NONE) return 0;
         unsigned g = grid_for(sp.nnode, 256);
         if (f32) hipLaunchKernelGGL((k_block_diag<float>), dim3(g), dim3(256), 0, st, sp, (const float *)vals, s->Dblk, s->Dinv);
@@ -2050,6 +2087,7 @@ int gmres(const Op &A, const double *b, double *x, int m, double rtol, int maxit
         g[0] = beta;
         int j = 0;
         for (; j < m;) {
+            if ((rc = basis_reserve(s, j + 2, st))) return rc;
             double *Vj = s->V + (long long)j * n, *W = s->V + (long long)(j + 1) * n;
             if ((rc = A.apply_pc(Vj, s->z))) return rc;
             if ((rc = A.spmv(s->z, W))) return rc;
@@ -2225,10 +2263,11 @@ extern "C" int igap4h_newton_default_opts(igap4h_newton_opts *o)
     o->tangent_prec = IGAP4H_FP64;
     o->vals_type = IGAP4H_VALS_F64;
     o->ksp_type = IGAP4H_KSP_GMRES;  // snes_init.c:32
-    o->restart = 300;                 // the reference's 10000 keeps every Krylov vector; at 64^3 no
-                                      // linear solve reaches 50 iterations, but the first Newton
+    o->restart = 300;                 // the reference's 10000 keeps every Krylov vector; 64^3
+                                      // linear solves take 20-50 iterations, but the first Newton
                                       // step's (indefinite) system on finer meshes needs hundreds:
-                                      // 96^3 1,852 iterations with restart 50, 525 with 300
+                                      // 96^3 1,852 iterations with restart 50, 525 with 300.
+                                      // The basis grows on demand up to restart + 1 vectors
     o->ksp_rtol = 1e-5;               // PETSc default (snes_init.c:26 keeps it)
     o->ksp_max_it = 20000;
     o->ksp_guess = 1;                 // snes_init.c:25
@@ -2355,7 +2394,9 @@ extern "C" int igap4h_newton_solve_dev(igap4h_ctx *c, double *d_Uh, const igap4h
             hipLaunchKernelGGL(k_zero, dim3(vgrid(n)), dim3(256), 0, st, s->Y, n);
             HIPCHK(hipGetLastError());
         }
-        if ((rebuild || s->pc_vals != s->vals) && (rc = A.setup_pc())) return rc;
+        // a reused tangent keeps its preconditioner only if that was built with
+        // this solve's preconditioner options
+        if ((rebuild || s->pc_vals != s->vals || !A.pc_key_matches()) && (rc = A.setup_pc())) return rc;
         int kits = 0;
         double relres = 0.0;
         rc = linear_solve(A, s->R, s->Y, o, &kits, &relres);

@@ -166,8 +166,10 @@ int igap4h_sizes(const igap4h_ctx *ctx, int64_t *ndof, int64_t *nnz, int64_t *nd
 
 /* The reference re-reads app->bc on every callback; a context copies it at
  * create.  Call this after changing bc_type, par_dirichlet or par_neumann
- * (load stepping): a no-op when nothing changed, else the Dirichlet rows,
- * values and mask and the Neumann load are rebuilt.                        */
+ * (load stepping): a no-op when nothing changed, else the device is
+ * synchronized (queued assembly kernels on any stream finish first) and the
+ * Dirichlet rows, values and mask and the Neumann load are rebuilt.  A
+ * lagged tangent of the Newton driver is discarded.                        */
 int igap4h_update_bc(igap4h_ctx *ctx, const BC *bc);
 
 /* ---- index sets (bit-exact with the reference) ------------------------ */
@@ -199,7 +201,8 @@ int igap4h_tangent_dev(igap4h_ctx *ctx, const double *d_U_hist, int prec, int va
                        void *stream);
 
 /* ---- material coefficients ------------------------------------------- */
-/* Replace the per-element coefficients (host array n_elem x 6).          */
+/* Replace the per-element coefficients (host array n_elem x 6).  The device
+ * is synchronized first; a lagged tangent of the Newton driver is discarded. */
 int igap4h_set_coef(igap4h_ctx *ctx, const double *elem_coef);
 
 /* ---- energy ----------------------------------------------------------- */
@@ -242,7 +245,9 @@ int igap4h_residual_vjp_coef_dev(igap4h_ctx *ctx, const double *d_U, const doubl
  * instance: one residual launch over all elements, one gather, one tangent
  * launch per colour over all instances, and the Dirichlet diagonals,
  * captured once into a HIP graph and replayed.  The contexts and arrays must
- * outlive the batch.                                                        */
+ * outlive the batch.  A context's igap4h_update_bc (which may reallocate its
+ * Dirichlet row list and Neumann load) is picked up by the next
+ * igap4h_batch_assemble / igap4h_batch_tune.                               */
 typedef struct igap4h_batch igap4h_batch;
 int igap4h_batch_create(igap4h_ctx *const *ctxs, int n, const double *const *d_U_hist, double *const *d_R,
                         void *const *d_vals, int prec, int vals_type, igap4h_batch **batch);
@@ -279,8 +284,11 @@ typedef struct {
     int vals_type;     /* matrix values used by the Krylov solve (F64/F32)     */
     int ksp_type;      /* IGAP4H_KSP_GMRES (reference, snes_init.c:32) or _CG (the
                           alternative left commented at snes_init.c:40-43)      */
-    int restart;       /* GMRES restart length (default 300; (restart + 1) Krylov
-                          vectors are kept: 2.1 GB at 64^3)                      */
+    int restart;       /* GMRES restart length (default 300).  The Krylov basis is
+                          allocated on demand, 32 vectors first, doubled when a
+                          cycle needs more, up to restart + 1 vectors (ndof x 8 B
+                          each: 6.9 MB at 64^3, so 2.1 GB for a full 300-cycle;
+                          the memory is kept by the context)                     */
     double ksp_rtol;   /* linear stop: ||b - K x|| < ksp_rtol ||b|| (PETSc 1e-5) */
     int ksp_max_it;    /* linear iterations per Newton step                    */
     int ksp_guess;     /* 1: start from the previous Newton direction

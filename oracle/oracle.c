@@ -25,6 +25,10 @@
  *                       quadratic); Dirichlet rows U - g in closed form
  *   or_tangent          FD of or_residual (K dU ~ R(U+dU) - R(U)); symmetry;
  *                       dense brute force sum_q B^T D B on one element
+ *   or_tangent_window   the rows of or_tangent on windows of a small mesh
+ *                       (same element matrices, other scatter)
+ *   or_residual_mt      or_residual on the same input (order of the sums
+ *                       only)
  */
 #include "oracle.h"
 #include <math.h>
@@ -413,7 +417,7 @@ void or_coef_from_par(const double par[9], double coef[6])
 
 static ge_out *ge_scratch(void)
 {
-    static ge_out *o = NULL;
+    static _Thread_local ge_out *o = NULL;  /* one per thread (or_residual_mt) */
     if (!o) o = (ge_out *)malloc(sizeof(ge_out));
     return o;
 }
@@ -927,41 +931,50 @@ static inline int64_t aij_find(const int32_t *col, int64_t lo0, int64_t hi0, int
     return -1;
 }
 
+/* element matrix val[81 x 81] of element (ex, ey, ez), rows/columns 3 A + i
+ * with A = 9 ax + 3 ay + az (snesiga_Tangent.c:104-175) */
+static void elem_tangent(const or_problem *pb, const asm_ctx *c, const double *U_hist, int64_t e, int ex, int ey,
+                         int ez, double *val)
+{
+    int nq = pb->nquad;
+    double N[270], xq[3], w, u[90], D[900];
+    const double *coef = elem_coef(pb, c, e);
+    for (int i = 0; i < 81 * 81; i++) val[i] = 0.0;
+    for (int qx = 0; qx < nq; qx++)
+        for (int qy = 0; qy < nq; qy++)
+            for (int qz = 0; qz < nq; qz++) {
+                elem_qp(pb, c, ex, ey, ez, qx, qy, qz, N, xq, &w);
+                interp_u(pb, c, U_hist, ex, ey, ez, N, xq, u);
+                pointwise_D(pb, c, coef, u, D);
+                /* snesiga_Tangent.c:149-175: 81 x 81 x 10 x 10 per point */
+                for (int A = 0; A < NBPE; A++)
+                    for (int i = 0; i < NDOF; i++) {
+                        int row = 3 * A + i;
+                        for (int B = 0; B < NBPE; B++)
+                            for (int j = 0; j < NDOF; j++) {
+                                int cl = 3 * B + j;
+                                double t = 0.0;
+                                for (int s = 0; s < NDDIM; s++)
+                                    for (int tt = 0; tt < NDDIM; tt++)
+                                        t += N[10 * A + s] * D[30 * (10 * i + s) + (10 * j + tt)] * N[10 * B + tt];
+                                val[81 * row + cl] += w * t;
+                            }
+                    }
+            }
+}
+
 int or_tangent_elements(const or_problem *pb, const double *U_hist, const int64_t *rowptr,
                         const int32_t *col, double *vals, int64_t e_begin, int64_t e_end)
 {
     asm_ctx c;
     if (asm_init(pb, &c)) return -1;
-    int nq = pb->nquad, nby = c.nb[1], nbz = c.nb[2];
-    double N[270], xq[3], w, u[90], D[900];
+    int nby = c.nb[1], nbz = c.nb[2];
     double *val = (double *)malloc(sizeof(double) * 81 * 81);
     int32_t idx[81];
     for (int64_t e = e_begin; e < e_end; e++) {
         int ex, ey, ez;
         elem_ijk(pb, e, &ex, &ey, &ez);
-        const double *coef = elem_coef(pb, &c, e);
-        for (int i = 0; i < 81 * 81; i++) val[i] = 0.0;
-        for (int qx = 0; qx < nq; qx++)
-            for (int qy = 0; qy < nq; qy++)
-                for (int qz = 0; qz < nq; qz++) {
-                    elem_qp(pb, &c, ex, ey, ez, qx, qy, qz, N, xq, &w);
-                    interp_u(pb, &c, U_hist, ex, ey, ez, N, xq, u);
-                    pointwise_D(pb, &c, coef, u, D);
-                    /* snesiga_Tangent.c:149-175: 81 x 81 x 10 x 10 per point */
-                    for (int A = 0; A < NBPE; A++)
-                        for (int i = 0; i < NDOF; i++) {
-                            int row = 3 * A + i;
-                            for (int B = 0; B < NBPE; B++)
-                                for (int j = 0; j < NDOF; j++) {
-                                    int cl = 3 * B + j;
-                                    double t = 0.0;
-                                    for (int s = 0; s < NDDIM; s++)
-                                        for (int tt = 0; tt < NDDIM; tt++)
-                                            t += N[10 * A + s] * D[30 * (10 * i + s) + (10 * j + tt)] * N[10 * B + tt];
-                                    val[81 * row + cl] += w * t;
-                                }
-                        }
-                }
+        elem_tangent(pb, &c, U_hist, e, ex, ey, ez, val);
         for (int ax = 0; ax < 3; ax++)
             for (int ay = 0; ay < 3; ay++)
                 for (int az = 0; az < 3; az++) {
@@ -1001,6 +1014,107 @@ int or_tangent(const or_problem *pb, const double *U_hist, const int64_t *rowptr
             if (z[r] || z[col[k]]) vals[k] = (z[r] && col[k] == r) ? 1.0 : 0.0;
     free(z);
     free(rows);
+    return 0;
+}
+
+/* Element sums of the tangent over the elements [lo, hi) per direction,
+ * without the Dirichlet step: out[r][k] for the row dofs r of the nodes a
+ * with a_d in [rlo_d, rhi_d] (rlo = 0 if lo = 0 else lo + 2, rhi = nb - 1 if
+ * hi = nelem else hi - 1: the nodes whose whole support lies in the window,
+ * i.e. complete rows), and the column dofs k of the nodes b with b_d in
+ * [lo_d, hi_d + 1]; both node-major (x slowest), dof fastest. */
+int or_tangent_window(const or_problem *pb, const double *U_hist, const int lo[3], const int hi[3], double *out)
+{
+    asm_ctx c;
+    if (asm_init(pb, &c)) return -1;
+    int rlo[3], nr[3], nc[3];
+    for (int d = 0; d < 3; d++) {
+        if (lo[d] < 0 || hi[d] > pb->nelem[d] || hi[d] <= lo[d]) return -1;
+        rlo[d] = lo[d] == 0 ? 0 : lo[d] + 2;
+        int rhi = hi[d] == pb->nelem[d] ? pb->nelem[d] + 1 : hi[d] - 1;
+        nr[d] = rhi - rlo[d] + 1;
+        nc[d] = hi[d] - lo[d] + 2;
+        if (nr[d] < 1) return -1;
+    }
+    const int64_t ncol = 3LL * nc[0] * nc[1] * nc[2];
+    memset(out, 0, sizeof(double) * (size_t)(3LL * nr[0] * nr[1] * nr[2] * ncol));
+    double *val = (double *)malloc(sizeof(double) * 81 * 81);
+    for (int ex = lo[0]; ex < hi[0]; ex++)
+        for (int ey = lo[1]; ey < hi[1]; ey++)
+            for (int ez = lo[2]; ez < hi[2]; ez++) {
+                int64_t e = ((int64_t)ex * pb->nelem[1] + ey) * pb->nelem[2] + ez;
+                elem_tangent(pb, &c, U_hist, e, ex, ey, ez, val);
+                for (int A = 0; A < NBPE; A++) {
+                    int a0 = ex + A / 9 - rlo[0], a1 = ey + (A / 3) % 3 - rlo[1], a2 = ez + A % 3 - rlo[2];
+                    if (a0 < 0 || a0 >= nr[0] || a1 < 0 || a1 >= nr[1] || a2 < 0 || a2 >= nr[2]) continue;
+                    int64_t rbase = 3LL * (((int64_t)a0 * nr[1] + a1) * nr[2] + a2);
+                    for (int B = 0; B < NBPE; B++) {
+                        int b0 = ex + B / 9 - lo[0], b1 = ey + (B / 3) % 3 - lo[1], b2 = ez + B % 3 - lo[2];
+                        int64_t cbase = 3LL * (((int64_t)b0 * nc[1] + b1) * nc[2] + b2);
+                        for (int i = 0; i < NDOF; i++)
+                            for (int j = 0; j < NDOF; j++)
+                                out[(rbase + i) * ncol + cbase + j] += val[81 * (3 * A + i) + 3 * B + j];
+                    }
+                }
+            }
+    free(val);
+    return 0;
+}
+
+/* ---- threaded residual (large-mesh checks) ----------------------------- */
+#include <pthread.h>
+
+typedef struct {
+    const or_problem *pb;
+    const double *U_hist;
+    double *R;
+    int64_t e0, e1;
+    int rc;
+} res_job;
+
+static void *res_worker(void *arg)
+{
+    res_job *j = (res_job *)arg;
+    j->rc = or_residual_elements(j->pb, j->U_hist, j->R, j->e0, j->e1);
+    return NULL;
+}
+
+int or_residual_mt(const or_problem *pb, const double *U_hist, double *R, int nthreads)
+{
+    asm_ctx c;
+    if (asm_init(pb, &c)) return -1;
+    int64_t ne = (int64_t)pb->nelem[0] * pb->nelem[1] * pb->nelem[2];
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    if (nthreads > ne) nthreads = (int)ne;
+    res_job *jobs = (res_job *)calloc((size_t)nthreads, sizeof(res_job));
+    pthread_t *th = (pthread_t *)calloc((size_t)nthreads, sizeof(pthread_t));
+    int rc = 0;
+    for (int t = 0; t < nthreads; t++) {
+        jobs[t].pb = pb;
+        jobs[t].U_hist = U_hist;
+        jobs[t].R = t == 0 ? R : (double *)calloc((size_t)c.ndof, sizeof(double));
+        jobs[t].e0 = ne * t / nthreads;
+        jobs[t].e1 = ne * (t + 1) / nthreads;
+        if (!jobs[t].R) rc = -2;
+    }
+    for (int64_t i = 0; i < c.ndof; i++) R[i] = 0.0;
+    for (int t = 0; t < nthreads && !rc; t++)
+        if (pthread_create(&th[t], NULL, res_worker, &jobs[t])) rc = -3;
+    for (int t = 0; t < nthreads; t++)
+        if (th[t]) pthread_join(th[t], NULL);
+    for (int t = 0; t < nthreads; t++) {
+        if (jobs[t].rc) rc = jobs[t].rc;
+        if (t > 0 && jobs[t].R) {
+            for (int64_t i = 0; i < c.ndof; i++) R[i] += jobs[t].R[i];
+            free(jobs[t].R);
+        }
+    }
+    free(jobs);
+    free(th);
+    if (rc) return rc;
+    neumann_add(pb, &c, R);
+    dirichlet_residual(pb, U_hist, R);
     return 0;
 }
 

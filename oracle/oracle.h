@@ -97,6 +97,15 @@ int or_tangent_elements(const or_problem *pb, const double *U_hist,
 int or_residual_elements(const or_problem *pb, const double *U_hist, double *R,
                          int64_t e_begin, int64_t e_end);
 
+/* Element sums of the tangent over the element window [lo, hi) per
+ * direction, no Dirichlet step, as a dense block: rows = the dofs of the
+ * nodes whose support lies in the window (complete rows), columns = the dofs
+ * of the window's nodes [lo, hi + 1]; node-major, dof fastest.            */
+int or_tangent_window(const or_problem *pb, const double *U_hist, const int lo[3], const int hi[3],
+                      double *out);
+/* or_residual with the element loop split over nthreads threads.          */
+int or_residual_mt(const or_problem *pb, const double *U_hist, double *R, int nthreads);
+
 /* Energy functional (src/iga_intdensity.c:10-293, volume part, ge only). */
 double or_energy(const or_problem *pb, const double *U_hist);
 

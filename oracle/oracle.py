@@ -69,6 +69,8 @@ def lib():
                                            ctypes.c_int64]
         L.or_evalN.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                ctypes.c_double]
+        L.or_residual_mt.argtypes = [ctypes.POINTER(OrProblem), _dp, _dp, ctypes.c_int]
+        L.or_tangent_window.argtypes = [ctypes.POINTER(OrProblem), _dp, _ip, _ip, _dp]
         _lib = L
     return _lib
 
@@ -152,6 +154,52 @@ def residual(pb, U_hist):
     if rc:
         raise RuntimeError("or_residual failed: %d" % rc)
     return R
+
+
+def residual_mt(pb, U_hist, nthreads=None):
+    """residual() with the element loop on nthreads threads (default: the
+    CPUs this process may use, at most 16)."""
+    if nthreads is None:
+        nthreads = min(16, len(os.sched_getaffinity(0)))
+    b = Bound(pb)
+    U = np.ascontiguousarray(U_hist, dtype=np.float64)
+    R = np.zeros(lib().or_ndof(b.ptr))
+    rc = lib().or_residual_mt(b.ptr, _d(U), _d(R), int(nthreads))
+    if rc:
+        raise RuntimeError("or_residual_mt failed: %d" % rc)
+    return R
+
+
+def window_nodes(pb, lo, hi):
+    """(row node ranges, column node ranges) of tangent_window: per
+    direction the nodes whose support lies in the element window [lo, hi)
+    and the window's nodes [lo, hi + 1]."""
+    rows, cols = [], []
+    for d in range(3):
+        rlo = 0 if lo[d] == 0 else lo[d] + 2
+        rhi = pb.nelem[d] + 1 if hi[d] == pb.nelem[d] else hi[d] - 1
+        rows.append(np.arange(rlo, rhi + 1))
+        cols.append(np.arange(lo[d], hi[d] + 2))
+    return rows, cols
+
+
+def tangent_window(pb, U_hist, lo, hi):
+    """Element sums of the tangent over the element window [lo, hi) per
+    direction (no Dirichlet step) on the complete rows of that window, as a
+    dense (3 * rows, 3 * cols) block (node-major, dof fastest; see
+    window_nodes)."""
+    b = Bound(pb)
+    U = np.ascontiguousarray(U_hist, dtype=np.float64)
+    rows, cols = window_nodes(pb, lo, hi)
+    nr = 3 * int(np.prod([len(r) for r in rows]))
+    nc = 3 * int(np.prod([len(c) for c in cols]))
+    out = np.zeros((nr, nc))
+    lo_a = np.ascontiguousarray(lo, dtype=np.int32)
+    hi_a = np.ascontiguousarray(hi, dtype=np.int32)
+    rc = lib().or_tangent_window(b.ptr, _d(U), lo_a.ctypes.data_as(_ip), hi_a.ctypes.data_as(_ip), _d(out))
+    if rc:
+        raise RuntimeError("or_tangent_window failed: %d" % rc)
+    return out
 
 
 def tangent(pb, U_hist, rowptr=None, col=None):

@@ -81,3 +81,44 @@ def test_batch_matches_oracle(oracle):
         assert np.linalg.norm(R.cpu().numpy() - R0) / np.linalg.norm(R0) < 1e-12
         assert np.linalg.norm(V.cpu().numpy() - v0) / np.linalg.norm(v0) < 1e-12
     B.close()
+
+
+def test_batch_follows_update_bc():
+    """igap4h_update_bc on a batched context (a Dirichlet row list of another
+    length, the Neumann load switched off and on again) is picked up by the
+    next batch assembly, which then equals the single-instance calls bit for
+    bit."""
+    import torch
+    from igap4standard_amd.batch import Batch
+    say("batch after update_bc")
+    pbs = small_batch()[:3]
+    pbs[1].neumann = lambda face, order, idof, xq: 0.01 * (1 + idof) * (xq[0] + 0.5 * xq[1] - xq[2])
+    B = Batch(pbs, L.IGAP4H_FP64, L.IGAP4H_VALS_F64)
+
+    def check():
+        B.assemble()
+        torch.cuda.synchronize()
+        Rb = [r.cpu().numpy().copy() for r in B.R]
+        Vb = [v.cpu().numpy().copy() for v in B.vals]
+        for a, U, R, V, rb, vb in zip(B.apps, B.U, B.R, B.vals, Rb, Vb):
+            a.residual_dev(U.data_ptr(), R.data_ptr())
+            a.tangent_dev(U.data_ptr(), V.data_ptr())
+            torch.cuda.synchronize()
+            assert np.array_equal(R.cpu().numpy(), rb)
+            assert np.array_equal(V.cpu().numpy(), vb)
+        return Rb
+
+    R0 = check()
+    n0 = B.apps[0].ndirichlet
+    pbs[0].bc_type[12:18] = 0            # face 2 clamped too: more Dirichlet rows
+    B.apps[0].update_bc()
+    pbs[1].bc_type = P.bc_all_clamped()  # no Neumann face left: the load is dropped
+    B.apps[1].update_bc()
+    assert B.apps[0].ndirichlet > n0
+    R1 = check()
+    assert not np.array_equal(R0[1], R1[1])
+    pbs[1].bc_type = P.bc_clamp_stretch()  # Neumann faces again: the load is rebuilt
+    B.apps[1].update_bc()
+    R2 = check()
+    assert np.array_equal(R2[1], R0[1])
+    B.close()

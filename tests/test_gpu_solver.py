@@ -313,3 +313,47 @@ def test_newton_options_rejected(torch_dev):
         app.newton_solve_dev(Uh.data_ptr(), mg_coarse_deg=9)
     with pytest.raises(L.IgapError):
         app.newton_solve_dev(Uh.data_ptr(), mg_cheb_ratio=1.0)
+
+
+def test_lag_persist_preconditioner_switch(torch_dev, oracle):
+    """A lagged tangent kept across solves (lag_persist) keeps its
+    preconditioner only if the next solve asks for the same one: a block-
+    Jacobi solve followed by a multigrid solve with binary16 smoothing on the
+    reused tangent builds the multigrid hierarchy and the 16-bit copy, and
+    reaches the solution a fresh context reaches."""
+    torch, dev = torch_dev
+    say("lag_persist with a preconditioner switch")
+    pb = P.stretch_problem(6, a=0.01, noise=0.1)
+    U0 = P.affine_guess(pb)
+    kw = dict(rtol=1e-10, atol=0.0, dirichlet_lift=1, tangent_prec=L.IGAP4H_FP32, vals_type=L.IGAP4H_VALS_F32,
+              lag_persist=1, lag_jacobian=100)
+    app = App(pb)
+    U1, i1 = newton_gpu(app, pb, U0, torch, dev, pc=L.IGAP4H_PC_BJACOBI3, **kw)
+    U2, i2 = newton_gpu(app, pb, U0, torch, dev, pc=L.IGAP4H_PC_MG, mg_smoother=L.IGAP4H_MG_CHEBYSHEV,
+                        mg_smooth_vals=L.IGAP4H_SMOOTH_F16, **kw)
+    fresh = App(pb)
+    U3, i3 = newton_gpu(fresh, pb, U0, torch, dev, rtol=1e-10, atol=0.0, dirichlet_lift=1,
+                        tangent_prec=L.IGAP4H_FP32, vals_type=L.IGAP4H_VALS_F32,
+                        mg_smooth_vals=L.IGAP4H_SMOOTH_F16)
+    say("  bjacobi", i1["newton_its"], i1["tangents"], "mg (lagged)", i2["newton_its"], i2["tangents"],
+        "fresh", i3["newton_its"])
+    assert i1["converged"] == 1 and i2["converged"] == 1 and i3["converged"] == 1
+    assert i2["tangents"] == 0          # the first solve's tangent was reused
+    assert rel(U2, U3) < 1e-8, rel(U2, U3)
+    assert np.linalg.norm(oracle.residual(pb, U2)) < 1e-10 * i2["fnorm0"] * 10
+
+
+def test_set_coef_discards_lagged_tangent(torch_dev):
+    """igap4h_set_coef invalidates a tangent kept with lag_persist: the next
+    solve assembles a new one."""
+    torch, dev = torch_dev
+    pb = P.stretch_problem(4, a=0.01, noise=0.1)
+    U0 = P.affine_guess(pb)
+    kw = dict(rtol=1e-10, atol=0.0, dirichlet_lift=1, lag_persist=1, lag_jacobian=100)
+    app = App(pb)
+    _, i1 = newton_gpu(app, pb, U0, torch, dev, **kw)
+    _, i2 = newton_gpu(app, pb, U0, torch, dev, **kw)
+    assert i1["tangents"] == 1 and i2["tangents"] == 0
+    app.set_coef(pb.coef() * 1.5)
+    _, i3 = newton_gpu(app, pb, U0, torch, dev, **kw)
+    assert i3["tangents"] == 1 and i3["converged"] == 1

@@ -193,3 +193,50 @@ def test_getime_assembly_consistency(oracle):
     fd = (oracle.residual(pb, np.concatenate([U + h * dU, U1, U0]))
           - oracle.residual(pb, np.concatenate([U - h * dU, U1, U0]))) / (2 * h)
     assert np.abs(K @ dU - fd).max() < 1e-7 * np.abs(fd).max()
+
+
+def window_dofs(pb, ranges):
+    nb = pb.nbasis
+    a, b, c = np.meshgrid(*ranges, indexing="ij")
+    node = ((a * nb[1] + b) * nb[2] + c).reshape(-1)
+    return (3 * node[:, None] + np.arange(3)[None, :]).reshape(-1)
+
+
+@pytest.mark.parametrize("physics", [P.GE, P.GETIME])
+def test_tangent_window_is_assembly_block(oracle, physics):
+    """or_tangent_window equals the rows of the assembled element sums
+    (or_tangent_elements, AIJ scatter) on windows at the corner, inside and
+    on a face, and those rows have no entry outside the window's columns."""
+    pb = P.stretch_problem((5, 4, 6), a=0.01, noise=0.1)
+    pb.physics = physics
+    U = P.smooth_state(pb, amp=0.05, noise=1e-3)
+    rowptr, col = oracle.pattern(pb)
+    v = np.zeros(len(col))
+    oracle.tangent_elements_timed(pb, U, rowptr, col, v, 0, pb.n_elem)
+    K = dense(rowptr, col, v, pb.ndof)
+    for lo, hi in [((0, 0, 0), (5, 4, 6)), ((1, 0, 2), (4, 3, 6)), ((0, 1, 0), (3, 4, 4))]:
+        W = oracle.tangent_window(pb, U, lo, hi)
+        rows, cols = oracle.window_nodes(pb, lo, hi)
+        ri, ci = window_dofs(pb, rows), window_dofs(pb, cols)
+        assert W.shape == (len(ri), len(ci))
+        assert np.array_equal(K[np.ix_(ri, ci)], W)
+        outside = np.ones(pb.ndof, bool)
+        outside[ci] = False
+        assert not np.any(K[np.ix_(ri, np.nonzero(outside)[0])])
+
+
+def test_residual_threads_same_sum(oracle):
+    """or_residual_mt (element loop split over threads, partial vectors
+    added) equals or_residual up to the order of the sums, Neumann and
+    Dirichlet rows included; getime too."""
+    pb = P.stretch_problem((5, 4, 6), a=0.01, noise=0.1)
+    pb.neumann = lambda face, order, idof, xq: 0.1 * (1 + idof) * xq[0] - 0.05 * order
+    U = P.smooth_state(pb, amp=0.05, noise=1e-3)
+    R = oracle.residual(pb, U)
+    for nt in (1, 3, 7):
+        Rm = oracle.residual_mt(pb, U, nt)
+        assert np.linalg.norm(Rm - R) <= 1e-14 * np.linalg.norm(R)
+    pt = P.transient_problem(4)
+    Ut = P.smooth_state(pt, amp=0.05, noise=1e-3)
+    assert np.linalg.norm(oracle.residual_mt(pt, Ut, 5) - oracle.residual(pt, Ut)) <= \
+        1e-14 * np.linalg.norm(oracle.residual(pt, Ut))
