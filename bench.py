@@ -15,13 +15,27 @@ Also timed: the assembly alone (residual + tangent at a non-equilibrium state
 U = 1e-3 sin(pi X) sin(pi Y) sin(pi Z) (1, .5, .25) + 1e-6 noise) and the
 fine-level row pass (SpMV) that the Krylov and multigrid passes share.
 
-    python bench.py [--steps K] [--warmup W] [--n 64] [--no-cpu]
+Extra keys ("configs"), each timed on the GPU in the same run:
+  config1  8^3, 5 load steps u_x = 0.01 k / 5, 3x3x3 Gauss, first guess
+           1e-4 sin(i), reference Dirichlet semantics (no lift);
+  config2  64^3, 5 load steps (Lame +-10 %), each from the previous solution
+           plus the affine load increment, in the three forms; also the
+           reference Dirichlet semantics and the 1e-4 sin(i) first guess;
+  config3  48^3 getime, 100 steps of dt = 1e-3, FP32-MFMA tangent, plus the
+           getime tangent kernel's MFMA roofline;
+  config4  256 batched instances {8..24}^3 (one graph replay per assembly),
+           autotuner table and FP32 MFMA fraction;
+  config5  32^3 adjoint: solves, gradient, backward kernel vs central
+           differences on 64 elements, dL/dE vs differences of the loss.
+
+    python bench.py [--steps K] [--warmup W] [--n 64] [--no-cpu] [--no-extras]
 
 Prints one JSON line.
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -77,6 +91,29 @@ def _cpu_worker(w):
     return t_k, t_r
 
 
+def host_description():
+    """CPU model, clock and core counts of this host (lscpu, else /proc/cpuinfo)."""
+    d = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "CPU max MHz", "CPU MHz", "CPU(s)", "Socket(s)", "Core(s) per socket",
+                             "Thread(s) per core"):
+                d[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            if k.strip() in ("model name", "cpu MHz") and k.strip() not in d:
+                d[k.strip()] = v.strip()
+    except OSError:
+        pass
+    d["cpus_usable"] = len(os.sched_getaffinity(0))
+    return d
+
+
 def cpu_baseline(pb, U, budget_s=10.0, procs=None):
     """The reference algorithm restated in C (oracle/): per-point basis, naive
     81x81x10x10 contraction, AIJ insertion, one process per core (as the
@@ -111,7 +148,118 @@ def cpu_baseline(pb, U, budget_s=10.0, procs=None):
                       "%dx%dx%d workload (%.1f s wall). Assembly only: an upper bound on a CPU Newton step's "
                       "elements/s (the CPU linear solve is not timed)" % ((nproc, nproc, n_k, n_r) + tuple(pb.nelem) + (wall,)),
             "tangent_s_per_element_per_core": t_k1, "residual_s_per_element_per_core": t_r1,
-            "single_core_elements_per_s": 1.0 / (t_k1 + t_r1)}
+            "single_core_elements_per_s": 1.0 / (t_k1 + t_r1),
+            "assembly_s_per_newton_step": pb.n_elem / rate,
+            "newton_step_note": "a CPU Newton step costs at least one residual + one tangent assembly of all "
+                                "elements (assembly_s_per_newton_step, on these cores); the linear solve and "
+                                "line-search residuals come on top and are not timed",
+            "host": host_description()}
+
+
+def _events(torch, stream, fn, reps):
+    """Mean ms of fn() over reps launches, HIP events on `stream`."""
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def _steps_summary(steps):
+    nt = sum(r["newton_its"] for r in steps)
+    return {"total_s": sum(r["s"] for r in steps), "newton_its": [r["newton_its"] for r in steps],
+            "ksp_its": [r["ksp_its_per_newton"] for r in steps],
+            "ms_per_newton_step": 1000.0 * sum(r["s"] for r in steps) / max(nt, 1),
+            "step_s": [r["s"] for r in steps], "converged": [r["converged"] for r in steps],
+            "fnorm": ["%.2e" % r["fnorm"] for r in steps],
+            "all_converged": all(r["converged"] == 1 for r in steps)}
+
+
+def config_legs(say):
+    """The other workloads, each on its own contexts (closed before the next)."""
+    import torch
+
+    from igap4standard_amd import _lib as L
+    from igap4standard_amd import adjoint, batch, loadsteps, transient
+    out = {}
+
+    say("config1: 8^3, 5 load steps")
+    st = loadsteps.run(n=8, steps=5, a=0.01, nquad=3, form="fp64", lift=False, init="sin")
+    out["config1_8cube_5_load_steps"] = dict(_steps_summary(st), workload=(
+        "8^3 clamp/stretch, lambda 1, mu 0.5, l 0.1, 3x3x3 Gauss, u_x = 0.01 k/5, first guess 1e-4 sin(i) "
+        "(example/ex1/main.c:49), reference Dirichlet semantics (no lift), ||R|| < 1e-10, FP64"))
+
+    say("config2: 64^3, 5 load steps")
+    c2 = {"workload": "64^3 clamp/stretch, Lame +-10 % (seed 0x5EED0002), nq 4, u_x = 0.01 k/5; each step from "
+                      "the previous solution plus the affine load increment (the app's initguess), Dirichlet rows "
+                      "lifted, ||R|| < 1e-10 (reference absolute tolerance)"}
+    for form in ("fp64", "mixed", "mixed_f16s"):
+        st = loadsteps.run(n=64, steps=5, a=0.01, nquad=4, noise=0.1, form=form, lift=True, init="zero",
+                           extend=True)
+        c2[form] = _steps_summary(st)
+        say("  ", form, json.dumps(c2[form]))
+    for form in ("mixed", "mixed_f16s"):
+        c2[form + "_speedup_vs_fp64"] = c2["fp64"]["total_s"] / c2[form]["total_s"]
+    st = loadsteps.run(n=64, steps=5, a=0.01, nquad=4, noise=0.1, form="fp64", lift=False, init="zero",
+                       extend=True, max_newton=10)
+    c2["fp64_reference_dirichlet"] = dict(_steps_summary(st), note=(
+        "no lift: the tangent's zeroed Dirichlet columns (snesiga_Tangent.c:270) drop the coupling of the "
+        "Dirichlet rows' correction, the interior step is wrong and the backtracking line search fails "
+        "(converged -3)"))
+    st = loadsteps.run(n=64, steps=5, a=0.01, nquad=4, noise=0.1, form="fp64", lift=False, init="sin",
+                       extend=True, kmax=1, max_newton=3, ksp_max_it=2000)
+    c2["fp64_sin_guess_first_step"] = dict(_steps_summary(st), note=(
+        "first load step from 1e-4 sin(i) (example/ex1/main.c:49), capped at 3 Newton steps of at most "
+        "2000 GMRES iterations: the guess oscillates at the grid scale, its strain gradients (~1e-4/h^2) "
+        "make the tangent indefinite at 64^3"))
+    out["config2_64cube_5_load_steps"] = c2
+
+    say("config3: getime 48^3, 100 steps")
+    c3, ts = transient.run(48, 100, 1e-3, fp64=False, say=lambda *a: None)
+    stream = torch.cuda.current_stream()
+    app, pb = ts.app, ts.pb
+    dR = torch.empty(app.ndof, device="cuda:0", dtype=torch.float64)
+    c3["residual_ms"] = _events(torch, stream, lambda: app.residual_dev(ts.Uh.data_ptr(), dR.data_ptr(),
+                                                                        stream.cuda_stream), 5)
+    for prec, vt, dt, nm, peak in ((L.IGAP4H_FP32, L.IGAP4H_VALS_F32, torch.float32, "k_tangent_gt<float,4>",
+                                    PEAK["f32"]),
+                                   (L.IGAP4H_FP64, L.IGAP4H_VALS_F64, torch.float64, "k_tangent_gt<double,2>",
+                                    PEAK["f64"])):
+        V = torch.empty(app.nnz, device="cuda:0", dtype=dt)
+        ms = _events(torch, stream, lambda: app.tangent_dev(ts.Uh.data_ptr(), V.data_ptr(), prec, vt,
+                                                            stream.cuda_stream), 3)
+        tf = pb.n_elem * FLOP_KE_PER_QP * pb.nquad ** 3 / (ms * 1e-3) / 1e12
+        c3["roofline_tangent_" + ("f32" if prec == L.IGAP4H_FP32 else "f64")] = {
+            "bound": "mfma", "kernel": nm + " (27 colour launches per tangent)", "tangent_ms": ms,
+            "achieved": tf, "peak": peak, "unit": "TFLOP/s", "frac": tf / peak}
+        del V
+    ts.app.close()
+    c3["workload"] = ("48^3 getime (torder 2), all faces clamped, rho 1, lambda 1, mu 0.5, l 0.05, dt 1e-3, v0 a "
+                      "Gaussian random field (std 1e-3, correlation 0.2), FP64 residual, FP32-MFMA tangent, "
+                      "||R|| < 1e-10 per step")
+    out["config3_getime_48cube_100_steps"] = c3
+    say("  ", json.dumps({k: v for k, v in c3.items() if k != "u_norm_history"}))
+
+    say("config4: 256 batched instances")
+    c4 = batch.run(256, reps=3, fp64=False, say=lambda *a: say("  ", *a))
+    best = min(c4["autotuner_ms"].values())
+    flop = c4["nelem"] * FLOP_KE_PER_QP * 27
+    c4["roofline_tangent_f32"] = {"bound": "mfma", "kernel": "k_tangent_ge_batch<float,QC> (tuned variant)",
+                                  "achieved": flop / (best * 1e-3) / 1e12, "peak": PEAK["f32"], "unit": "TFLOP/s",
+                                  "frac": flop / (best * 1e-3) / 1e12 / PEAK["f32"]}
+    c4["assembly_tflops_f32_tangent_share"] = flop / (c4["graph_ms_per_assembly"] * 1e-3) / 1e12
+    out["config4_batch_256"] = c4
+    say("  ", json.dumps(c4))
+
+    say("config5: 32^3 adjoint")
+    c5 = adjoint.run(32, nfd=64, h=1e-6, loss_fd=4, loss_h=1e-3)
+    out["config5_adjoint_32cube"] = c5
+    say("  ", json.dumps(c5))
+    return out
 
 
 def main():
@@ -129,6 +277,7 @@ def main():
                          "fine-level multigrid smoothing passes on a row-scaled binary16 / bfloat16 copy "
                          "of the FP32 tangent)")
     ap.add_argument("--no-newton", action="store_true", help="assembly and SpMV only")
+    ap.add_argument("--no-extras", action="store_true", help="only the 64^3 headline workload")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -322,8 +471,11 @@ def main():
         if name in newton and "fp64" in newton:
             line["newton"][name + "_speedup_vs_fp64"] = newton["fp64"]["solve_s"] / newton[name]["solve_s"]
     line["cpu_baseline"] = cpu
-    print(json.dumps(line), flush=True)
     app.close()
+    del dK64, dK32, dUh, dR
+    if not args.no_extras:
+        line["configs"] = config_legs(say)
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

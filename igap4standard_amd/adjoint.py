@@ -102,20 +102,13 @@ class Calibration:
         return np.array(out)
 
 
-def main(argv=None):
-    import argparse
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--n", type=int, default=32)
-    ap.add_argument("--nfd", type=int, default=64, help="elements checked by finite differences")
-    ap.add_argument("--h", type=float, default=1e-6, help="central FD step in E_e")
-    ap.add_argument("--loss-fd", type=int, default=0, help="elements also checked by FD of the full loss")
-    ap.add_argument("--loss-h", type=float, default=1e-3)
-    args = ap.parse_args(argv)
-
-    def say(*a):
-        print("adjoint:", *a, file=sys.stderr, flush=True)
-
-    pb, E, Et = P.adjoint_problem(args.n)
+def run(n=32, nfd=64, h=1e-6, loss_fd=0, loss_h=1e-3, say=None):
+    """Config-5 run: target and forward solves, loss, adjoint gradient, the
+    backward kernel against central differences on nfd random elements
+    (seed 0x5EED0007) and, on loss_fd of them, dL/dE against central
+    differences of the loss."""
+    say = say or (lambda *a: None)
+    pb, E, Et = P.adjoint_problem(n)
     cal = Calibration(pb)
     t0 = time.perf_counter()
     Ut, it_t = cal.solve(Et)
@@ -128,31 +121,47 @@ def main(argv=None):
     t_adj = time.perf_counter() - t0
     say("loss %.6e, adjoint GMRES its %d, %.3f s" % (loss, its, t_adj))
     rng = np.random.default_rng(0x5EED0007)
-    elems = rng.choice(pb.n_elem, size=min(args.nfd, pb.n_elem), replace=False)
+    elems = rng.choice(pb.n_elem, size=min(nfd, pb.n_elem), replace=False)
     # backward kernel vs central FD of lam^T R in E_e (step h)
     v_adj = cal.vjp_E(U, cal.lam, E)[elems]
-    v_fd = cal.vjp_E_fd(U, cal.lam, E, elems, args.h)
+    v_fd = cal.vjp_E_fd(U, cal.lam, E, elems, h)
     rel_vjp = float(np.linalg.norm(v_adj - v_fd) / np.linalg.norm(v_fd))
     say("VJP vs FD on %d elements: rel err %.2e (max elementwise %.2e)" % (
         len(elems), rel_vjp, float(np.max(np.abs(v_adj - v_fd) / np.abs(v_fd)))))
-    out = {"n": args.n, "nelem": pb.n_elem, "ndof": pb.ndof, "loss": loss, "forward_s": t_fwd, "adjoint_s": t_adj,
-           "adjoint_its": its, "vjp_fd": {"elements": len(elems), "step": args.h, "rel_err": rel_vjp},
+    out = {"n": n, "nelem": pb.n_elem, "ndof": pb.ndof, "loss": loss, "forward_s": t_fwd, "adjoint_s": t_adj,
+           "forward_newton_its": it_f["newton_its"], "adjoint_its": its,
+           "vjp_fd": {"elements": len(elems), "step": h, "rel_err": rel_vjp},
            "grad_norm": float(np.linalg.norm(gE))}
-    if args.loss_fd:
-        chk = elems[:args.loss_fd]
+    if loss_fd:
+        chk = elems[:loss_fd]
         fd = []
         for e in chk:
             lv = []
             for sgn in (1.0, -1.0):
                 Ep = E.copy()
-                Ep[e] += sgn * args.loss_h
+                Ep[e] += sgn * loss_h
                 Up, _ = cal.solve(Ep, warm=U)
                 lv.append(cal.loss(Up, Ut))
-            fd.append((lv[0] - lv[1]) / (2.0 * args.loss_h))
+            fd.append((lv[0] - lv[1]) / (2.0 * loss_h))
         fd = np.array(fd)
         rel = float(np.linalg.norm(gE[chk] - fd) / np.linalg.norm(fd))
-        say("dL/dE vs FD of the loss on %d elements (step %g): rel err %.2e" % (len(chk), args.loss_h, rel))
-        out["loss_fd"] = {"elements": len(chk), "step": args.loss_h, "rel_err": rel}
+        say("dL/dE vs FD of the loss on %d elements (step %g): rel err %.2e" % (len(chk), loss_h, rel))
+        out["loss_fd"] = {"elements": len(chk), "step": loss_h, "rel_err": rel}
+    cal.app.close()
+    return out
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=32)
+    ap.add_argument("--nfd", type=int, default=64, help="elements checked by finite differences")
+    ap.add_argument("--h", type=float, default=1e-6, help="central FD step in E_e")
+    ap.add_argument("--loss-fd", type=int, default=0, help="elements also checked by FD of the full loss")
+    ap.add_argument("--loss-h", type=float, default=1e-3)
+    args = ap.parse_args(argv)
+    out = run(args.n, args.nfd, args.h, args.loss_fd, args.loss_h,
+              say=lambda *a: print("adjoint:", *a, file=sys.stderr, flush=True))
     print(json.dumps(out), flush=True)
 
 

@@ -94,6 +94,53 @@ class Batch:
             pass
 
 
+def run(count=256, reps=5, fp64=False, seed=0x5EED0004, sequential=True, say=None):
+    """Config-4 run: count instances, autotuner, timed graph replays of the
+    whole assembly (and, optionally, the same work instance by instance)."""
+    import torch
+    say = say or (lambda *a: None)
+    pbs = batch_problems(count, seed=seed)
+    prec, vt = (L.IGAP4H_FP64, L.IGAP4H_VALS_F64) if fp64 else (L.IGAP4H_FP32, L.IGAP4H_VALS_F32)
+    t0 = time.perf_counter()
+    B = Batch(pbs, prec, vt)
+    t_setup = time.perf_counter() - t0
+    say("%d instances, %d elements, set-up %.1f s" % (len(pbs), B.nelem, t_setup))
+    best, table = B.tune(reps)
+    say("autotuner (Gauss points per MFMA chunk -> tangent ms):", table, "-> %d" % best)
+    stream = torch.cuda.current_stream()
+    B.assemble(stream.cuda_stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        B.assemble(stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    ms_seq = None
+    if sequential:  # the same work instance by instance (residual_dev + tangent_dev per context)
+        def seq():
+            for a, U, R, V in zip(B.apps, B.U, B.R, B.vals):
+                a.residual_dev(U.data_ptr(), R.data_ptr(), stream.cuda_stream)
+                a.tangent_dev(U.data_ptr(), V.data_ptr(), prec, vt, stream.cuda_stream)
+        seq()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            seq()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms_seq = e0.elapsed_time(e1) / reps
+    sizes = {n: sum(1 for pb in pbs if pb.nelem[0] == n) for n in SIZES}
+    out = {"instances": len(pbs), "nelem": B.nelem, "sizes": sizes, "nquad": 3, "setup_s": t_setup,
+           "precision": "fp64" if fp64 else "fp32-mfma tangent (fp64 chunk accumulation), fp32 values, fp64 residual",
+           "autotuner_ms": table, "variant": best, "graph_ms_per_assembly": ms,
+           "elements_per_s": B.nelem / (ms * 1e-3), "sequential_ms_per_assembly": ms_seq,
+           "graph_speedup_vs_sequential": (ms_seq / ms) if ms_seq else None}
+    B.close()
+    return out
+
+
 def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser()
@@ -101,50 +148,8 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--fp64", action="store_true")
     args = ap.parse_args(argv)
-    import torch
-
-    def say(*a):
-        print("batch:", *a, file=sys.stderr, flush=True)
-
-    pbs = batch_problems(args.count)
-    prec, vt = (L.IGAP4H_FP64, L.IGAP4H_VALS_F64) if args.fp64 else (L.IGAP4H_FP32, L.IGAP4H_VALS_F32)
-    t0 = time.perf_counter()
-    B = Batch(pbs, prec, vt)
-    say("%d instances, %d elements, set-up %.1f s" % (len(pbs), B.nelem, time.perf_counter() - t0))
-    best, table = B.tune(args.reps)
-    say("autotuner (Gauss points per MFMA chunk -> tangent ms):", table, "-> %d" % best)
-    stream = torch.cuda.current_stream()
-    B.assemble(stream.cuda_stream)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(args.reps):
-        B.assemble(stream.cuda_stream)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / args.reps
-    # the same work instance by instance (igap4h_residual_dev + igap4h_tangent_dev per context)
-    def sequential():
-        for a, U, R, V in zip(B.apps, B.U, B.R, B.vals):
-            a.residual_dev(U.data_ptr(), R.data_ptr(), stream.cuda_stream)
-            a.tangent_dev(U.data_ptr(), V.data_ptr(), prec, vt, stream.cuda_stream)
-    sequential()
-    torch.cuda.synchronize()
-    e0.record(stream)
-    for _ in range(args.reps):
-        sequential()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms_seq = e0.elapsed_time(e1) / args.reps
-    sizes = {n: sum(1 for pb in pbs if pb.nelem[0] == n) for n in SIZES}
-    out = {"instances": len(pbs), "nelem": B.nelem, "sizes": sizes, "nquad": 3,
-           "precision": "fp64" if args.fp64 else "fp32-mfma tangent (fp64 chunk accumulation), fp32 values, fp64 residual",
-           "autotuner_ms": table, "variant": best, "graph_ms_per_assembly": ms,
-           "elements_per_s": B.nelem / (ms * 1e-3), "sequential_ms_per_assembly": ms_seq,
-           "graph_speedup_vs_sequential": ms_seq / ms}
-    say(json.dumps(out))
+    out = run(args.count, args.reps, args.fp64, say=lambda *a: print("batch:", *a, file=sys.stderr, flush=True))
     print(json.dumps(out), flush=True)
-    B.close()
 
 
 if __name__ == "__main__":

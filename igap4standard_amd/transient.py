@@ -56,6 +56,47 @@ class TimeStepper:
         return info
 
 
+def run(n=48, steps=100, dt=1e-3, fp64=False, lag=1, persist=False, say=None):
+    """Config-3 run: n^3 getime, all faces clamped, v0 a Gaussian random field
+    (std 1e-3, correlation 0.2), `steps` steps of dt.  Returns (summary dict,
+    the TimeStepper, whose context stays open)."""
+    import torch
+    say = say or (lambda *a: None)
+    pb = P.transient_problem(n, dt=dt)
+    v0 = P.initial_velocity(pb)
+    ts = TimeStepper(pb, mixed=not fp64, lag_jacobian=lag, lag_persist=int(persist))
+    ts.set_state(np.zeros(pb.ndof), -dt * v0)
+    say("%d^3 getime, %d dofs, dt %g, %s tangent" % (n, pb.ndof, dt, "FP64" if fp64 else "FP32"))
+    infos, norms = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        info = ts.step()
+        infos.append(info)
+        if info["converged"] != 1:
+            say("step %d did not converge: %s" % (k, info))
+            break
+        if k % 10 == 0 or k == steps - 1:
+            u = float(torch.linalg.vector_norm(ts.Uh[:ts.n]))
+            norms.append((k + 1, u))
+            say("step %d: %d Newton, %d Krylov, |R| %.2e, |U| %.4e, %.1f s" % (
+                k + 1, info["newton_its"], info["ksp_its"], info["fnorm"], u, time.perf_counter() - t0))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    done = len(infos)
+    out = {"n": n, "ndof": pb.ndof, "nelem": pb.n_elem, "dt": dt, "steps": done,
+           "tangent": "fp64" if fp64 else "fp32-mfma (fp32 values)", "total_s": wall,
+           "ms_per_step": 1000.0 * wall / max(done, 1),
+           "newton_its_mean": float(np.mean([i["newton_its"] for i in infos])),
+           "ksp_its_mean": float(np.mean([i["ksp_its"] for i in infos])),
+           "tangents": int(sum(i["tangents"] for i in infos)), "lag_jacobian": lag, "lag_persist": persist,
+           "max_final_fnorm": float(max(i["fnorm"] for i in infos)),
+           "all_converged": all(i["converged"] == 1 for i in infos),
+           "elements_per_s_per_newton_step": pb.n_elem * sum(i["newton_its"] for i in infos) / wall,
+           "u_norm_history": norms}
+    return out, ts
+
+
 def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser()
@@ -66,43 +107,9 @@ def main(argv=None):
     ap.add_argument("--lag", type=int, default=1, help="rebuild the tangent every LAG Newton steps")
     ap.add_argument("--persist", action="store_true", help="count the lag across time steps")
     args = ap.parse_args(argv)
-    import torch
-
-    def say(*a):
-        print("transient:", *a, file=sys.stderr, flush=True)
-
-    pb = P.transient_problem(args.n, dt=args.dt)
-    v0 = P.initial_velocity(pb)
-    ts = TimeStepper(pb, mixed=not args.fp64, lag_jacobian=args.lag, lag_persist=int(args.persist))
-    ts.set_state(np.zeros(pb.ndof), -args.dt * v0)
-    say("%d^3 getime, %d dofs, dt %g, %s tangent" % (args.n, pb.ndof, args.dt, "FP64" if args.fp64 else "FP32"))
-    infos, norms = [], []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        info = ts.step()
-        infos.append(info)
-        if info["converged"] != 1:
-            say("step %d did not converge: %s" % (k, info))
-            break
-        if k % 10 == 0 or k == args.steps - 1:
-            u = float(torch.linalg.vector_norm(ts.Uh[:ts.n]))
-            norms.append((k + 1, u))
-            say("step %d: %d Newton, %d Krylov, |R| %.2e, |U| %.4e, %.1f s" % (
-                k + 1, info["newton_its"], info["ksp_its"], info["fnorm"], u, time.perf_counter() - t0))
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    done = len(infos)
-    out = {"n": args.n, "ndof": pb.ndof, "nelem": pb.n_elem, "dt": args.dt, "steps": done,
-           "tangent": "fp64" if args.fp64 else "fp32-mfma (fp32 values)", "total_s": wall,
-           "ms_per_step": 1000.0 * wall / max(done, 1),
-           "newton_its_mean": float(np.mean([i["newton_its"] for i in infos])),
-           "ksp_its_mean": float(np.mean([i["ksp_its"] for i in infos])),
-           "tangents": int(sum(i["tangents"] for i in infos)), "lag_jacobian": args.lag, "lag_persist": args.persist,
-           "max_final_fnorm": float(max(i["fnorm"] for i in infos)),
-           "all_converged": all(i["converged"] == 1 for i in infos),
-           "elements_per_s_per_newton_step": pb.n_elem * sum(i["newton_its"] for i in infos) / wall,
-           "u_norm_history": norms}
+    out, ts = run(args.n, args.steps, args.dt, args.fp64, args.lag, args.persist,
+                  say=lambda *a: print("transient:", *a, file=sys.stderr, flush=True))
+    ts.app.close()
     print(json.dumps(out), flush=True)
 
 

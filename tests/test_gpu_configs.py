@@ -1,0 +1,340 @@
+"""The benchmark workloads at their stated sizes, through the C-ABI, against
+the oracle.
+
+- 64^3 clamp/stretch, Lame +-10 % per element (seed 0x5EED0002), nq = 4:
+  pattern and Dirichlet rows bit-exact (nnz 306,110,016); R in full within
+  1e-12; K on the complete rows of three 6^3-element windows (interior, the
+  clamped corner, the stretched face) within 1e-12 (FP64) / 1e-5 (FP32) of
+  the oracle's element sums (src/snesiga_Tangent.c:147-190) with
+  MatZeroRowsColumns applied; the Newton solutions of the FP64, mixed and
+  binary16-smoothed forms satisfy the oracle's residual.
+- 8^3 with 5 load steps u_x = 0.01 k / 5 (nq 3 and 4, reference Dirichlet
+  semantics and lifted): every step satisfies the oracle's residual.
+- 48^3 getime, 100 steps of dt = 1e-3 from a Gaussian random velocity field:
+  steps 1, 50 and 100 satisfy the oracle's getime residual with the same
+  history; the FP32-tangent trajectory matches the FP64 one.
+- 256 batched instances {8..24}^3 (seed 0x5EED0004): every instance bit for
+  bit equal to the single-instance entry points; the 8^3 instances' R and K
+  within 1e-12 of the oracle.
+- 32^3 adjoint (log-normal Young's modulus): backward kernel vs central
+  differences on 64 elements (seed 0x5EED0007, step 1e-6) and dL/dE vs
+  central differences of the loss, both within 1e-6."""
+import concurrent.futures
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from igap4standard_amd import _lib as L
+from igap4standard_amd import problem as P
+from igap4standard_amd.snesiga import App
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def workers():
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+# ------------------------------------------------------------------ 64^3
+@pytest.fixture(scope="module")
+def cube64():
+    say("64^3: building the context")
+    pb = P.stretch_problem(64, a=0.01, lam=1.0, mu=0.5, ell=0.1, noise=0.1, seed=0x5EED0002, nquad=4)
+    U = P.smooth_state(pb, amp=1e-3, noise=1e-6, seed=0x5EED0012)
+    app = App(pb)
+    rowptr, col = app.pattern()
+    yield pb, U, app, rowptr, col
+    app.close()
+
+
+def test_64cube_pattern_and_dirichlet_rows(cube64, oracle):
+    pb, U, app, rowptr, col = cube64
+    say("64^3: pattern")
+    assert app.ndof == 862488 and app.nnz == 306110016 == pb.nnz
+    r0, c0 = oracle.pattern(pb)
+    assert np.array_equal(rowptr, r0)
+    assert np.array_equal(col, c0)
+    del r0, c0
+    assert np.array_equal(app.dirichlet_rows(), oracle.dirichlet_rows(pb))
+
+
+def test_64cube_residual(cube64, oracle):
+    import torch
+    pb, U, app, _, _ = cube64
+    say("64^3: residual")
+    dU = torch.tensor(U, device="cuda:0")
+    dR = torch.empty_like(dU)
+    app.residual_dev(dU.data_ptr(), dR.data_ptr())
+    torch.cuda.synchronize()
+    R = dR.cpu().numpy()
+    R0 = oracle.residual_mt(pb, U, workers())
+    e = rel(R, R0)
+    say("  |dR|/|R| = %.2e" % e)
+    assert e < 1e-12
+    rows = oracle.dirichlet_rows(pb)
+    assert np.abs(R[rows] - R0[rows]).max() <= 1e-15
+
+
+WINDOWS = [((29, 29, 29), (35, 35, 35)), ((0, 0, 0), (6, 6, 6)), ((58, 29, 58), (64, 35, 64))]
+
+
+def window_dofs(pb, ranges):
+    nb = pb.nbasis
+    a, b, c = np.meshgrid(*ranges, indexing="ij")
+    node = ((a * nb[1] + b) * nb[2] + c).reshape(-1)
+    return (3 * node[:, None] + np.arange(3)[None, :]).reshape(-1)
+
+
+@pytest.fixture(scope="module")
+def windows64(cube64, oracle):
+    """Oracle blocks of the three windows with MatZeroRowsColumns(diag 1)."""
+    pb, U, _, _, _ = cube64
+    say("64^3: oracle tangent windows")
+    drows = np.zeros(pb.ndof, bool)
+    drows[oracle.dirichlet_rows(pb)] = True
+    with concurrent.futures.ThreadPoolExecutor(len(WINDOWS)) as ex:
+        blocks = list(ex.map(lambda w: oracle.tangent_window(pb, U, w[0], w[1]), WINDOWS))
+    out = []
+    for (lo, hi), W in zip(WINDOWS, blocks):
+        rows, cols = oracle.window_nodes(pb, lo, hi)
+        ri, ci = window_dofs(pb, rows), window_dofs(pb, cols)
+        W[drows[ri], :] = 0.0
+        W[:, drows[ci]] = 0.0
+        for k, r in enumerate(ri):
+            if drows[r]:
+                W[k, np.searchsorted(ci, r)] = 1.0
+        out.append((lo, hi, ri, ci, W))
+    return out
+
+
+@pytest.mark.parametrize("prec,vt,tol", [(L.IGAP4H_FP64, L.IGAP4H_VALS_F64, 1e-12),
+                                         (L.IGAP4H_FP32, L.IGAP4H_VALS_F32, 1e-5),
+                                         (L.IGAP4H_FP32, L.IGAP4H_VALS_F64, 1e-5)])
+def test_64cube_tangent_windows(cube64, windows64, prec, vt, tol):
+    import torch
+    pb, U, app, rowptr, col = cube64
+    say("64^3: tangent", prec, vt)
+    dU = torch.tensor(U, device="cuda:0")
+    dK = torch.empty(app.nnz, device="cuda:0", dtype=torch.float64 if vt == L.IGAP4H_VALS_F64 else torch.float32)
+    app.tangent_dev(dU.data_ptr(), dK.data_ptr(), prec, vt)
+    torch.cuda.synchronize()
+    vals = dK.cpu().numpy().astype(np.float64)
+    del dK
+    lut = np.full(pb.ndof, -1, np.int64)
+    for lo, hi, ri, ci, W in windows64:
+        lut[ci] = np.arange(len(ci))
+        start, stop = rowptr[ri], rowptr[ri + 1]
+        cnt = stop - start
+        idx = np.concatenate([np.arange(a, b) for a, b in zip(start, stop)])
+        rloc = np.repeat(np.arange(len(ri)), cnt)
+        cloc = lut[col[idx]]
+        assert np.all(cloc >= 0), "a complete row has a column outside its window"
+        G = np.zeros_like(W)
+        G[rloc, cloc] = vals[idx]
+        e = rel(G, W)
+        say("  window", lo, hi, "rows", len(ri), "|dK|/|K| = %.2e" % e)
+        assert e < tol
+        lut[ci] = -1
+
+
+_SOLUTIONS64 = {}
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed", "mixed_f16s"])
+def test_64cube_newton_solutions(cube64, oracle, form):
+    """The bench's solve (affine guess, Dirichlet rows lifted, ||R||/||R0||
+    < 1e-10) in each form: the solution satisfies the oracle's residual to
+    the reference's absolute tolerance, and the forms agree."""
+    import torch
+    from igap4standard_amd.loadsteps import FORMS
+    pb, _, app, _, _ = cube64
+    say("64^3: Newton", form)
+    U = torch.tensor(P.affine_guess(pb), device="cuda:0")
+    info = app.newton_solve_dev(U.data_ptr(), rtol=1e-10, atol=0.0, dirichlet_lift=1, **FORMS[form])
+    torch.cuda.synchronize()
+    say("  ", info["newton_its"], "Newton steps,", info["ksp_its"], "Krylov, |R| = %.2e" % info["fnorm"])
+    assert info["converged"] == 1 and info["fnorm"] < 1e-10 * info["fnorm0"]
+    Un = U.cpu().numpy()
+    R0 = oracle.residual_mt(pb, Un, workers())
+    say("  oracle |R| = %.2e" % np.linalg.norm(R0))
+    assert np.linalg.norm(R0) < 1e-10
+    _SOLUTIONS64[form] = Un
+    if "fp64" in _SOLUTIONS64 and form != "fp64":
+        assert rel(Un, _SOLUTIONS64["fp64"]) < 1e-8, rel(Un, _SOLUTIONS64["fp64"])
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed_f16s"])
+def test_64cube_five_load_steps(oracle, form):
+    """u_x = 0.01 k / 5 at 64^3 (Lame +-10 %): each step from the previous
+    solution plus the affine load increment, Dirichlet rows lifted; every
+    step reaches the reference's absolute tolerance on the oracle's
+    residual."""
+    from igap4standard_amd import loadsteps
+    say("64^3 load steps", form)
+    norms = []
+
+    def cb(pb, U, rec):
+        R = oracle.residual_mt(pb, U.cpu().numpy(), workers())
+        norms.append(float(np.linalg.norm(R)))
+        say("  step", rec["step"], rec["newton_its"], rec["ksp_its_per_newton"], "%.3f s" % rec["s"],
+            "|R| gpu %.2e oracle %.2e" % (rec["fnorm"], norms[-1]))
+        assert rec["converged"] == 1
+
+    steps = loadsteps.run(n=64, steps=5, a=0.01, nquad=4, noise=0.1, form=form, lift=True, init="zero",
+                          extend=True, callback=cb)
+    assert len(steps) == 5 and max(norms) < 1e-10
+    e = [r["energy"] for r in steps]
+    assert all(e[k] < e[k + 1] for k in range(4))
+
+
+# ------------------------------------------------------------------ 8^3, 5 load steps
+# The reference's own par_mat (phys/ge/init.c:18) is not a case here: its
+# deviatoric energy has A37 < 0, so the undeformed state is a local maximum
+# (the wells sit at |e_dev| = 0.25) and a 1 % stretch from it has no nearby
+# solution to converge to; that material is checked on the assembly (R, K).
+@pytest.mark.parametrize("nquad,lift,form", [(3, False, "fp64"), (4, False, "fp64"), (4, True, "fp64"),
+                                             (4, False, "mixed")])
+def test_8cube_five_load_steps(oracle, nquad, lift, form):
+    from igap4standard_amd import loadsteps
+    say("8^3 load steps", nquad, lift, form)
+    pb = loadsteps.problem(8, 5, 0.01, nquad)
+    checks = []
+
+    def cb(pb, U, rec):
+        Un = U.cpu().numpy()
+        R = oracle.residual(pb, Un)
+        rows = oracle.dirichlet_rows(pb)
+        checks.append((rec, float(np.linalg.norm(R)), float(np.abs(R[rows]).max())))
+        say("  step", rec["step"], rec["newton_its"], rec["ksp_its_per_newton"], "|R| oracle %.2e" % checks[-1][1])
+
+    steps = loadsteps.run(steps=5, a=0.01, form=form, lift=lift, pb=pb, callback=cb)
+    assert len(steps) == 5
+    for rec, rn, rd in checks:
+        assert rec["converged"] == 1 and rec["fnorm"] < 1e-10
+        assert rn < 1e-10 and rd < 1e-12
+    e = [r["energy"] for r in steps]
+    assert all(e[k] < e[k + 1] for k in range(4))
+
+
+# ------------------------------------------------------------------ 48^3 getime
+def test_48cube_getime_100_steps(oracle):
+    import torch
+    from igap4standard_amd.transient import TimeStepper
+    pb = P.transient_problem(48, dt=1e-3)
+    v0 = P.initial_velocity(pb)
+    n = pb.ndof
+    U1, U0 = np.zeros(n), -1e-3 * v0
+    say("48^3 getime: FP32 tangent, 100 steps")
+    ts = TimeStepper(pb, mixed=True)
+    ts.set_state(U1, U0)
+    checked = {}
+    for k in range(1, 101):
+        hist = ts.Uh[n:].cpu().numpy() if k in (1, 50, 100) else None
+        info = ts.step()
+        assert info["converged"] == 1 and info["fnorm"] < 1e-10, (k, info)
+        if hist is not None:
+            torch.cuda.synchronize()
+            Uh = np.concatenate([ts.Uh[:n].cpu().numpy(), hist])
+            R = oracle.residual_mt(pb, Uh, workers())
+            checked[k] = float(np.linalg.norm(R))
+            say("  step", k, info["newton_its"], "Newton, |R| gpu %.2e oracle %.2e" % (info["fnorm"], checked[k]))
+    U32 = ts.Uh[:n].cpu().numpy()
+    ts.app.close()
+    assert all(v < 1e-10 for v in checked.values()), checked
+    say("48^3 getime: FP64 tangent, 100 steps")
+    ts = TimeStepper(pb, mixed=False)
+    ts.set_state(U1, U0)
+    for k in range(100):
+        info = ts.step()
+        assert info["converged"] == 1
+    U64 = ts.Uh[:n].cpu().numpy()
+    ts.app.close()
+    e = rel(U32, U64)
+    say("  |U32 - U64| / |U64| = %.2e, |U| = %.3e" % (e, np.linalg.norm(U64)))
+    assert np.linalg.norm(U64) > 0 and e < 1e-8
+
+
+# ------------------------------------------------------------------ 256 instances
+def test_256_instance_batch(oracle):
+    import torch
+    from igap4standard_amd.batch import Batch, batch_problems
+    pbs = batch_problems(256, seed=0x5EED0004)
+    say("batch: 256 instances, %d elements" % sum(p.n_elem for p in pbs))
+    B = Batch(pbs, L.IGAP4H_FP32, L.IGAP4H_VALS_F32)
+    B.assemble()
+    torch.cuda.synchronize()
+    maxdof = max(a.ndof for a in B.apps)
+    maxnnz = max(a.nnz for a in B.apps)
+    R1 = torch.empty(maxdof, device="cuda:0", dtype=torch.float64)
+    V1 = torch.empty(maxnnz, device="cuda:0", dtype=torch.float32)
+    for i, (a, U, R, V) in enumerate(zip(B.apps, B.U, B.R, B.vals)):
+        a.residual_dev(U.data_ptr(), R1.data_ptr())
+        a.tangent_dev(U.data_ptr(), V1.data_ptr(), L.IGAP4H_FP32, L.IGAP4H_VALS_F32)
+        assert torch.equal(R1[:a.ndof], R), i
+        assert torch.equal(V1[:a.nnz], V), i
+    say("  all 256 equal to the single-instance calls")
+    B.close()
+    small = [p for p in pbs if p.nelem[0] == 8]
+    say("batch: %d 8^3 instances in FP64 against the oracle" % len(small))
+    B = Batch(small, L.IGAP4H_FP64, L.IGAP4H_VALS_F64)
+    B.assemble()
+    torch.cuda.synchronize()
+    got = [(U.cpu().numpy(), R.cpu().numpy(), V.cpu().numpy()) for U, R, V in zip(B.U, B.R, B.vals)]
+    B.close()
+
+    def check(k):
+        pb = small[k]
+        Un, R, V = got[k]
+        R0 = oracle.residual(pb, Un)
+        _, _, v0 = oracle.tangent(pb, Un)
+        return rel(R, R0), rel(V, v0)
+
+    with concurrent.futures.ThreadPoolExecutor(workers()) as ex:
+        errs = list(ex.map(check, range(len(small))))
+    er, ek = max(e[0] for e in errs), max(e[1] for e in errs)
+    say("  worst |dR|/|R| = %.2e, |dK|/|K| = %.2e" % (er, ek))
+    assert len(small) > 0 and er < 1e-12 and ek < 1e-12
+
+
+# ------------------------------------------------------------------ 32^3 adjoint
+def test_32cube_adjoint_finite_differences(oracle):
+    from igap4standard_amd.adjoint import Calibration
+    pb, E, Et = P.adjoint_problem(32)
+    say("32^3 adjoint: forward solves")
+    cal = Calibration(pb)
+    Ut, _ = cal.solve(Et)
+    U, info = cal.solve(E)
+    Un = U.cpu().numpy()
+    assert np.linalg.norm(oracle.residual_mt(pb, Un, workers())) < 1e-10
+    loss, gE, _, its = cal.gradient(E, U, Ut)
+    say("  loss %.6e, adjoint %d GMRES iterations" % (loss, its))
+    rng = np.random.default_rng(0x5EED0007)
+    elems = rng.choice(pb.n_elem, size=64, replace=False)
+    v_adj = cal.vjp_E(U, cal.lam, E)[elems]
+    v_fd = cal.vjp_E_fd(U, cal.lam, E, elems, 1e-6)
+    e_vjp = rel(v_adj, v_fd)
+    say("  backward kernel vs central FD (64 elements, step 1e-6): %.2e" % e_vjp)
+    assert e_vjp < 1e-6
+    fd = []
+    for e in elems[:4]:
+        lv = []
+        for sgn in (1.0, -1.0):
+            Ep = E.copy()
+            Ep[e] += sgn * 1e-3
+            Up, _ = cal.solve(Ep, warm=U)
+            lv.append(cal.loss(Up, Ut))
+        fd.append((lv[0] - lv[1]) / 2e-3)
+    e_loss = rel(gE[elems[:4]], np.array(fd))
+    say("  dL/dE vs central FD of the loss (4 elements, step 1e-3): %.2e" % e_loss)
+    assert e_loss < 1e-6
