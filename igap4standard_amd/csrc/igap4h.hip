@@ -10,6 +10,7 @@
 // Connectivity is arithmetic (structured patch); no index arrays per element.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -25,19 +26,18 @@
 using namespace igap4h;
 
 // ============================================================== errors
-static thread_local std::string g_err;
+// a fixed buffer per thread: reporting an error allocates nothing
+static thread_local char g_err[512];
 int igap4h::fail(int code, const char *fmt, ...)
 {
-    char buf[512];
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
     va_end(ap);
-    g_err = buf;
     return code;
 }
 
-extern "C" const char *igap4h_last_error(void) { return g_err.c_str(); }
+extern "C" const char *igap4h_last_error(void) { return g_err; }
 
 // ============================================================== kernel args
 struct MeshArgs {
@@ -326,7 +326,8 @@ __device__ __forceinline__ Coef load_coef(const double *coef, long long e)
 // Writes R_e[81] to Re; k_gather sums it per dof.
 template <int PK>
 __device__ __forceinline__ void residual_elem_body(const MeshArgs &m, const double *__restrict__ Uh,
-                                                   const double *__restrict__ coef, double *__restrict__ Re, long long e)
+                                                   const double *__restrict__ coef, double *__restrict__ Re, long long e,
+                                                   long long slot)
 {
     constexpr int NH = PhysT<PK>::NH;
     __shared__ double sU[NH * 81];
@@ -437,7 +438,7 @@ __device__ __forceinline__ void residual_elem_body(const MeshArgs &m, const doub
                 const double *z = sphi + (8 + qz2) * 9 + az * 3;
                 t += z[0] * p[qz2] + z[1] * p[4 + qz2] + z[2] * p[8 + qz2];
             }
-            Re[e * 81 + 3 * A + i] = t;
+            Re[slot * 81 + 3 * A + i] = t;
         }
         __syncthreads();
     }
@@ -447,7 +448,7 @@ template <int PK>
 __global__ __launch_bounds__(64) void k_residual_elem(MeshArgs m, const double *__restrict__ Uh,
                                                       const double *__restrict__ coef, double *__restrict__ Re)
 {
-    residual_elem_body<PK>(m, Uh, coef, Re, (long long)blockIdx.x);
+    residual_elem_body<PK>(m, Uh, coef, Re, (long long)blockIdx.x, (long long)blockIdx.x);
 }
 
 // Sum of element vectors per dof in element order (the reference's ADD order),
@@ -483,6 +484,63 @@ __global__ __launch_bounds__(256) void k_gather(MeshArgs m, const double *__rest
     long long dof = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (dof >= m.ndof) return;
     gather_body(m, Re, fneu, dmask, gval, U0, R, dof);
+}
+
+// ---------------------------------------------------------------- residual without allocation
+// Element vectors of a chunk of consecutive elements [e0, e0 + n) staged in a
+// fixed device array (slot = e - e0).  Elements are in the reference order (x
+// slowest), so a chunk's elements all touch the x-planes of nodes
+// [ex(e0), ex(e0 + n - 1) + 2] and every dof sees its contributions chunk
+// after chunk in element order: adding them to the running R one at a time
+// repeats k_gather's additions exactly.
+constexpr long long kScratchElems = 32768;            // element vectors per chunk
+static __device__ __align__(256) double scratch[kScratchElems * 81];   // 21.2 MB per device
+
+template <int PK>
+__global__ __launch_bounds__(64) void k_residual_chunk(MeshArgs m, const double *__restrict__ Uh,
+                                                       const double *__restrict__ coef, double *__restrict__ Re,
+                                                       long long e0)
+{
+    residual_elem_body<PK>(m, Uh, coef, Re, e0 + blockIdx.x, (long long)blockIdx.x);
+}
+
+// R[dof] += contributions of the chunk's elements, dof in [d0, d1)
+__global__ __launch_bounds__(256) void k_gather_chunk(MeshArgs m, const double *__restrict__ Re, long long e0, long long e1,
+                                                      long long d0, long long d1, double *__restrict__ R)
+{
+    long long dof = d0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (dof >= d1) return;
+    long long node = dof / 3;
+    int i = (int)(dof % 3);
+    int a2 = (int)(node % m.nb[2]);
+    int a1 = (int)((node / m.nb[2]) % m.nb[1]);
+    int a0 = (int)(node / ((long long)m.nb[2] * m.nb[1]));
+    double s = R[dof];
+    bool any = false;
+    for (int ex = max(0, a0 - 2); ex <= min(a0, m.ne[0] - 1); ex++)
+        for (int ey = max(0, a1 - 2); ey <= min(a1, m.ne[1] - 1); ey++)
+            for (int ez = max(0, a2 - 2); ez <= min(a2, m.ne[2] - 1); ez++) {
+                long long e = ((long long)ex * m.ne[1] + ey) * m.ne[2] + ez;
+                if (e < e0 || e >= e1) continue;
+                int A = (a0 - ex) * 9 + (a1 - ey) * 3 + (a2 - ez);
+                s += Re[(e - e0) * 81 + 3 * A + i];
+                any = true;
+            }
+    if (any) R[dof] = s;
+}
+
+// Neumann load, then Dirichlet rows R = U - g (as gather_body ends)
+__global__ __launch_bounds__(256) void k_gather_finish(MeshArgs m, const double *__restrict__ fneu,
+                                                       const unsigned char *__restrict__ dmask,
+                                                       const double *__restrict__ gval, const double *__restrict__ U0,
+                                                       double *__restrict__ R)
+{
+    long long dof = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (dof >= m.ndof) return;
+    double s = R[dof];
+    if (fneu) s += fneu[dof];
+    if (dmask[dof]) s = U0[dof] - gval[dof];
+    R[dof] = s;
 }
 
 // ============================================================== tangent
@@ -1280,6 +1338,25 @@ static void ctx_free(igap4h_ctx *c)
 
 extern "C" void igap4h_destroy(igap4h_ctx *ctx) { ctx_free(ctx); }
 
+// Device address of `scratch` on each device, looked up once (with that
+// device current) by igap4h_create; the array's name is not a device address
+// in host code.
+constexpr int kMaxDevices = 64;
+static double *g_scratch[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_init(int dev)
+{
+    if (dev < 0 || dev >= kMaxDevices) return fail(IGAP4H_ENODEV, "device %d beyond the %d this library tracks", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[dev]) return 0;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(IGAP4H_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch[dev] = (double *)p;
+    return 0;
+}
+
 static int check_device(int *dev)
 {
     int n = 0;
@@ -1385,6 +1462,7 @@ extern "C" int igap4h_create(const IGA *app, const igap4h_opts *opts, igap4h_ctx
     c->U_hist_app = app->U_hist;
     c->neumann = o.neumann;
     int rc = check_device(&c->device);
+    if (!rc) rc = scratch_init(c->device);
     if (rc) {
         delete c;
         return rc;
@@ -1576,6 +1654,38 @@ extern "C" int igap4h_residual_dev(igap4h_ctx *c, const double *d_Uh, double *d_
     long long nd = m.ndof;
     hipLaunchKernelGGL(k_gather, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, st, m, (const double *)c->d_Re,
                        (const double *)c->d_fneu, (const unsigned char *)c->d_mask, (const double *)c->d_g, d_Uh, d_R);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int igap4h_residual_dev_noalloc(igap4h_ctx *c, const double *d_Uh, double *d_R, void *stream)
+{
+    if (!c || !d_Uh || !d_R) return fail(IGAP4H_EINVAL, "NULL argument");
+    if (c->device < 0 || c->device >= kMaxDevices || !g_scratch[c->device])
+        return fail(IGAP4H_EINVAL, "no scratch address for device %d (context not made by igap4h_create)", c->device);
+    double *Re = g_scratch[c->device];
+    hipStream_t st = (hipStream_t)stream;
+    MeshArgs m = mesh_args(c);
+    const long long ne = c->mesh.nelem(), nd = m.ndof;
+    const long long plane = 3LL * m.nb[1] * m.nb[2];  // dofs per x-plane of nodes
+    const long long eplane = (long long)m.ne[1] * m.ne[2];
+    hipLaunchKernelGGL(k_fill_zero, dim3(2048), dim3(256), 0, st, (unsigned int *)d_R, 2 * nd);
+    HIPCHK(hipGetLastError());
+    for (long long e0 = 0; e0 < ne; e0 += kScratchElems) {
+        const long long e1 = std::min(ne, e0 + kScratchElems);
+        if (c->physics == IGAP4H_PHYS_GE)
+            hipLaunchKernelGGL(k_residual_chunk<0>, dim3((unsigned)(e1 - e0)), dim3(64), 0, st, m, d_Uh, (const double *)c->d_coef, Re, e0);
+        else
+            hipLaunchKernelGGL(k_residual_chunk<1>, dim3((unsigned)(e1 - e0)), dim3(64), 0, st, m, d_Uh, (const double *)c->d_coef, Re, e0);
+        HIPCHK(hipGetLastError());
+        // dofs of the node planes [ex(e0), ex(e1 - 1) + 2]
+        const long long d0 = (e0 / eplane) * plane, d1 = std::min(nd, ((e1 - 1) / eplane + 3) * plane);
+        hipLaunchKernelGGL(k_gather_chunk, dim3((unsigned)((d1 - d0 + 255) / 256)), dim3(256), 0, st, m, (const double *)Re, e0, e1,
+                           d0, d1, d_R);
+        HIPCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_gather_finish, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, st, m, (const double *)c->d_fneu,
+                       (const unsigned char *)c->d_mask, (const double *)c->d_g, d_Uh, d_R);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1778,7 +1888,8 @@ __global__ __launch_bounds__(64) void k_residual_batch(const BatchInst *__restri
     const int i = batch_find(eoff, n, blockIdx.x);
     const BatchInst &b = I[i];
     const MeshArgs &m = b.m;
-    residual_elem_body<0>(m, b.Uh, b.coef, b.Re, (long long)blockIdx.x - eoff[i]);
+    const long long e = (long long)blockIdx.x - eoff[i];
+    residual_elem_body<0>(m, b.Uh, b.coef, b.Re, e, e);
 }
 
 // blockIdx.y = instance; grid-stride over its dofs

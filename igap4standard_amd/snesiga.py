@@ -171,6 +171,12 @@ class App:
         L.check(L.lib().igap4h_residual_dev(self.ctx, ctypes.c_void_p(d_Uh), ctypes.c_void_p(d_R),
                                             ctypes.c_void_p(stream or 0)))
 
+    def residual_dev_noalloc(self, d_Uh, d_R, stream=None):
+        """residual_dev staged through the library's fixed device array (no
+        allocation)."""
+        L.check(L.lib().igap4h_residual_dev_noalloc(self.ctx, ctypes.c_void_p(d_Uh), ctypes.c_void_p(d_R),
+                                                    ctypes.c_void_p(stream or 0)))
+
     def tangent_dev(self, d_Uh, d_vals, prec=L.IGAP4H_FP64, vals_type=L.IGAP4H_VALS_F64, stream=None):
         L.check(L.lib().igap4h_tangent_dev(self.ctx, ctypes.c_void_p(d_Uh), prec, vals_type,
                                            ctypes.c_void_p(d_vals), ctypes.c_void_p(stream or 0)))

@@ -200,6 +200,19 @@ int igap4h_residual_dev(igap4h_ctx *ctx, const double *d_U_hist, double *d_R, vo
 int igap4h_tangent_dev(igap4h_ctx *ctx, const double *d_U_hist, int prec, int vals_type, void *d_vals,
                        void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): igap4h_residual_dev without the context's n_elem x 81
+ * element-vector buffer.  Element vectors are staged in a fixed device array
+ * of 32,768 element vectors (21.2 MB per device, held until the process
+ * ends; its address is looked up once per device by igap4h_create); larger
+ * meshes are taken in chunks of that many elements, so no mesh size is
+ * refused.  d_R is zeroed and accumulated in stream order (no memset node).
+ * The result equals igap4h_residual_dev bit for bit.  Allocates nothing, on
+ * the host or the device; the array is shared by every context on a device,
+ * so calls that overlap on one device (concurrent streams or threads) race. */
+int igap4h_residual_dev_noalloc(igap4h_ctx *ctx, const double *d_U_hist, double *d_R, void *stream);
+
 /* ---- material coefficients ------------------------------------------- */
 /* Replace the per-element coefficients (host array n_elem x 6).  The device
  * is synchronized first; a lagged tangent of the Newton driver is discarded. */
