@@ -1182,17 +1182,21 @@ __global__ void k_cols(MeshArgs m, const long long *__restrict__ rowptr, int *__
         for (int j = 0; j < 3; j++) col[pos++] = (int)(3 * node_of(m, b0, b1, b2) + j);
 }
 
-// ============================================================== energy
-// ge: sum_q w W(u) per element; getime (AV = 1): W at (u0 + u1) / 2 from
-// history slices 2 and 1 (phys/getime/density.c:6-50).
+// ============================================================== density outputs
+// iga_intdensity (src/iga_intdensity.c:10-293), volume part: per element the
+// weighted sum over its Gauss points of the physics' density outputs (13 for
+// phys/ge, 2 for phys/getime), then fixed-order sums of consecutive element
+// ranges (k_density_blocks) whose partials the host adds in order.  ge reads
+// slice 0 of U; getime reads slices 1 (u1 = t_n) and 2 (u0 = t_{n-1}).
 template <int AV>
-__global__ __launch_bounds__(64) void k_energy_elem(MeshArgs m, const double *__restrict__ U,
-                                                    const double *__restrict__ coef, double *__restrict__ Pe)
+__global__ __launch_bounds__(64) void k_density_elem(MeshArgs m, const double *__restrict__ U,
+                                                     const double *__restrict__ coef, double *__restrict__ Pe)
 {
+    constexpr int ND = AV ? 2 : 13;
     constexpr int NS = AV ? 2 : 1;
     __shared__ double sU[NS * 81];
     __shared__ double sphi[3 * 4 * 9];
-    __shared__ double sred[64];
+    __shared__ double sred[ND][64];
     const int tid = threadIdx.x;
     const long long e = blockIdx.x;
     int ev[3];
@@ -1206,22 +1210,63 @@ __global__ __launch_bounds__(64) void k_energy_elem(MeshArgs m, const double *__
     fill_phi(m, ev, sphi, tid, 64);
     __syncthreads();
     const int nqp = m.nq * m.nq * m.nq;
-    double val = 0.0;
+    double hh[ND];
+    for (int k = 0; k < ND; k++) hh[k] = 0.0;
     if (tid < nqp) {
         int qx = tid / (m.nq * m.nq), qy = (tid / m.nq) % m.nq, qz = tid % m.nq;
         double u[NS][30], w;
         interp_slots<NS>(m, sU, sphi, ev, qx, qy, qz, u, &w);
-        if (AV)
-            for (int s = 0; s < 30; s++) u[0][s] = 0.5 * (u[0][s] + u[NS - 1][s]);
-        val = w * energy_of<double>(u[0], load_coef(coef, e));
+        const Coef c = load_coef(coef, e);
+        if constexpr (AV) {
+            getime_density_outputs(u[0], u[NS - 1], c, m.rho, m.dt, hh);
+        } else {
+            ge_density_outputs(u[0], c, hh);
+        }
+        for (int k = 0; k < ND; k++) hh[k] *= w;
     }
-    sred[tid] = val;
+    for (int k = 0; k < ND; k++) sred[k][tid] = hh[k];
     __syncthreads();
-    if (tid == 0) {
+    if (tid < ND) {
         double s = 0.0;
-        for (int l = 0; l < nqp; l++) s += sred[l];
-        Pe[e] = s;
+        for (int l = 0; l < nqp; l++) s += sred[tid][l];
+        Pe[e * ND + tid] = s;
     }
+}
+
+// part[b * nd + k] = sum of Pe[e * nd + k] over the elements of block b's
+// range [b ne / nblk, (b + 1) ne / nblk), in element order
+__global__ __launch_bounds__(64) void k_density_blocks(const double *__restrict__ Pe, long long ne, int nd,
+                                                       double *__restrict__ part)
+{
+    const long long b = blockIdx.x, nblk = gridDim.x;
+    const long long e0 = b * ne / nblk, e1 = (b + 1) * ne / nblk;
+    for (int k = threadIdx.x; k < nd; k += blockDim.x) {
+        double s = 0.0;
+        for (long long e = e0; e < e1; e++) s += Pe[e * nd + k];
+        part[b * nd + k] = s;
+    }
+}
+
+// Neumann boundary work (src/iga_intdensity.c:196-287 with densityb): the
+// traction integral of the face values is -fneu . U (fneu = -int t N, the
+// Neumann load of the residual), U = slice 0 (ge) or (slice 1 + slice 2) / 2
+// (getime, phys/getime/density.c: densityb).  Fixed-order block partials.
+__global__ __launch_bounds__(256) void k_neumann_work(const double *__restrict__ fneu, const double *__restrict__ U,
+                                                      long long ndof, int avg, double *__restrict__ part)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < ndof; i += (long long)gridDim.x * 256) {
+        double u = avg ? 0.5 * (U[ndof + i] + U[2 * ndof + i]) : U[i];
+        s += fneu[i] * u;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
 // ============================================================== adjoint (ge)
@@ -1331,6 +1376,7 @@ static void ctx_free(igap4h_ctx *c)
     (void)hipFree(c->d_R);
     (void)hipFree(c->d_vals);
     (void)hipFree(c->d_Pe);
+    (void)hipFree(c->d_part);
     solver_free(c);
     if (cur >= 0) (void)hipSetDevice(cur);
     delete c;
@@ -1473,6 +1519,7 @@ extern "C" int igap4h_create(const IGA *app, const igap4h_opts *opts, igap4h_ctx
     std::vector<double> g;
     dirichlet_values(m, mask, g);
     std::vector<double> fneu = neumann_load(m, nquad, o.neumann);
+    c->neu_measure = neumann_measure(m, nquad);
 
 #define ALLOC(p, n)                   \
     do {                              \
@@ -1604,6 +1651,7 @@ extern "C" int igap4h_update_bc(igap4h_ctx *c, const BC *bc)
     std::vector<double> g;
     dirichlet_values(m, mask, g);
     std::vector<double> fneu = neumann_load(m, c->nquad, c->neumann);
+    c->neu_measure = neumann_measure(m, c->nquad);
     if (rows.size() != c->drows.size()) {
         (void)hipFree(c->d_drows);
         c->d_drows = nullptr;
@@ -1825,28 +1873,76 @@ extern "C" int igap4h_residual_vjp_coef_dev(igap4h_ctx *c, const double *d_U, co
     return 0;
 }
 
-extern "C" int igap4h_energy_dev(igap4h_ctx *c, const double *d_U, double *energy, void *stream)
+constexpr int kDensityBlocks = 1024, kWorkBlocks = 256;
+
+// HH[0..nd) = volume integrals of the density outputs; with boundary, minus
+// the Neumann boundary terms (densityb) as src/iga_intdensity.c:196-287.
+static int density_integrals(igap4h_ctx *c, const double *d_U, bool boundary, double *HH, int *ndout, void *stream)
 {
-    if (!c || !d_U || !energy) return fail(IGAP4H_EINVAL, "NULL argument");
-    long long ne = c->mesh.nelem();
-    if (!c->d_Pe) {
-        int r = dev_alloc(&c->d_Pe, (size_t)ne);
+    HIPCHK(hipSetDevice(c->device));
+    const long long ne = c->mesh.nelem();
+    const int nd = c->physics == IGAP4H_PHYS_GE ? 13 : 2;
+    if (c->d_Pe_n < (size_t)ne * nd) {
+        (void)hipFree(c->d_Pe);
+    (void)hipFree(c->d_part);
+        c->d_Pe = nullptr;
+        c->d_Pe_n = 0;
+        int r = dev_alloc(&c->d_Pe, (size_t)ne * nd);
+        if (r) return r;
+        c->d_Pe_n = (size_t)ne * nd;
+    }
+    if (!c->d_part) {
+        int r = dev_alloc(&c->d_part, (size_t)kDensityBlocks * 13 + kWorkBlocks);
         if (r) return r;
     }
     hipStream_t st = (hipStream_t)stream;
     MeshArgs m = mesh_args(c);
     if (c->physics == IGAP4H_PHYS_GE)
-        hipLaunchKernelGGL(k_energy_elem<0>, dim3((unsigned)ne), dim3(64), 0, st, m, d_U, (const double *)c->d_coef, c->d_Pe);
+        hipLaunchKernelGGL(k_density_elem<0>, dim3((unsigned)ne), dim3(64), 0, st, m, d_U, (const double *)c->d_coef, c->d_Pe);
     else
-        hipLaunchKernelGGL(k_energy_elem<1>, dim3((unsigned)ne), dim3(64), 0, st, m, d_U, (const double *)c->d_coef, c->d_Pe);
+        hipLaunchKernelGGL(k_density_elem<1>, dim3((unsigned)ne), dim3(64), 0, st, m, d_U, (const double *)c->d_coef, c->d_Pe);
     HIPCHK(hipGetLastError());
-    std::vector<double> h((size_t)ne);
-    HIPCHK(hipMemcpyAsync(h.data(), c->d_Pe, (size_t)ne * sizeof(double), hipMemcpyDeviceToHost, st));
+    const int nblk = (int)std::min<long long>(kDensityBlocks, ne);
+    hipLaunchKernelGGL(k_density_blocks, dim3(nblk), dim3(64), 0, st, (const double *)c->d_Pe, ne, nd, c->d_part);
+    HIPCHK(hipGetLastError());
+    const bool work = boundary && c->d_fneu;
+    if (work) {
+        hipLaunchKernelGGL(k_neumann_work, dim3(kWorkBlocks), dim3(256), 0, st, (const double *)c->d_fneu, d_U, (long long)m.ndof,
+                           c->physics == IGAP4H_PHYS_GETIME ? 1 : 0, c->d_part + (size_t)kDensityBlocks * 13);
+        HIPCHK(hipGetLastError());
+    }
+    std::vector<double> h((size_t)kDensityBlocks * 13 + kWorkBlocks);
+    HIPCHK(hipMemcpyAsync(h.data(), c->d_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    double s = 0.0;
-    for (double v : h) s += v;
-    *energy = s;
+    for (int k = 0; k < nd; k++) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; b++) s += h[(size_t)b * nd + k];
+        HH[k] = s;
+    }
+    if (work) {
+        double s = 0.0;
+        for (int b = 0; b < kWorkBlocks; b++) s += h[(size_t)kDensityBlocks * 13 + b];
+        HH[0] += s;  // minus the traction work: fneu = -int t N
+    }
+    if (boundary && c->physics == IGAP4H_PHYS_GETIME) HH[1] -= c->neu_measure;  // densityb hh[1] = 1
+    if (ndout) *ndout = nd;
     return 0;
+}
+
+extern "C" int igap4h_energy_dev(igap4h_ctx *c, const double *d_U, double *energy, void *stream)
+{
+    if (!c || !d_U || !energy) return fail(IGAP4H_EINVAL, "NULL argument");
+    double HH[13];
+    int rc = density_integrals(c, d_U, false, HH, nullptr, stream);
+    if (rc) return rc;
+    *energy = HH[0];
+    return 0;
+}
+
+extern "C" int igap4h_intdensity_dev(igap4h_ctx *c, const double *d_U_hist, double *HH, int *ndensity, void *stream)
+{
+    if (!c || !d_U_hist || !HH) return fail(IGAP4H_EINVAL, "NULL argument");
+    return density_integrals(c, d_U_hist, true, HH, ndensity, stream);
 }
 
 // ============================================================== batched assembly

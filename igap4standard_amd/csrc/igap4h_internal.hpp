@@ -45,7 +45,10 @@ struct igap4h_ctx {
     double *d_R = nullptr;
     void *d_vals = nullptr;      // staging for the host API
     size_t d_vals_bytes = 0;
-    double *d_Pe = nullptr;
+    double *d_Pe = nullptr;      // per-element energies / density outputs
+    size_t d_Pe_n = 0;
+    double *d_part = nullptr;    // block partial sums of the density reductions
+    double neu_measure = 0.0;    // neumann_measure() of the current boundary data
     double *U_hist_app = nullptr;
     igap4h_neumann_fn neumann = nullptr;  // traction hook given at create
     igap4h::Solver *solver = nullptr;  // Krylov work space, created on first use

@@ -280,6 +280,52 @@ IGAP4H_HD T energy_of(const T *u, const Coef &c)
     return W;
 }
 
+// The 13 density outputs of phys/ge (phys/ge/density.c:6-66) at one point:
+// hh[0] = W; hh[1..6] = 2 dW/dE00, 2 dW/dE01, 2 dW/dE02, 2 dW/dE11,
+// 2 dW/dE12, 2 dW/dE22 with W written on the six independent strains (the
+// off-diagonal ones once), i.e. 2 S00, 4 S01, 4 S02, 2 S11, 4 S12, 2 S22;
+// hh[7..9] the variant indicators of (e2, e3) (sectors around the wells);
+// hh[10..12] the same sectors where W < -1/2.
+IGAP4H_HD void ge_density_outputs(const double *u, const Coef &c, double hh[13])
+{
+    Strain<double> st;
+    strain_of<double>(u, c, st);
+    double S[6];
+    stress_of<double>(st, c, S);
+    const double W = energy_of<double>(u, c);
+    hh[0] = W;
+    hh[1] = 2.0 * S[0];
+    hh[2] = 4.0 * S[3];
+    hh[3] = 4.0 * S[4];
+    hh[4] = 2.0 * S[1];
+    hh[5] = 4.0 * S[5];
+    hh[6] = 2.0 * S[2];
+    const double e2 = st.h2, e3 = st.h3;
+    hh[7] = (e2 > 0 && e3 > -0.5 * e2) ? 1.0 : 0.0;
+    hh[8] = (e2 < 0 && e3 > 0.5 * e2) ? 1.0 : 0.0;
+    hh[9] = (e3 < 0.5 * e2 && e3 < -0.5 * e2) ? 1.0 : 0.0;
+    hh[10] = (W < -0.5 && e2 > 0 && e3 > 0) ? 1.0 : 0.0;
+    hh[11] = (W < -0.5 && e2 < 0 && e3 > 0) ? 1.0 : 0.0;
+    hh[12] = (W < -0.5 && e3 < 0) ? 1.0 : 0.0;
+}
+
+// The 2 density outputs of phys/getime (phys/getime/density.c:6-53) from the
+// slots of t_n (u1) and t_{n-1} (u0): hh[0] = W((u0 + u1) / 2) + rho
+// |u1 - u0|^2 / (2 dt^2) (value slots), hh[1] = 0.
+IGAP4H_HD void getime_density_outputs(const double *u1, const double *u0, const Coef &c, double rho, double dt,
+                                             double hh[2])
+{
+    double ub[30];
+    for (int s = 0; s < 30; s++) ub[s] = 0.5 * (u0[s] + u1[s]);
+    double kin = 0.0;
+    for (int i = 0; i < 3; i++) {
+        const double d = u1[10 * i] - u0[10 * i];
+        kin += rho * d * d / (2.0 * dt * dt);
+    }
+    hh[0] = energy_of<double>(ub, c) + kin;
+    hh[1] = 0.0;
+}
+
 // The six energies W is linear in: W = sum_k coef_k phi_k with coef =
 // (A36, A37, A38, e0, e345, le^2) and
 // phi = (h3 (h3^2 - 3 h2^2), q, q^2, h1^2, E01^2 + E02^2 + E12^2, |dev g|^2).

@@ -256,6 +256,28 @@ std::vector<double> neumann_load(const Mesh &m, int nquad, igap4h_neumann_fn hoo
     return R;
 }
 
+double neumann_measure(const Mesh &m, int nquad)
+{
+    double cq[4], wq[4], total = 0.0;
+    gauss01(nquad, cq, wq);
+    for (int ib = 0; ib < m.nboun; ib++) {
+        int f = m.boun[ib];
+        FaceMap fm = face_map(f, m.nb);
+        int d0 = fm.dim[0], d1 = fm.dim[1];
+        const double *k0 = m.knot[d0].data(), *k1 = m.knot[d1].data();
+        for (int order = 0; order < 2; order++)
+            for (int i = 0; i < 3; i++) {
+                if (m.bc_type[6 * f + 3 * order + i] != 1) continue;
+                for (int e0 = 0; e0 < m.ne[d0]; e0++)
+                    for (int e1 = 0; e1 < m.ne[d1]; e1++)
+                        for (int qx = 0; qx < nquad; qx++)
+                            for (int qy = 0; qy < nquad; qy++)
+                                total += wq[qx] * wq[qy] * (k0[e0 + P + 1] - k0[e0 + P]) * (k1[e1 + P + 1] - k1[e1 + P]);
+            }
+    }
+    return total;
+}
+
 void coef_from_par(const double *par, double coef[6])
 {
     double r = par[0], c = par[1];

@@ -44,6 +44,12 @@ void dirichlet_values(const Mesh &m, std::vector<unsigned char> &mask, std::vect
 // no face carries a Neumann flag.
 std::vector<double> neumann_load(const Mesh &m, int nquad, igap4h_neumann_fn hook);
 
+// Sum over the Neumann flags (face in boun, order, dof with bc_type 1) of
+// the face's quadrature weights: the boundary part of the getime density
+// output hh[1] (phys/getime/density.c, densityb hh[1] = 1), integrated as
+// src/iga_intdensity.c:196-287 does.
+double neumann_measure(const Mesh &m, int nquad);
+
 // (A36, A37, A38, e0, e345, le^2) from par_mat (phys/ge/residual.c:45-47).
 void coef_from_par(const double *par, double coef[6]);
 

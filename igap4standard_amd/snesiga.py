@@ -222,6 +222,15 @@ class App:
         L.check(L.lib().igap4h_residual_vjp_coef_dev(self.ctx, ctypes.c_void_p(d_U), ctypes.c_void_p(d_lam),
                                                      ctypes.c_void_p(d_g), ctypes.c_void_p(stream or 0)))
 
+    def intdensity_dev(self, d_Uh, stream=None):
+        """iga_intdensity: the ndensity integrated density outputs (13 for
+        phys/ge, 2 for phys/getime), Neumann boundary terms included."""
+        HH = (ctypes.c_double * 13)()
+        nd = ctypes.c_int()
+        L.check(L.lib().igap4h_intdensity_dev(self.ctx, ctypes.c_void_p(d_Uh), HH, ctypes.byref(nd),
+                                              ctypes.c_void_p(stream or 0)))
+        return np.array(HH[:nd.value])
+
     def energy_dev(self, d_U, stream=None):
         e = ctypes.c_double()
         L.check(L.lib().igap4h_energy_dev(self.ctx, ctypes.c_void_p(d_U), ctypes.byref(e),

@@ -218,11 +218,24 @@ int igap4h_residual_dev_noalloc(igap4h_ctx *ctx, const double *d_U_hist, double 
  * is synchronized first; a lagged tangent of the Newton driver is discarded. */
 int igap4h_set_coef(igap4h_ctx *ctx, const double *elem_coef);
 
-/* ---- energy ----------------------------------------------------------- */
-/* Volume energy Pi = sum_e sum_q w W (src/iga_intdensity.c:10-293, volume
- * part, density hh[0]).  ge: W(u), d_U = device ndof (slice 0).  getime:
- * W((u0 + u1) / 2) (phys/getime/density.c:6-50), d_U = device U_hist with
- * its three slices (u1 = slice 1, u0 = slice 2).                          */
+/* ---- energy and density outputs -------------------------------------- */
+/* iga_intdensity (src/iga_intdensity.c:10-293) on one rank: HH[k] = volume
+ * integral of the physics' density output k minus its Neumann boundary term
+ * (densityb), for the ndensity outputs of the physics (Phys.ndensity of
+ * phys/ge/init.c:14 and phys/getime/init.c: 13 for phys/ge, 2 for
+ * phys/getime; HH must hold 13 doubles; *ndensity, if not NULL, gets the
+ * count).  phys/ge (phys/ge/density.c:6-84): hh[0] = W, hh[1..6] the
+ * stress-like outputs 2 dW/dE00, 4 S01, 4 S02, 2 dW/dE11, 4 S12, 2 dW/dE22,
+ * hh[7..12] the variant indicators; boundary: minus the traction work of
+ * slice 0.  phys/getime (phys/getime/density.c:6-62): hh[0] = W((u0 + u1)/2)
+ * + rho |u1 - u0|^2 / (2 dt^2), hh[1] = 0; boundary: minus the traction work
+ * of (u0 + u1) / 2, and hh[1] minus the measure of the Neumann faces (one per
+ * flagged face, order and dof).  Tractions come from the hook given at
+ * create (zero if none), as the residual's Neumann load.  The context's
+ * nquad is used (the reference hard-codes 4).  d_U_hist: device U_hist with
+ * its torder + 1 slices.  Fixed-order sums: repeatable on one device.      */
+int igap4h_intdensity_dev(igap4h_ctx *ctx, const double *d_U_hist, double *HH, int *ndensity, void *stream);
+/* Volume part of output 0 alone (the energy Pi without boundary work).   */
 int igap4h_energy_dev(igap4h_ctx *ctx, const double *d_U, double *energy, void *stream);
 
 /* ---- host utilities (no device needed) --------------------------------- */

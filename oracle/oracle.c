@@ -29,6 +29,10 @@
  *                       (same element matrices, other scatter)
  *   or_residual_mt      or_residual on the same input (order of the sums
  *                       only)
+ *   or_ge_density,      reference ge_density / getime_density, all outputs
+ *   or_getime_density   (golden vectors from oracle/_ref, value slots set)
+ *   or_intdensity       dHH[0]/dU = R off the Dirichlet rows (FD, Neumann
+ *                       hook on); getime HH[1] = -(flags x face area)
  */
 #include "oracle.h"
 #include <math.h>
@@ -434,6 +438,59 @@ double or_ge_energy(const double u[30], const double coef[6])
     ge_out *o = ge_scratch();
     ge_derivs(uj, coef, 0, 0, 0, o);
     return o->W.c[0];
+}
+
+/* phys/ge/density.c:6-66, all 13 outputs: W; twice the derivatives of W in
+ * the six strains (the off-diagonal ones appear once in W); the variant
+ * indicators of (e2, e3) and the energy-based ones. */
+void or_ge_density(double hh[13], const double u[30], const double coef[6])
+{
+    double F[3][3], E[3][3];
+    for (int a = 0; a < 3; a++)
+        for (int J = 0; J < 3; J++) F[a][J] = (a == J ? 1.0 : 0.0) + u[10 * a + 1 + J];
+    for (int I = 0; I < 3; I++)
+        for (int J = 0; J < 3; J++) {
+            double t = 0.0;
+            for (int a = 0; a < 3; a++) t += F[a][I] * F[a][J];
+            E[I][J] = 0.5 * (t - (I == J ? 1.0 : 0.0));
+        }
+    const double s3 = sqrt(3.0), s2 = sqrt(2.0), s6 = sqrt(6.0);
+    double e1 = (E[0][0] + E[1][1] + E[2][2]) / s3;
+    double e2 = (E[0][0] - E[1][1]) / s2;
+    double e3 = (E[0][0] + E[1][1] - 2.0 * E[2][2]) / s6;
+    double q = e2 * e2 + e3 * e3;
+    double A36 = coef[0], A37 = coef[1], A38 = coef[2], e0 = coef[3], e345 = coef[4];
+    double W1 = 2.0 * e0 * e1;
+    double W2 = -6.0 * A36 * e2 * e3 + 2.0 * A37 * e2 + 4.0 * A38 * q * e2;
+    double W3 = 3.0 * A36 * (e3 * e3 - e2 * e2) + 2.0 * A37 * e3 + 4.0 * A38 * q * e3;
+    double W = or_ge_energy(u, coef);
+    hh[0] = W;
+    hh[1] = 2.0 * (W1 / s3 + W2 / s2 + W3 / s6);
+    hh[2] = 4.0 * e345 * E[0][1];
+    hh[3] = 4.0 * e345 * E[0][2];
+    hh[4] = 2.0 * (W1 / s3 - W2 / s2 + W3 / s6);
+    hh[5] = 4.0 * e345 * E[1][2];
+    hh[6] = 2.0 * (W1 / s3 - 2.0 * W3 / s6);
+    hh[7] = (double)(e2 > 0 && e3 > -0.5 * e2);
+    hh[8] = (double)(e2 < 0 && e3 > 0.5 * e2);
+    hh[9] = (double)(e3 < 0.5 * e2 && e3 < -0.5 * e2);
+    hh[10] = (double)(W < -0.5 && e2 > 0 && e3 > 0);
+    hh[11] = (double)(W < -0.5 && e2 < 0 && e3 > 0);
+    hh[12] = (double)(W < -0.5 && e3 < 0);
+}
+
+/* phys/getime/density.c:6-53: W at (u0 + u1) / 2 plus the kinetic energy
+ * rho |u1 - u0|^2 / (2 dt^2) of the value slots; hh[1] = 0. */
+void or_getime_density(double hh[2], const double u[90], const double coef[6], const double tp[3])
+{
+    double ub[30], kin = 0.0;
+    for (int k = 0; k < 30; k++) ub[k] = 0.5 * (u[30 + k] + u[60 + k]);
+    for (int i = 0; i < 3; i++) {
+        double d = u[30 + 10 * i] - u[60 + 10 * i];
+        kin += tp[0] * d * d / (2.0 * tp[2] * tp[2]);
+    }
+    hh[0] = or_ge_energy(ub, coef) + kin;
+    hh[1] = 0.0;
 }
 
 void or_ge_residual(double r[30], const double u[30], const double coef[6])
@@ -1135,14 +1192,100 @@ double or_energy(const or_problem *pb, const double *U_hist)
                     elem_qp(pb, &c, ex, ey, ez, qx, qy, qz, N, xq, &w);
                     interp_u(pb, &c, U_hist, ex, ey, ez, N, xq, u);
                     if (pb->physics == 1) {
-                        /* phys/getime/density.c:6-50: W at (u0 + u1) / 2 (slices 2 and 1) */
-                        double ub[30];
-                        for (int k = 0; k < 30; k++) ub[k] = 0.5 * (u[30 + k] + u[60 + k]);
-                        Pi += w * or_ge_energy(ub, coef);
+                        /* phys/getime/density.c:6-53: W at (u0 + u1) / 2 plus kinetic energy */
+                        double hh[2];
+                        or_getime_density(hh, u, coef, c.tp);
+                        Pi += w * hh[0];
                     } else {
                         Pi += w * or_ge_energy(u, coef);
                     }
                 }
     }
     return Pi;
+}
+
+/* iga_intdensity (src/iga_intdensity.c:10-293) on one rank: the volume
+ * integral of every density output, then for each face in boun and each dof
+ * with a Neumann flag (standard: face values; high order: the normal
+ * derivative through the two layers) the face integral of densityb
+ * subtracted.  densityb: ge (phys/ge/density.c:68-84) hh[0] = u t, the rest
+ * 0; getime (phys/getime/density.c:57-62) hh[0] = (u1 + u0) / 2 t, hh[1] = 1.
+ * Returns ndensity (13 ge, 2 getime). */
+int or_intdensity(const or_problem *pb, const double *U_hist, double *HH)
+{
+    asm_ctx c;
+    if (asm_init(pb, &c)) return -1;
+    const int nd = pb->physics == 1 ? 2 : 13;
+    int nq = pb->nquad;
+    double N[270], xq[3], w, u[90], hh[13];
+    for (int k = 0; k < nd; k++) HH[k] = 0.0;
+    int64_t ne = (int64_t)pb->nelem[0] * pb->nelem[1] * pb->nelem[2];
+    for (int64_t e = 0; e < ne; e++) {
+        int ex, ey, ez;
+        elem_ijk(pb, e, &ex, &ey, &ez);
+        const double *coef = elem_coef(pb, &c, e);
+        for (int qx = 0; qx < nq; qx++)
+            for (int qy = 0; qy < nq; qy++)
+                for (int qz = 0; qz < nq; qz++) {
+                    elem_qp(pb, &c, ex, ey, ez, qx, qy, qz, N, xq, &w);
+                    interp_u(pb, &c, U_hist, ex, ey, ez, N, xq, u);
+                    if (pb->physics == 1) or_getime_density(hh, u, coef, c.tp);
+                    else or_ge_density(hh, u, coef);
+                    for (int k = 0; k < nd; k++) HH[k] += w * hh[k];
+                }
+    }
+    int nb[3];
+    nbasis_of(pb, nb);
+    int nk[3] = {pb->nelem[0] + 5, pb->nelem[1] + 5, pb->nelem[2] + 5};
+    for (int ib = 0; ib < pb->nboun; ib++) {
+        int f = pb->boun[ib];
+        face_map m = face_of(f, nb);
+        int d0 = m.dim[0], d1 = m.dim[1], d2 = m.dim[2];
+        const double *k0 = pb->knot[d0], *k1 = pb->knot[d1], *k2 = pb->knot[d2];
+        xq[d2] = (f % 2 == 0) ? k2[0] : k2[nk[d2] - 1];
+        for (int order = 0; order < 2; order++)
+            for (int i = 0; i < NDOF; i++) {
+                if (pb->bc_type[2 * NDOF * f + NDOF * order + i] != 1) continue;
+                for (int e0 = 0; e0 < pb->nelem[d0]; e0++)
+                    for (int e1 = 0; e1 < pb->nelem[d1]; e1++) {
+                        double X00 = k0[e0 + PORDER], X10 = k0[e0 + PORDER + 1];
+                        double X01 = k1[e1 + PORDER], X11 = k1[e1 + PORDER + 1];
+                        for (int qx = 0; qx < nq; qx++)
+                            for (int qy = 0; qy < nq; qy++) {
+                                double xi = c.cq[qx];
+                                xq[d0] = (-xi + 1.0) * X00 + xi * X10;
+                                xi = c.cq[qy];
+                                xq[d1] = (-xi + 1.0) * X01 + xi * X11;
+                                double ub[3] = {0.0, 0.0, 0.0};
+                                for (int bx = 0; bx < 3; bx++)
+                                    for (int by = 0; by < 3; by++) {
+                                        double nxy = or_evalN(k0, nk[d0], 0, e0 + bx, PORDER, xq[d0]) *
+                                                     or_evalN(k1, nk[d1], 0, e1 + by, PORDER, xq[d1]);
+                                        int64_t base = m.stride[0] * (e0 + bx) + m.stride[1] * (e1 + by);
+                                        for (int h = 0; h < c.nhist; h++) {
+                                            const double *Uh = U_hist + c.ndof * h;
+                                            if (order == 0) {
+                                                ub[h] += nxy * Uh[NDOF * (base + m.stride[2] * m.iz0) + i];
+                                            } else {
+                                                ub[h] += nxy * or_evalN(k2, nk[d2], 1, m.iz0, PORDER, xq[d2]) *
+                                                         Uh[NDOF * (base + m.stride[2] * m.iz0) + i];
+                                                ub[h] += nxy * or_evalN(k2, nk[d2], 1, m.iz1, PORDER, xq[d2]) *
+                                                         Uh[NDOF * (base + m.stride[2] * m.iz1) + i];
+                                            }
+                                        }
+                                    }
+                                double t = 0.0;
+                                if (pb->neumann) pb->neumann(&t, pb->par_neumann, f, order, i, xq);
+                                double wf = c.wq[qx] * c.wq[qy] * (X10 - X00) * (X11 - X01);
+                                if (pb->physics == 1) {
+                                    HH[0] -= wf * (ub[1] + ub[2]) / 2.0 * t;
+                                    HH[1] -= wf * 1.0;
+                                } else {
+                                    HH[0] -= wf * ub[0] * t;
+                                }
+                            }
+                    }
+            }
+    }
+    return nd;
 }

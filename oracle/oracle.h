@@ -106,6 +106,14 @@ int or_tangent_window(const or_problem *pb, const double *U_hist, const int lo[3
 /* or_residual with the element loop split over nthreads threads.          */
 int or_residual_mt(const or_problem *pb, const double *U_hist, double *R, int nthreads);
 
+/* Density outputs at one point: phys/ge/density.c:6-66 (13) and
+ * phys/getime/density.c:6-53 (2; tp = rho, c, dt).                       */
+void or_ge_density(double hh[13], const double u[30], const double coef[6]);
+void or_getime_density(double hh[2], const double u[90], const double coef[6], const double tp[3]);
+/* iga_intdensity (src/iga_intdensity.c:10-293): HH[0..ndensity); returns
+ * ndensity (13 ge, 2 getime).                                              */
+int or_intdensity(const or_problem *pb, const double *U_hist, double *HH);
+
 /* Energy functional (src/iga_intdensity.c:10-293, volume part, ge only). */
 double or_energy(const or_problem *pb, const double *U_hist);
 

@@ -220,6 +220,31 @@ def energy(pb, U_hist):
     return float(lib().or_energy(b.ptr, _d(U)))
 
 
+def density(physics, u, coef, tp=None):
+    """Density outputs at one point: 13 (ge, u of 30 slots) or 2 (getime, u
+    of 90 slots, tp = (rho, c, dt))."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    if physics == 1:
+        h = np.zeros(2)
+        lib().or_getime_density(_d(h), _d(u), _d(c), _d(np.ascontiguousarray(tp, dtype=np.float64)))
+    else:
+        h = np.zeros(13)
+        lib().or_ge_density(_d(h), _d(u), _d(c))
+    return h
+
+
+def intdensity(pb, U_hist):
+    """iga_intdensity: the ndensity integrated outputs (Neumann terms included)."""
+    b = Bound(pb)
+    U = np.ascontiguousarray(U_hist, dtype=np.float64)
+    HH = np.zeros(13)
+    nd = lib().or_intdensity(b.ptr, _d(U), _d(HH))
+    if nd < 0:
+        raise RuntimeError("or_intdensity failed: %d" % nd)
+    return HH[:nd]
+
+
 def gauss(n):
     x, w = np.zeros(n), np.zeros(n)
     if lib().or_gauss(n, _d(x), _d(w)):
@@ -357,6 +382,33 @@ def ref_ge_energy(u, par):
     ref().ge_density(_d(h), _d(np.ascontiguousarray(u, dtype=np.float64)),
                      _d(np.ascontiguousarray(par, dtype=np.float64)))
     return h[0]
+
+
+def ref_ge_density(u, par):
+    """phys/ge/density.c:6-66, all 13 outputs."""
+    h = np.zeros(13)
+    ref().ge_density(_d(h), _d(np.ascontiguousarray(u, dtype=np.float64)),
+                     _d(np.ascontiguousarray(par, dtype=np.float64)))
+    return h
+
+
+def ref_getime_density(u90, par):
+    """phys/getime/density.c:6-53, both outputs."""
+    h = np.zeros(2)
+    ref().getime_density(_d(h), _d(np.ascontiguousarray(u90, dtype=np.float64)),
+                         _d(np.ascontiguousarray(par, dtype=np.float64)))
+    return h
+
+
+def ref_densityb(physics, ub, traction, par, order):
+    """phys/ge/density.c:68-84 (13 outputs) or phys/getime/density.c:57-62
+    (2 outputs) at the face values ub (one per history slice)."""
+    n = 2 if physics == 1 else 13
+    h = np.zeros(n)
+    fn = ref().getime_densityb if physics == 1 else ref().ge_densityb
+    fn(_d(h), _d(np.ascontiguousarray(ub, dtype=np.float64)), ctypes.c_double(traction),
+       _d(np.ascontiguousarray(par, dtype=np.float64)), ctypes.c_int(order))
+    return h
 
 
 def ref_getime_energy(u90, par):

@@ -10,6 +10,7 @@ Covers (reference file:line):
   basis.npz             mesh/nurbs/eval_Bspline.c:9-93, evalN.c:7-62,
                         src/quadrature.c:381-429
   dirichlet.npz         src/iga_dirichlet.c:7-80
+  density.npz           phys/ge/density.c:6-84, phys/getime/density.c:6-62
 Inputs are seeded; the reference parameters are phys/ge/init.c:18.
 """
 import os
@@ -87,6 +88,22 @@ def main():
     for j, r in enumerate(recs):
         arr[j, :len(r)] = r
     np.savez_compressed(os.path.join(HERE, "dirichlet.npz"), par=pd, rec=arr)
+    # ---- every density output of ge and getime, value slots included, and
+    # the boundary densities (own generator: the draws above are unchanged)
+    rd = np.random.default_rng(0xDE75)
+    amps = np.repeat([1e-3, 1e-2, 5e-2, 1e-1, 3e-1], 12)
+    Uge = rd.uniform(-1, 1, (amps.size, 30)) * amps[:, None]
+    Hge = np.array([O.ref_ge_density(u, PAR) for u in Uge])
+    Ugt = rd.uniform(-1, 1, (amps.size, 90)) * amps[:, None]
+    Hgt = np.array([O.ref_getime_density(u, PAR) for u in Ugt])
+    nb = 24
+    UB = rd.uniform(-1, 1, (nb, 3))
+    T = rd.uniform(-2, 2, nb)
+    ORD = rd.integers(0, 2, nb)
+    HBge = np.array([O.ref_densityb(0, UB[k, :1], T[k], PAR, int(ORD[k])) for k in range(nb)])
+    HBgt = np.array([O.ref_densityb(1, UB[k], T[k], PAR, int(ORD[k])) for k in range(nb)])
+    np.savez_compressed(os.path.join(HERE, "density.npz"), par=PAR, u_ge=Uge, hh_ge=Hge, u_getime=Ugt,
+                        hh_getime=Hgt, ub=UB, traction=T, order=ORD, hhb_ge=HBge, hhb_getime=HBgt)
     print("golden vectors written to", HERE)
 
 

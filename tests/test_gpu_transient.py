@@ -70,8 +70,9 @@ def test_lagged_tangent_steps(oracle):
 
 
 def test_getime_energy_matches_oracle(oracle):
-    """igap4h_energy_dev for getime: W at (u0 + u1) / 2 integrated over the
-    volume (phys/getime/density.c via iga_intdensity), against the oracle."""
+    """igap4h_energy_dev for getime: W at (u0 + u1) / 2 plus the kinetic
+    energy rho |u1 - u0|^2 / (2 dt^2) integrated over the volume
+    (phys/getime/density.c via iga_intdensity), against the oracle."""
     import torch
     from igap4standard_amd.snesiga import App
     pb = P.reference_example(3, physics=P.GETIME)

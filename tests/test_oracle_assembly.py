@@ -240,3 +240,55 @@ def test_residual_threads_same_sum(oracle):
     Ut = P.smooth_state(pt, amp=0.05, noise=1e-3)
     assert np.linalg.norm(oracle.residual_mt(pt, Ut, 5) - oracle.residual(pt, Ut)) <= \
         1e-14 * np.linalg.norm(oracle.residual(pt, Ut))
+
+
+def test_intdensity_energy_gradient_with_boundary_work(oracle):
+    """or_intdensity HH[0] (volume energy minus the Neumann traction work,
+    src/iga_intdensity.c:196-287) has gradient R (or_residual, Neumann load
+    included) on every row that is not a Dirichlet row: central differences
+    on a ragged mesh with standard and high-order Neumann faces, a position-
+    dependent traction and a periodic F0."""
+    pb = P.stretch_problem((3, 2, 4), a=0.01, noise=0.1)
+    pb.bc_type[18 + 3:18 + 6] = 1
+    pb.par_periodic = np.array([0.001, 0.0, 0.0, 0.0, -0.002, 0.0, 0.0, 0.0, 0.0005])
+    pb.neumann = lambda face, order, idof, xq: (0.3 + xq[0] - 0.5 * xq[1] * xq[2]) * (1 + idof) * (1 - 0.4 * order)
+    U = P.smooth_state(pb, amp=0.05, noise=1e-3)
+    R = oracle.residual(pb, U)
+    free = np.ones(pb.ndof, bool)
+    free[oracle.dirichlet_rows(pb)] = False
+    rows = np.nonzero(free)[0][::5]
+    h = 1e-6
+    fd = []
+    for r in rows:
+        Up, Um = U.copy(), U.copy()
+        Up[r] += h
+        Um[r] -= h
+        fd.append((oracle.intdensity(pb, Up)[0] - oracle.intdensity(pb, Um)[0]) / (2 * h))
+    fd = np.array(fd)
+    assert np.linalg.norm(fd - R[rows]) <= 1e-6 * np.linalg.norm(R[rows])
+    HH = oracle.intdensity(pb, U)
+    assert len(HH) == 13 and HH[0] != oracle.energy(pb, U)   # the traction work enters
+
+
+def test_intdensity_getime_outputs(oracle):
+    """getime: HH[0] without tractions is the volume energy (W at the mean
+    state plus kinetic energy; or_energy), HH[1] = -(number of Neumann flags
+    x face area) on the unit cube; with a traction the work of the mean
+    state (slices 1, 2) enters HH[0] linearly."""
+    pb = P.transient_problem(3)
+    pb.bc_type = P.bc_ex1()                       # high-order Neumann on all 6 faces: 18 flags
+    U = P.smooth_state(pb, amp=0.05, noise=1e-3)
+    HH = oracle.intdensity(pb, U)
+    assert len(HH) == 2
+    assert abs(HH[0] - oracle.energy(pb, U)) <= 1e-14 * abs(HH[0])
+    assert abs(HH[1] + 18.0) < 1e-12 * 18.0
+    n = pb.ndof
+    pb.neumann = lambda face, order, idof, xq: 0.2 + xq[0]
+    H1 = oracle.intdensity(pb, U)
+    V = U.copy()
+    V[n:] *= 2.0                                  # mean state doubled: work doubles
+    H2 = oracle.intdensity(pb, V)
+    pb.neumann = None
+    W2 = oracle.intdensity(pb, V)[0]
+    assert abs((H2[0] - W2) - 2.0 * (H1[0] - HH[0])) <= 1e-12 * abs(H1[0] - HH[0])
+    assert H1[0] != HH[0]

@@ -205,3 +205,36 @@ def test_getime_energy_matches_reference_density(oracle):
         e_or = oracle.ge_energy(ub, coef)
         e_ref = oracle.ref_getime_energy(u90, par)
         assert abs(e_or - e_ref) <= 1e-12 * max(1.0, abs(e_ref)), (amp, e_or, e_ref)
+
+
+def test_density_outputs_match_golden(oracle):
+    """All 13 ge density outputs (phys/ge/density.c:6-66: W, the stress-like
+    outputs, the variant indicators) and both getime outputs (W at the mean
+    state plus the kinetic energy of the value slots) against the compiled
+    reference's vectors (tests/golden/density.npz)."""
+    import numpy as np
+    g = np.load(os.path.join(GOLD, "density.npz"))
+    par = g["par"]
+    coef = oracle.coef_from_par(par)
+    for u, h in zip(g["u_ge"], g["hh_ge"]):
+        d = oracle.density(0, u, coef)
+        assert np.abs(d[:7] - h[:7]).max() <= 1e-12 * np.abs(h[:7]).max()
+        assert np.array_equal(d[7:], h[7:])
+    assert g["hh_ge"][:, 7:].any()           # the indicators are exercised
+    for u, h in zip(g["u_getime"], g["hh_getime"]):
+        d = oracle.density(1, u, coef, par[5:8])
+        assert abs(d[0] - h[0]) <= 1e-12 * abs(h[0]) and d[1] == h[1] == 0.0
+    # the kinetic term is present: value slots differ between the slices
+    assert np.abs(g["u_getime"][:, 30] - g["u_getime"][:, 60]).max() > 0
+
+
+def test_boundary_density_matches_golden():
+    """densityb as or_intdensity applies it: ge hh[0] = u t, the rest 0
+    (phys/ge/density.c:68-84); getime hh[0] = (u1 + u0) / 2 t (slices 1 and
+    2), hh[1] = 1 (phys/getime/density.c:57-62)."""
+    import numpy as np
+    g = np.load(os.path.join(GOLD, "density.npz"))
+    ub, t = g["ub"], g["traction"]
+    assert np.array_equal(g["hhb_ge"][:, 0], ub[:, 0] * t) and not g["hhb_ge"][:, 1:].any()
+    assert np.array_equal(g["hhb_getime"][:, 0], (ub[:, 1] + ub[:, 2]) / 2 * t)
+    assert np.all(g["hhb_getime"][:, 1] == 1.0)
