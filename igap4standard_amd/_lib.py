@@ -55,6 +55,7 @@ class Opts(ctypes.Structure):
                 ("neumann", NEUMANN_FN)]
 
 
+IGAP4H_OK, IGAP4H_EINVAL, IGAP4H_ENOMEM, IGAP4H_EHIP, IGAP4H_ENODEV = 0, -1, -2, -3, -4
 IGAP4H_FP64, IGAP4H_FP32 = 0, 1
 IGAP4H_VALS_F64, IGAP4H_VALS_F32 = 0, 1
 IGAP4H_PHYS_GE, IGAP4H_PHYS_GETIME, IGAP4H_PHYS_FROM_TORDER = 0, 1, -1
@@ -99,7 +100,7 @@ SYMBOLS = ("igap4h_create", "igap4h_destroy", "igap4h_last_error", "igap4h_sizes
            "igap4h_newton_default_opts", "igap4h_newton_solve_dev", "igap4h_spmv_dev", "igap4h_linear_solve_dev",
            "igap4h_residual_vjp_coef_dev", "igap4h_batch_create", "igap4h_batch_destroy", "igap4h_batch_sizes",
            "igap4h_batch_tune", "igap4h_batch_set_variant", "igap4h_batch_assemble", "igap4h_write_solution",
-           "igap4h_read_solution", "igap4h_knot_insert", "igap4h_update_bc")
+           "igap4h_read_solution", "igap4h_knot_insert", "igap4h_update_bc", "igap4h_update_neumann")
 
 _lib = None
 
@@ -147,6 +148,7 @@ def lib():
         L.igap4h_batch_set_variant.argtypes = [vp, ctypes.c_int]
         L.igap4h_batch_assemble.argtypes = [vp, vp]
         L.igap4h_update_bc.argtypes = [vp, ctypes.POINTER(BC)]
+        L.igap4h_update_neumann.argtypes = [vp]
         L.igap4h_write_solution.argtypes = [ctypes.c_char_p, c_double_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int]
         L.igap4h_read_solution.argtypes = [ctypes.c_char_p, c_double_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int]
         L.igap4h_knot_insert.argtypes = [c_double_p, c_int_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -164,11 +166,15 @@ def lib():
 
 
 class IgapError(RuntimeError):
-    pass
+    """A negative igap4h status; .code holds it (IGAP4H_EINVAL, ...)."""
+
+    def __init__(self, msg, code=0):
+        super().__init__(msg)
+        self.code = code
 
 
 def check(rc):
     if rc != 0:
         msg = lib().igap4h_last_error()
-        raise IgapError("igap4h error %d: %s" % (rc, msg.decode() if msg else ""))
+        raise IgapError("igap4h error %d: %s" % (rc, msg.decode() if msg else ""), rc)
     return rc

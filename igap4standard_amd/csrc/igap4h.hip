@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -38,6 +39,15 @@ int igap4h::fail(int code, const char *fmt, ...)
 }
 
 extern "C" const char *igap4h_last_error(void) { return g_err; }
+
+bool igap4h::alloc_refused(size_t bytes)
+{
+    const char *v = std::getenv("IGAP4H_DEBUG_ALLOC_LIMIT");
+    if (!v || !*v) return false;
+    char *end = nullptr;
+    unsigned long long lim = std::strtoull(v, &end, 10);
+    return end != v && bytes > lim;
+}
 
 // ============================================================== kernel args
 struct MeshArgs {
@@ -1675,6 +1685,30 @@ extern "C" int igap4h_update_bc(igap4h_ctx *c, const BC *bc)
     return 0;
 }
 
+extern "C" int igap4h_update_neumann(igap4h_ctx *c)
+{
+    if (!c) return fail(IGAP4H_EINVAL, "NULL context");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipDeviceSynchronize());
+    const int64_t ndof = c->mesh.ndof();
+    std::vector<double> fneu = neumann_load(c->mesh, c->nquad, c->neumann);
+    if (fneu.empty()) {
+        if (c->d_fneu) {
+            (void)hipFree(c->d_fneu);
+            c->d_fneu = nullptr;
+            c->bc_gen++;
+        }
+        return 0;
+    }
+    if (!c->d_fneu) {
+        int r = dev_alloc(&c->d_fneu, (size_t)ndof);
+        if (r) return r;
+        c->bc_gen++;
+    }
+    HIPCHK(hipMemcpy(c->d_fneu, fneu.data(), (size_t)ndof * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
 extern "C" int igap4h_set_coef(igap4h_ctx *c, const double *elem_coef)
 {
     if (!c) return fail(IGAP4H_EINVAL, "NULL context");
@@ -1851,7 +1885,7 @@ extern "C" int igap4h_tangent(igap4h_ctx *c, const double *U, double *U_hist, in
         (void)hipFree(c->d_vals);
         c->d_vals = nullptr;
         c->d_vals_bytes = 0;
-        if (hipMalloc(&c->d_vals, bytes) != hipSuccess) return fail(IGAP4H_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
+        if (dev_malloc(&c->d_vals, bytes) != hipSuccess) return fail(IGAP4H_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
         c->d_vals_bytes = bytes;
     }
     rc = igap4h_tangent_dev(c, c->d_Uh, prec, vals_type, c->d_vals, nullptr);
@@ -2221,8 +2255,8 @@ extern "C" int igap4h_batch_create(igap4h_ctx *const *ctxs, int n, const double 
         return fail(IGAP4H_EINVAL, "batch has %lld elements; at most 2^31 - 1", b->eoff[n]);
     }
     hipError_t e;
-    if ((e = hipSetDevice(b->device)) != hipSuccess || (e = hipMalloc(&b->d_inst, sizeof(BatchInst) * n)) != hipSuccess ||
-        (e = hipMalloc(&b->d_off, sizeof(long long) * 28 * (n + 1))) != hipSuccess ||
+    if ((e = hipSetDevice(b->device)) != hipSuccess || (e = dev_malloc((void **)&b->d_inst, sizeof(BatchInst) * n)) != hipSuccess ||
+        (e = dev_malloc((void **)&b->d_off, sizeof(long long) * 28 * (n + 1))) != hipSuccess ||
         (e = hipMemcpy(b->d_inst, b->h_inst.data(), sizeof(BatchInst) * n, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(b->d_off, b->eoff.data(), sizeof(long long) * (n + 1), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(b->d_off + (n + 1), b->coff.data(), sizeof(long long) * 27 * (n + 1), hipMemcpyHostToDevice)) != hipSuccess ||

@@ -58,11 +58,26 @@ struct igap4h_ctx {
 };
 
 namespace igap4h {
+// Fault injection: with IGAP4H_DEBUG_ALLOC_LIMIT=<bytes> in the environment,
+// device allocations of more than that many bytes are refused as if
+// hipMalloc had failed, so that the out-of-memory paths can be exercised.
+bool alloc_refused(size_t bytes);
+
+// hipMalloc with the fault-injection check
+static inline hipError_t dev_malloc(void **p, size_t bytes)
+{
+    if (alloc_refused(bytes)) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc(p, bytes);
+}
+
 template <class X>
 static inline int dev_alloc(X **p, size_t n)
 {
     if (n == 0) n = 1;
-    if (hipMalloc((void **)p, n * sizeof(X)) != hipSuccess) {
+    if (dev_malloc((void **)p, n * sizeof(X)) != hipSuccess) {
         *p = nullptr;
         return fail(IGAP4H_ENOMEM, "hipMalloc of %zu bytes failed", n * sizeof(X));
     }

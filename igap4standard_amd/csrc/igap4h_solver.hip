@@ -1317,10 +1317,13 @@ static int solver_ws(igap4h_ctx *c, int restart, size_t vals_bytes)
             (rc = dev_alloc(&s->r, n)) || (rc = dev_alloc(&s->z, n)) || (rc = dev_alloc(&s->p, n)) ||
             (rc = dev_alloc(&s->q, n)) || (rc = dev_alloc(&s->w, n)) || (rc = dev_alloc(&s->t, n)) ||
             (rc = dev_alloc(&s->Y, n)) || (rc = dev_alloc(&s->U0, n)) || (rc = dev_alloc(&s->R, n)) ||
-            (rc = dev_alloc(&s->sc, 64)) || (rc = dev_alloc(&s->pspmv, (size_t)s->nnode + 1)))
+            (rc = dev_alloc(&s->sc, 64)) || (rc = dev_alloc(&s->pspmv, (size_t)s->nnode + 1))) {
+            solver_free(c);  // no half-built work space survives a failure
             return rc;
+        }
         if (hipHostMalloc((void **)&s->hp, 64 * sizeof(double)) != hipSuccess) {
             s->hp = nullptr;
+            solver_free(c);
             return fail(IGAP4H_ENOMEM, "hipHostMalloc of 512 bytes failed");
         }
     }
@@ -1351,7 +1354,7 @@ static int solver_ws(igap4h_ctx *c, int restart, size_t vals_bytes)
         (void)hipFree(s->vals);
         s->vals = nullptr;
         s->vals_bytes = 0;
-        if (hipMalloc(&s->vals, vals_bytes) != hipSuccess) return fail(IGAP4H_ENOMEM, "hipMalloc of %zu bytes failed", vals_bytes);
+        if (dev_malloc(&s->vals, vals_bytes) != hipSuccess) return fail(IGAP4H_ENOMEM, "hipMalloc of %zu bytes failed", vals_bytes);
         s->vals_bytes = vals_bytes;
     }
     return 0;
@@ -1591,7 +1594,7 @@ struct Op {
             if (cheb16_lds_bytes(sp.nb[2]) > 65536)
                 return fail(IGAP4H_EINVAL, "16-bit smoothing copy: %d nodes in direction 2 (limit 617)", sp.nb[2]);
             const size_t bytes = (size_t)npk * NU16 * 64 * sizeof(uint4);
-            if (hipMalloc(&s->sm16, bytes) != hipSuccess) {
+            if (dev_malloc((void **)&s->sm16, bytes) != hipSuccess) {
                 s->sm16 = nullptr;
                 return fail(IGAP4H_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
             }
@@ -1682,7 +1685,7 @@ struct Op {
             int3 g3 = make_int3(C.nb[0], C.nb[1], C.nb[2]);
             auto nthr = [](int3 g) { return grid_for((long long)g.x * g.y * g.z * 125, 256); };
             if (l == 0) {
-                if (!s->nmask && hipMalloc(&s->nmask, (size_t)sp.nnode) != hipSuccess) {
+                if (!s->nmask && dev_malloc((void **)&s->nmask, (size_t)sp.nnode) != hipSuccess) {
                     s->nmask = nullptr;
                     return fail(IGAP4H_ENOMEM, "hipMalloc of %lld bytes failed", sp.nnode);
                 }
@@ -2364,8 +2367,8 @@ extern "C" int igap4h_newton_solve_dev(igap4h_ctx *c, double *d_Uh, const igap4h
     double fold = 999.0;  // snes_solve.c:23
     info->converged = 0;
     for (int it = 0; it < o->max_newton; it++) {
-        if (!std::isfinite(fnorm)) {
-            info->converged = -3;
+        if (!std::isfinite(fnorm)) {  // SNES_DIVERGED_FNORM_NAN
+            info->converged = -4;
             break;
         }
         if ((o->atol > 0.0 && fnorm < o->atol) || (o->rtol > 0.0 && fnorm < o->rtol * info->fnorm0)) {

@@ -146,6 +146,10 @@ class App:
         L.check(L.lib().igap4h_sizes(self.ctx, None, None, ctypes.byref(nd), None, None))
         self.ndirichlet = nd.value
 
+    def update_neumann(self):
+        """Evaluate the traction hook again (its values changed)."""
+        L.check(L.lib().igap4h_update_neumann(self.ctx))
+
     def set_coef(self, coef):
         self._coef = np.ascontiguousarray(coef, dtype=np.float64)
         L.check(L.lib().igap4h_set_coef(self.ctx, _dp(self._coef)))

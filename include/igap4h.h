@@ -172,6 +172,19 @@ int igap4h_sizes(const igap4h_ctx *ctx, int64_t *ndof, int64_t *nnz, int64_t *nd
  * lagged tangent of the Newton driver is discarded.                        */
 int igap4h_update_bc(igap4h_ctx *ctx, const BC *bc);
 
+/* The reference calls the app's neumann hook at every face Gauss point in
+ * every residual callback (snesiga_Residual.c:188-320); a context evaluates
+ * it at create (and when igap4h_update_bc changes the boundary data).  Call
+ * this when the hook's values change between callbacks (a traction that
+ * depends on time or on par_neumann's contents): the hook is evaluated again
+ * and the Neumann load replaced (device synchronized first, as
+ * igap4h_update_bc).                                                       */
+int igap4h_update_neumann(igap4h_ctx *ctx);
+
+/* Fault injection: with IGAP4H_DEBUG_ALLOC_LIMIT=<bytes> in the environment,
+ * every device allocation of the library larger than that is refused with
+ * IGAP4H_ENOMEM (read at each allocation).                                  */
+
 /* ---- index sets (bit-exact with the reference) ------------------------ */
 /* AIJ pattern of P (snesiga_init.c:112-124 with iga_setnz :341-359): rows in
  * globalIdx order, columns ascending.  Host outputs (ndof+1, nnz).        */
@@ -363,7 +376,8 @@ typedef struct {
 
 typedef struct {
     int converged;     /* 1 converged, 0 iteration limit, -1 stagnated, -2 linear solve failed,
-                          -3 line search failed */
+                          -3 line search failed, -4 residual norm not finite
+                          (SNES_DIVERGED_FNORM_NAN) */
     int newton_its;
     int ksp_its;       /* summed over Newton steps */
     double fnorm0, fnorm;

@@ -212,3 +212,32 @@ def test_getime_ragged_q2(oracle):
     rowptr, col = app.pattern()
     _, _, v0 = oracle.tangent(pb, Uh, rowptr, col)
     assert rel(app.tangent(U), v0) < TOL64
+
+
+@pytest.mark.parametrize("physics", [P.GE, P.GETIME])
+def test_tangent_periodic_F0(oracle, physics):
+    """The tangent with a par_periodic F0 added to every slice's slots
+    (snesiga_Tangent.c:125-144) against the oracle, FP64 and FP32."""
+    say("tangent with F0", physics)
+
+    def problem():
+        pb = P.stretch_problem((3, 2, 4), a=0.01, noise=0.1)
+        pb.physics = physics
+        pb.par_mat = pb.par_mat.copy()
+        pb.par_mat[7] = 0.5            # dt: a mass term rho/dt^2 that does not hide the elastic part
+        return pb
+
+    pb = problem()
+    pb.par_periodic = np.array([0.001, 0.0, 0.0, 0.0, -0.002, 0.0, 0.0, 0.0, 0.0005])
+    U = state(pb) if physics == P.GE else P.smooth_state(pb, amp=0.05, noise=1e-3)
+    app = App(pb)
+    rowptr, col = app.pattern()
+    _, _, v0 = oracle.tangent(pb, U, rowptr, col)
+    pb0 = problem()
+    _, _, vz = oracle.tangent(pb0, U, rowptr, col)
+    assert rel(vz, v0) > 1e-6                       # F0 changes the tangent
+    app.U_hist[:] = U                               # history slices (getime) as the oracle has them
+    v = app.tangent(U[:pb.ndof], prec=L.IGAP4H_FP64)
+    assert rel(v, v0) < TOL64, rel(v, v0)
+    v32 = app.tangent(U[:pb.ndof], prec=L.IGAP4H_FP32)
+    assert rel(v32, v0) < TOL32, rel(v32, v0)
