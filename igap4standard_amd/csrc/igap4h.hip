@@ -482,7 +482,8 @@ __device__ __forceinline__ void gather_body(const MeshArgs &m, const double *__r
                 s += Re[e * 81 + 3 * A + i];
             }
     if (fneu) s += fneu[dof];
-    if (dmask[dof]) s = U0[dof] - gval[dof];
+    if (dmask[dof] == 1) s = U0[dof] - gval[dof];
+    else if (dmask[dof] == 2) s = 0.0;  // another rank's Dirichlet row: it sets the value
     R[dof] = s;
 }
 
@@ -549,7 +550,8 @@ __global__ __launch_bounds__(256) void k_gather_finish(MeshArgs m, const double 
     if (dof >= m.ndof) return;
     double s = R[dof];
     if (fneu) s += fneu[dof];
-    if (dmask[dof]) s = U0[dof] - gval[dof];
+    if (dmask[dof] == 1) s = U0[dof] - gval[dof];
+    else if (dmask[dof] == 2) s = 0.0;
     R[dof] = s;
 }
 
@@ -1474,12 +1476,22 @@ extern "C" int igap4h_create(const IGA *app, const igap4h_opts *opts, igap4h_ctx
     Mesh &m = c->mesh;
     for (int d = 0; d < 3; d++) {
         m.ne[d] = nu->nelem[d];
-        m.nb[d] = nu->nbasis[d];
-        if (m.ne[d] < 1 || m.nb[d] != m.ne[d] + 2 || nu->nknot[d] != m.ne[d] + 5 || !nu->knotVector[d]) {
+        m.nb[d] = m.ne[d] + 2;        // the local window: every basis function of the local elements
+        m.nown[d] = nu->nbasis[d];    // owned: nelem + 2 (last part of the direction) or nelem
+        if (m.ne[d] < 1 || (m.nown[d] != m.ne[d] + 2 && m.nown[d] != m.ne[d]) || nu->nknot[d] != m.ne[d] + 5 ||
+            !nu->knotVector[d]) {
             delete c;
-            return fail(IGAP4H_EINVAL, "direction %d: need nbasis = nelem + 2, nknot = nelem + 5 (one rank, no periodicity)", d);
+            return fail(IGAP4H_EINVAL, "direction %d: need nbasis = nelem + 2 (or nelem on a rank that is not the last "
+                                       "part of the direction) and nknot = nelem + 5 (no periodicity)", d);
         }
         m.knot[d].assign(nu->knotVector[d], nu->knotVector[d] + nu->nknot[d]);
+    }
+    // a rank's block when it owns less than its window, or (the last part in
+    // every direction) its rows are not the first ones
+    c->partitioned = m.nown[0] != m.nb[0] || m.nown[1] != m.nb[1] || m.nown[2] != m.nb[2];
+    if (!c->partitioned && app->globalIdx) {
+        const int64_t n = 3LL * m.nb[0] * m.nb[1] * m.nb[2];
+        for (int64_t i = 0; i < n && !c->partitioned; i++) c->partitioned = app->globalIdx[i] != i;
     }
     if (nu->nboun < 0 || nu->nboun > 6 || (nu->nboun > 0 && !nu->boun)) {
         delete c;
@@ -1504,16 +1516,47 @@ extern "C" int igap4h_create(const IGA *app, const igap4h_opts *opts, igap4h_ctx
         delete c;
         return fail(IGAP4H_EINVAL, "ndof %lld exceeds int32 column indices", (long long)ndof);
     }
-    if (app->nselfIdx != 0 && app->nselfIdx != ndof) {
-        delete c;
-        return fail(IGAP4H_EINVAL, "nselfIdx %d != ndof %lld (multi-rank partitions are not supported)", app->nselfIdx, (long long)ndof);
-    }
-    if (app->globalIdx) {
-        for (int64_t i = 0; i < ndof; i++)
-            if (app->globalIdx[i] != i) {
+    if (!c->partitioned) {
+        if (app->nselfIdx != 0 && app->nselfIdx != ndof) {
+            delete c;
+            return fail(IGAP4H_EINVAL, "nselfIdx %d != ndof %lld", app->nselfIdx, (long long)ndof);
+        }
+    } else {
+        // one rank's block (src/snesiga_init.c:134-339): owned basis functions in
+        // the local window through selfIdx, the received ghost layers through
+        // recvIdx, globalIdx for the window
+        const int64_t nself = 3LL * m.nown[0] * m.nown[1] * m.nown[2];
+        if (app->nselfIdx != nself || app->nrecvIdx != ndof - nself || !app->selfIdx || (ndof > nself && !app->recvIdx) ||
+            !app->globalIdx) {
+            delete c;
+            return fail(IGAP4H_EINVAL, "partitioned block: need nselfIdx = %lld, nrecvIdx = %lld, selfIdx, recvIdx and "
+                                       "globalIdx (got %d, %d)", (long long)nself, (long long)(ndof - nself), app->nselfIdx,
+                        app->nrecvIdx);
+        }
+        std::vector<unsigned char> seen((size_t)ndof, 0);
+        int64_t ia = 0;
+        for (int x = 0; x < m.nown[0]; x++)
+            for (int y = 0; y < m.nown[1]; y++)
+                for (int z = 0; z < m.nown[2]; z++)
+                    for (int i = 0; i < 3; i++, ia++) {
+                        const int64_t want = 3 * (((int64_t)x * m.nb[1] + y) * m.nb[2] + z) + i;
+                        if (app->selfIdx[ia] != want) {
+                            delete c;
+                            return fail(IGAP4H_EINVAL, "selfIdx[%lld] = %d, expected %lld (iga_setmpicomm layout)", (long long)ia,
+                                        app->selfIdx[ia], (long long)want);
+                        }
+                        seen[want] = 1;
+                    }
+        for (int64_t k = 0; k < ndof - nself; k++) {
+            const int r = app->recvIdx[k];
+            if (r < 0 || r >= ndof || seen[r]) {
                 delete c;
-                return fail(IGAP4H_EINVAL, "globalIdx is not the identity at %lld (one rank only)", (long long)i);
+                return fail(IGAP4H_EINVAL, "recvIdx[%lld] = %d is outside the window or repeats an owned/received entry",
+                            (long long)k, r);
             }
+            seen[r] = 1;
+        }
+        c->gidx.assign(app->globalIdx, app->globalIdx + ndof);
     }
     c->U_hist_app = app->U_hist;
     c->neumann = o.neumann;
@@ -1631,7 +1674,11 @@ extern "C" int igap4h_pattern(igap4h_ctx *c, int64_t *rowptr, int32_t *col)
 extern "C" int igap4h_dirichlet_rows(igap4h_ctx *c, int32_t *rows)
 {
     if (!c || !rows) return fail(IGAP4H_EINVAL, "NULL argument");
-    if (!c->drows.empty()) std::memcpy(rows, c->drows.data(), c->drows.size() * sizeof(int32_t));
+    if (c->partitioned) {  // row_dirichlet holds globalIdx of the local rows (snesiga_Tangent.c:256,264)
+        for (size_t k = 0; k < c->drows.size(); k++) rows[k] = c->gidx[(size_t)c->drows[k]];
+    } else if (!c->drows.empty()) {
+        std::memcpy(rows, c->drows.data(), c->drows.size() * sizeof(int32_t));
+    }
     return 0;
 }
 
@@ -1863,6 +1910,7 @@ static int stage_U(igap4h_ctx *c, const double *U, double *U_hist)
 extern "C" int igap4h_residual(igap4h_ctx *c, const double *U, double *U_hist, double *R)
 {
     if (!c || !R) return fail(IGAP4H_EINVAL, "NULL argument");
+    if (c->partitioned) return fail(IGAP4H_EINVAL, "a partitioned block takes the device entry points (ghosted local vectors)");
     HIPCHK(hipSetDevice(c->device));
     int rc = stage_U(c, U, U_hist);
     if (rc) return rc;
@@ -1877,6 +1925,7 @@ extern "C" int igap4h_residual(igap4h_ctx *c, const double *U, double *U_hist, d
 extern "C" int igap4h_tangent(igap4h_ctx *c, const double *U, double *U_hist, int prec, int vals_type, void *vals)
 {
     if (!c || !vals) return fail(IGAP4H_EINVAL, "NULL argument");
+    if (c->partitioned) return fail(IGAP4H_EINVAL, "a partitioned block takes the device entry points (ghosted local vectors)");
     HIPCHK(hipSetDevice(c->device));
     int rc = stage_U(c, U, U_hist);
     if (rc) return rc;
@@ -2213,6 +2262,10 @@ extern "C" int igap4h_batch_create(igap4h_ctx *const *ctxs, int n, const double 
         if (c->physics != IGAP4H_PHYS_GE) {
             igap4h_batch_destroy(b);
             return fail(IGAP4H_EINVAL, "batch instance %d: batched assembly is implemented for phys/ge", i);
+        }
+        if (c->partitioned) {
+            igap4h_batch_destroy(b);
+            return fail(IGAP4H_EINVAL, "batch instance %d is a partitioned block (whole patches only)", i);
         }
         if (i == 0) b->device = c->device;
         else if (c->device != b->device) {

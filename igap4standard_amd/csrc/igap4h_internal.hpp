@@ -55,6 +55,10 @@ struct igap4h_ctx {
     // incremented whenever the Dirichlet/Neumann device data change (their
     // arrays may be reallocated), so that users of copies (batches) refresh
     int bc_gen = 0;
+    // one rank's block of a partitioned patch: mesh.nown < mesh.nb in some
+    // direction; gidx = globalIdx of the local window
+    bool partitioned = false;
+    std::vector<int> gidx;
 };
 
 namespace igap4h {

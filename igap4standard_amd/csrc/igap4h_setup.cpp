@@ -90,14 +90,16 @@ struct FaceMap {
     int iz0, iz1;
 };
 
-FaceMap face_map(int f, const int nb[3])
+// strides from the window's basis counts nb; the far face's layers from the
+// owned counts nown (snesiga_Residual.c:352-361 uses Nurbs.nbasis)
+FaceMap face_map(int f, const int nb[3], const int nown[3])
 {
     FaceMap m;
     int64_t sx = (int64_t)nb[1] * nb[2], sy = nb[2], sz = 1;
     if (f / 2 == 0) { m.dim[0] = 1; m.dim[1] = 2; m.dim[2] = 0; m.stride[0] = sy; m.stride[1] = sz; m.stride[2] = sx; }
     else if (f / 2 == 1) { m.dim[0] = 0; m.dim[1] = 2; m.dim[2] = 1; m.stride[0] = sx; m.stride[1] = sz; m.stride[2] = sy; }
     else { m.dim[0] = 0; m.dim[1] = 1; m.dim[2] = 2; m.stride[0] = sx; m.stride[1] = sy; m.stride[2] = sz; }
-    int nz = nb[m.dim[2]];
+    int nz = nown[m.dim[2]];
     m.iz0 = (f % 2 == 0) ? 0 : nz - 1;
     m.iz1 = (f % 2 == 0) ? 1 : nz - 2;
     return m;
@@ -112,7 +114,7 @@ void face_dirichlet(const Mesh &m, int f, int order, int idof, const FaceMap &fm
     for (int t = 0; t < 3; t++) {
         kv[t] = m.knot[fm.dim[t]].data();
         nk[t] = (int)m.knot[fm.dim[t]].size();
-        nbf[t] = m.nb[fm.dim[t]];
+        nbf[t] = m.nown[fm.dim[t]];   // iga_dirichlet gets Nurbs.nbasis (owned counts)
     }
     const double *par = m.par_dirichlet;
     double Z = (f % 2 == 0) ? kv[2][0] : kv[2][nk[2] - 1] - 1.e-15;
@@ -154,13 +156,13 @@ std::vector<int32_t> dirichlet_rows(const Mesh &m)
     std::vector<int32_t> rows;
     for (int ib = 0; ib < m.nboun; ib++) {
         int f = m.boun[ib];
-        FaceMap fm = face_map(f, m.nb);
+        FaceMap fm = face_map(f, m.nb, m.nown);
         for (int order = 0; order < 2; order++) {
             int iz = order == 0 ? fm.iz0 : fm.iz1;
             for (int i = 0; i < 3; i++) {
                 if (m.bc_type[6 * f + 3 * order + i] != 0) continue;
-                for (int x = 0; x < m.nb[fm.dim[0]]; x++)
-                    for (int y = 0; y < m.nb[fm.dim[1]]; y++)
+                for (int x = 0; x < m.nown[fm.dim[0]]; x++)
+                    for (int y = 0; y < m.nown[fm.dim[1]]; y++)
                         rows.push_back((int32_t)(3 * (fm.stride[0] * x + fm.stride[1] * y + fm.stride[2] * iz) + i));
             }
         }
@@ -175,18 +177,36 @@ void dirichlet_values(const Mesh &m, std::vector<unsigned char> &mask, std::vect
     std::vector<double> vals;
     for (int ib = 0; ib < m.nboun; ib++) {
         int f = m.boun[ib];
-        FaceMap fm = face_map(f, m.nb);
+        FaceMap fm = face_map(f, m.nb, m.nown);
         for (int order = 0; order < 2; order++) {
             int iz = order == 0 ? fm.iz0 : fm.iz1;
             for (int i = 0; i < 3; i++) {
                 if (m.bc_type[6 * f + 3 * order + i] != 0) continue;
                 face_dirichlet(m, f, order, i, fm, vals);
                 size_t ia = 0;
-                for (int x = 0; x < m.nb[fm.dim[0]]; x++)
-                    for (int y = 0; y < m.nb[fm.dim[1]]; y++) {
+                for (int x = 0; x < m.nown[fm.dim[0]]; x++)
+                    for (int y = 0; y < m.nown[fm.dim[1]]; y++) {
                         int64_t r = 3 * (fm.stride[0] * x + fm.stride[1] * y + fm.stride[2] * iz) + i;
                         mask[r] = 1;
                         g[r] = vals[ia++];
+                    }
+            }
+        }
+    }
+    // ghost nodes of the window on the same faces: Dirichlet rows of the ranks
+    // owning them (the same face and flags; they set those rows)
+    for (int ib = 0; ib < m.nboun; ib++) {
+        int f = m.boun[ib];
+        FaceMap fm = face_map(f, m.nb, m.nown);
+        for (int order = 0; order < 2; order++) {
+            int iz = order == 0 ? fm.iz0 : fm.iz1;
+            for (int i = 0; i < 3; i++) {
+                if (m.bc_type[6 * f + 3 * order + i] != 0) continue;
+                for (int x = 0; x < m.nb[fm.dim[0]]; x++)
+                    for (int y = 0; y < m.nb[fm.dim[1]]; y++) {
+                        if (x < m.nown[fm.dim[0]] && y < m.nown[fm.dim[1]]) continue;
+                        int64_t r = 3 * (fm.stride[0] * x + fm.stride[1] * y + fm.stride[2] * iz) + i;
+                        if (!mask[r]) mask[r] = 2;
                     }
             }
         }
@@ -206,7 +226,7 @@ std::vector<double> neumann_load(const Mesh &m, int nquad, igap4h_neumann_fn hoo
     gauss01(nquad, cq, wq);
     for (int ib = 0; ib < m.nboun; ib++) {
         int f = m.boun[ib];
-        FaceMap fm = face_map(f, m.nb);
+        FaceMap fm = face_map(f, m.nb, m.nown);
         int d0 = fm.dim[0], d1 = fm.dim[1], d2 = fm.dim[2];
         const double *k0 = m.knot[d0].data(), *k1 = m.knot[d1].data(), *k2 = m.knot[d2].data();
         int nk0 = (int)m.knot[d0].size(), nk1 = (int)m.knot[d1].size(), nk2 = (int)m.knot[d2].size();
@@ -262,7 +282,7 @@ double neumann_measure(const Mesh &m, int nquad)
     gauss01(nquad, cq, wq);
     for (int ib = 0; ib < m.nboun; ib++) {
         int f = m.boun[ib];
-        FaceMap fm = face_map(f, m.nb);
+        FaceMap fm = face_map(f, m.nb, m.nown);
         int d0 = fm.dim[0], d1 = fm.dim[1];
         const double *k0 = m.knot[d0].data(), *k1 = m.knot[d1].data();
         for (int order = 0; order < 2; order++)

@@ -11,7 +11,9 @@ namespace igap4h {
 
 struct Mesh {
     int ne[3];                    // elements per direction
-    int nb[3];                    // basis functions per direction (ne + 2)
+    int nb[3];                    // basis functions per direction of the (local) window: ne + 2
+    int nown[3];                  // basis functions owned (Nurbs.nbasis): nb on one rank; ne on a
+                                  // rank that is not the last part of its direction
     std::vector<double> knot[3];  // ne + 5 knots each
     int nboun;
     int boun[6];
@@ -32,12 +34,15 @@ double evalN(const double *kv, int nk, int k, int i, int p, double xi);
 // Number of AIJ nonzeros (snesiga_init.c:341-359 row counts summed).
 int64_t pattern_nnz(const int nb[3]);
 
-// Dirichlet row list in the order of snesiga_Tangent.c:214-267.
+// Dirichlet row list in the order of snesiga_Tangent.c:214-267 (local
+// window numbering): per face in boun, flag and dof, the owned face nodes.
 std::vector<int32_t> dirichlet_rows(const Mesh &m);
 
 // Final Dirichlet values per row after the INSERT sequence of
 // snesiga_Residual.c:331-409 (later faces overwrite shared rows):
-// mask[r] = 1 and g[r] = value for every row in the list.
+// mask[r] = 1 and g[r] = value for every row in the list; mask[r] = 2 for
+// the other face nodes of the window on those faces (ghost nodes, whose
+// Dirichlet rows the ranks owning them set).
 void dirichlet_values(const Mesh &m, std::vector<unsigned char> &mask, std::vector<double> &g);
 
 // Neumann load vector (snesiga_Residual.c:188-320); empty if hook is NULL or

@@ -2300,6 +2300,8 @@ extern "C" int igap4h_spmv_dev(igap4h_ctx *c, const void *d_vals, int vals_type,
 extern "C" int igap4h_linear_solve_dev(igap4h_ctx *c, const void *d_vals, int vals_type, const double *d_b, double *d_x,
                                        const igap4h_newton_opts *o, int *its, double *relres, void *stream)
 {
+    if (c && c->partitioned)
+        return fail(IGAP4H_EINVAL, "the Newton-Krylov driver solves whole patches; a partitioned block only assembles");
     if (!c || !d_vals || !d_b || !d_x || !its || !relres) return fail(IGAP4H_EINVAL, "NULL argument");
     int rc = check_opts(o);
     if (rc) return rc;
@@ -2322,6 +2324,8 @@ extern "C" int igap4h_linear_solve_dev(igap4h_ctx *c, const void *d_vals, int va
 extern "C" int igap4h_newton_solve_dev(igap4h_ctx *c, double *d_Uh, const igap4h_newton_opts *o, igap4h_newton_info *info,
                                        void *stream)
 {
+    if (c && c->partitioned)
+        return fail(IGAP4H_EINVAL, "the Newton-Krylov driver solves whole patches; a partitioned block only assembles");
     if (!c || !d_Uh || !info) return fail(IGAP4H_EINVAL, "NULL argument");
     int rc = check_opts(o);
     if (rc) return rc;

@@ -50,24 +50,30 @@ class CSR:
 
 
 class App:
-    """IGA application context (include/iga.h:8-25) bound to a device context."""
+    """IGA application context (include/iga.h:8-25) bound to a device context.
 
-    def __init__(self, pb: Problem, physics=None):
+    rank: a partition.RankBlock to make the context of that rank's block of
+    the partitioned patch pb (its local elements, knot window, owned basis
+    functions, boundary faces, selfIdx/recvIdx/globalIdx, the coefficients of
+    its elements); its device vectors are in the local ghosted numbering."""
+
+    def __init__(self, pb: Problem, physics=None, rank=None):
         self.pb = pb
+        self.rank = rank
         lib = L.lib()
         nh = pb.nhist
-        nb = pb.nbasis
+        nb = pb.nbasis if rank is None else rank.nbasis
         # --- Phys
         self._par = np.ascontiguousarray(pb.par_mat, dtype=np.float64)
         self.phys = L.Phys(ndim=3, ndof=3, torder=pb.torder, sorder=2, par_mat=_dp(self._par),
                            ndensity=1, nfield=0)
-        # --- Nurbs (one rank, non-periodic)
-        self._nelem = np.array(pb.nelem, dtype=np.int32)
+        # --- Nurbs (non-periodic; one rank owning the patch, or one rank's block)
+        self._nelem = np.array(pb.nelem if rank is None else rank.nelem, dtype=np.int32)
         self._nbasis = np.array(nb, dtype=np.int32)
-        self._knots = [np.ascontiguousarray(k, dtype=np.float64) for k in pb.knots]
+        self._knots = [np.ascontiguousarray(k, dtype=np.float64) for k in (pb.knots if rank is None else rank.knots)]
         self._nknot = np.array([len(k) for k in self._knots], dtype=np.int32)
         self._kv = (L.c_double_p * 3)(*[_dp(k) for k in self._knots])
-        self._boun = np.array(pb.boun, dtype=np.int32)
+        self._boun = np.array(pb.boun if rank is None else [f for f in rank.boun if f in pb.boun], dtype=np.int32)
         self.nurbs = L.Nurbs(porder=2, nelem=_ip(self._nelem), nbasis=_ip(self._nbasis), nknot=_ip(self._nknot),
                              knotVector=ctypes.cast(self._kv, ctypes.POINTER(L.c_double_p)),
                              nboun=len(self._boun), boun=_ip(self._boun))
@@ -80,13 +86,35 @@ class App:
         self.bc = L.BC(bc_type=_ip(self._bc_type), par_periodic=_dp(self._pp), par_dirichlet=_dp(self._pd),
                        par_neumann=_dp(self._pn))
         # --- IGA
-        self.ndof = pb.ndof
-        self.U_hist = np.zeros(nh * self.ndof)
-        self._gidx = np.arange(self.ndof, dtype=np.int32)
-        self.iga = L.IGA(phys=ctypes.pointer(self.phys), nurbs=ctypes.pointer(self.nurbs), bc=ctypes.pointer(self.bc),
-                         U_hist=_dp(self.U_hist), nselfIdx=self.ndof, globalIdx=_ip(self._gidx))
+        if rank is None:
+            self.ndof = pb.ndof
+            self.U_hist = np.zeros(nh * self.ndof)
+            self._gidx = np.arange(self.ndof, dtype=np.int32)
+            self.iga = L.IGA(phys=ctypes.pointer(self.phys), nurbs=ctypes.pointer(self.nurbs),
+                             bc=ctypes.pointer(self.bc), U_hist=_dp(self.U_hist), nselfIdx=self.ndof,
+                             globalIdx=_ip(self._gidx))
+        else:
+            self.ndof = rank.nlocal                          # the local ghosted window
+            self.U_hist = np.zeros(nh * rank.nself)
+            self._gidx = np.ascontiguousarray(rank.globalIdx, dtype=np.int32)
+            self._self = np.ascontiguousarray(rank.selfIdx, dtype=np.int32)
+            self._send = np.ascontiguousarray(rank.sendIdx, dtype=np.int32)
+            self._sptr = np.ascontiguousarray(rank.sendPtr, dtype=np.int32)
+            self._recv = np.ascontiguousarray(rank.recvIdx, dtype=np.int32)
+            self._rptr = np.ascontiguousarray(rank.recvPtr, dtype=np.int32)
+            self.iga = L.IGA(phys=ctypes.pointer(self.phys), nurbs=ctypes.pointer(self.nurbs),
+                             bc=ctypes.pointer(self.bc), U_hist=_dp(self.U_hist), nselfIdx=rank.nself,
+                             selfIdx=_ip(self._self), nsendIdx=len(self._send), sendIdx=_ip(self._send),
+                             sendPtr=_ip(self._sptr), nrecvIdx=len(self._recv), recvIdx=_ip(self._recv),
+                             recvPtr=_ip(self._rptr), globalIdx=_ip(self._gidx))
         # --- options
-        self._coef = None if pb.elem_coef is None else np.ascontiguousarray(pb.elem_coef, dtype=np.float64)
+        coef = pb.elem_coef
+        if coef is not None and rank is not None:       # the rank's elements, in its local order
+            ne = pb.nelem
+            c3 = np.asarray(coef, dtype=np.float64).reshape(ne[0], ne[1], ne[2], 6)
+            d, n = rank.displ, rank.nelem
+            coef = c3[d[0]:d[0] + n[0], d[1]:d[1] + n[1], d[2]:d[2] + n[2]].reshape(-1, 6)
+        self._coef = None if coef is None else np.ascontiguousarray(coef, dtype=np.float64)
         if pb.neumann is not None:
             fn = pb.neumann
 

@@ -17,9 +17,28 @@
  * include/iga.h:8-25) under the reference's include guards, so this header
  * can be included before or after the reference's own headers.
  *
- * Single GPU, single rank: globalIdx must be the identity and nbasis =
- * nelem + 2 in every direction (one MPI rank owns the whole patch).
- * Periodic directions are not supported.
+ * One rank owning the whole patch (nbasis = nelem + 2, globalIdx the
+ * identity), or one rank's block of a partitioned patch as nurbs_init and
+ * snesiga_init build it for mpiexec -n P (mesh/nurbs/nurbs_init.c:9-199,
+ * src/snesiga_init.c:80-339): local elements and knot window, nbasis =
+ * nelem (nelem + 2 on the last part of a direction) owned basis functions,
+ * the boundary faces the rank touches, selfIdx / recvIdx placing the owned
+ * and the received (ghost) basis functions in the local window of
+ * (nelem + 2)^3, globalIdx the PETSc rows of the window.  A block's device
+ * vectors (U_hist slices, R) are in that local ghosted numbering (nselfIdx +
+ * nrecvIdx entries per slice, as the callbacks build U after the exchange,
+ * snesiga_Residual.c:53-74) and its CSR pattern is the window's; R and the
+ * values are the rank's contributions (its elements' integrals and Neumann
+ * terms on every local row, owned or ghost, as its VecSetValues /
+ * MatSetValues(ADD) put them), with its own Dirichlet rows set (R = U - g;
+ * rows and columns zeroed, 1 on the diagonal) and the Dirichlet rows of the
+ * other ranks in the window zeroed: the ranks' contributions summed through
+ * globalIdx are the assembled R and P after VecAssembly / MatAssembly and
+ * MatZeroRowsColumns.  igap4h_dirichlet_rows gives the rank's row list in
+ * global rows (globalIdx), as row_dirichlet.  Blocks take the device entry
+ * points (and energy / intdensity: the rank's part before MPI_Allreduce);
+ * the host-array entry points, the Newton-Krylov driver and the batch take
+ * whole patches.  Periodic directions are not supported.
  *
  * Status codes: 0 = success, negative = error (igap4h_last_error() has the
  * message).  No entry point exits the process.
