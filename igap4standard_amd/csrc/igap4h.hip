@@ -2071,40 +2071,52 @@ __global__ __launch_bounds__(64) void k_residual_batch(const BatchInst *__restri
     residual_elem_body<0>(m, b.Uh, b.coef, b.Re, e, e);
 }
 
-// blockIdx.y = instance; grid-stride over its dofs
-__global__ __launch_bounds__(256) void k_gather_batch(const BatchInst *__restrict__ I)
+// The per-dof and per-value passes of a batch run on one 1D grid whose
+// blocks are split among the instances by their own sizes (block offsets
+// off[0..n], instance = batch_find): no block is spent on the largest
+// instance's size for every instance.
+__global__ __launch_bounds__(256) void k_gather_batch(const BatchInst *__restrict__ I, const long long *__restrict__ goff, int n)
 {
-    const BatchInst &b = I[blockIdx.y];
-    const MeshArgs &m = b.m;
-    for (long long dof = (long long)blockIdx.x * blockDim.x + threadIdx.x; dof < m.ndof; dof += (long long)gridDim.x * blockDim.x)
-        gather_body(m, b.Re, b.fneu, b.mask, b.g, b.Uh, b.R, dof);
+    const int i = batch_find(goff, n, blockIdx.x);
+    const BatchInst &b = I[i];
+    const long long dof = ((long long)blockIdx.x - goff[i]) * 256 + threadIdx.x;
+    if (dof < b.m.ndof) gather_body(b.m, b.Re, b.fneu, b.mask, b.g, b.Uh, b.R, dof);
 }
 
-__global__ void k_fill_zero_batch(const BatchInst *__restrict__ I, int vals_f32)
+constexpr int kFillWords = 8 * 256;  // 32-bit words zeroed per block of k_fill_zero_batch
+
+__global__ __launch_bounds__(256) void k_fill_zero_batch(const BatchInst *__restrict__ I, const long long *__restrict__ foff,
+                                                         int n, int vals_f32)
 {
-    const BatchInst &b = I[blockIdx.y];
+    const int i = batch_find(foff, n, blockIdx.x);
+    const BatchInst &b = I[i];
     const long long n32 = vals_f32 ? b.nnz : 2 * b.nnz;
     unsigned int *p = (unsigned int *)b.vals;
-    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n32; k += (long long)gridDim.x * blockDim.x) p[k] = 0u;
+    const long long base = ((long long)blockIdx.x - foff[i]) * kFillWords + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if (base + 256 * k < n32) p[base + 256 * k] = 0u;
 }
 
-__global__ void k_dirichlet_diag_batch(const BatchInst *__restrict__ I, int vals_f32)
+__global__ __launch_bounds__(256) void k_dirichlet_diag_batch(const BatchInst *__restrict__ I, const long long *__restrict__ doff,
+                                                              int n, int vals_f32)
 {
-    const BatchInst &b = I[blockIdx.y];
+    const int i = batch_find(doff, n, blockIdx.x);
+    const BatchInst &b = I[i];
     const MeshArgs &m = b.m;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < b.ndrows; t += gridDim.x * blockDim.x) {
-        long long r = b.drows[t];
-        long long node = r / 3;
-        int i = (int)(r % 3);
-        int a2 = (int)(node % m.nb[2]);
-        int a1 = (int)((node / m.nb[2]) % m.nb[1]);
-        int a0 = (int)(node / ((long long)m.nb[2] * m.nb[1]));
-        int lo0 = max(0, a0 - 2), lo1 = max(0, a1 - 2), lo2 = max(0, a2 - 2);
-        int w1 = min(m.nb[1] - 1, a1 + 2) - lo1 + 1, w2 = min(m.nb[2] - 1, a2 + 2) - lo2 + 1;
-        long long pos = b.rowptr[r] + 3LL * (((a0 - lo0) * w1 + (a1 - lo1)) * w2 + (a2 - lo2)) + i;
-        if (vals_f32) ((float *)b.vals)[pos] = 1.0f;
-        else ((double *)b.vals)[pos] = 1.0;
-    }
+    const long long t = ((long long)blockIdx.x - doff[i]) * 256 + threadIdx.x;
+    if (t >= b.ndrows) return;
+    long long r = b.drows[t];
+    long long node = r / 3;
+    int ic = (int)(r % 3);
+    int a2 = (int)(node % m.nb[2]);
+    int a1 = (int)((node / m.nb[2]) % m.nb[1]);
+    int a0 = (int)(node / ((long long)m.nb[2] * m.nb[1]));
+    int lo0 = max(0, a0 - 2), lo1 = max(0, a1 - 2), lo2 = max(0, a2 - 2);
+    int w1 = min(m.nb[1] - 1, a1 + 2) - lo1 + 1, w2 = min(m.nb[2] - 1, a2 + 2) - lo2 + 1;
+    long long pos = b.rowptr[r] + 3LL * (((a0 - lo0) * w1 + (a1 - lo1)) * w2 + (a2 - lo2)) + ic;
+    if (vals_f32) ((float *)b.vals)[pos] = 1.0f;
+    else ((double *)b.vals)[pos] = 1.0;
 }
 
 // colour (cx, cy, cz) of every instance; coff = this colour's element offsets
@@ -2128,10 +2140,11 @@ struct igap4h_batch {
     int n = 0, prec = 0, vals_f32 = 0, device = 0;
     std::vector<BatchInst> h_inst;
     BatchInst *d_inst = nullptr;
-    long long *d_off = nullptr;             // [eoff (n+1) | coff 27 x (n+1)]
+    long long *d_off = nullptr;             // [eoff | coff (27) | goff | foff | doff], n + 1 each
     std::vector<long long> eoff, coff;      // host copies
-    long long maxdof = 0, maxnnz = 0;
-    int maxnd = 0;
+    // block offsets per instance of the gather, fill and Dirichlet passes
+    // (on the device after eoff and coff: d_off + 28 (n + 1), + 29 (n + 1), + 30 (n + 1))
+    std::vector<long long> goff, foff, doff;
     int qc = 0;                             // chosen variant
     hipStream_t cap = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -2165,11 +2178,11 @@ static int batch_refresh(igap4h_batch *b)
             changed = true;
         }
     if (!changed) return 0;
-    b->maxnd = 0;
-    for (int i = 0; i < b->n; i++) b->maxnd = std::max(b->maxnd, b->h_inst[i].ndrows);
+    for (int i = 0; i < b->n; i++) b->doff[i + 1] = b->doff[i] + (b->h_inst[i].ndrows + 255) / 256;
     batch_drop_graph(b);
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(b->d_inst, b->h_inst.data(), sizeof(BatchInst) * b->n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->d_off + (size_t)30 * (b->n + 1), b->doff.data(), sizeof(long long) * (b->n + 1), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -2203,20 +2216,21 @@ static int batch_tangent(igap4h_batch *b, int qc, hipStream_t st)
 
 static int batch_sequence(igap4h_batch *b, hipStream_t st)
 {
-    const unsigned n = (unsigned)b->n;
-    hipLaunchKernelGGL(k_fill_zero_batch, dim3((unsigned)std::min<long long>(1024, (b->maxnnz + 255) / 256), n), dim3(256), 0, st,
-                       (const BatchInst *)b->d_inst, b->vals_f32);
+    const int n = b->n;
+    const long long *off = b->d_off;
+    hipLaunchKernelGGL(k_fill_zero_batch, dim3((unsigned)b->foff[n]), dim3(256), 0, st, (const BatchInst *)b->d_inst,
+                       off + (size_t)29 * (n + 1), n, b->vals_f32);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_residual_batch, dim3((unsigned)b->eoff[b->n]), dim3(64), 0, st, (const BatchInst *)b->d_inst,
-                       (const long long *)b->d_off, b->n);
+    hipLaunchKernelGGL(k_residual_batch, dim3((unsigned)b->eoff[n]), dim3(64), 0, st, (const BatchInst *)b->d_inst, off, n);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_gather_batch, dim3((unsigned)((b->maxdof + 255) / 256), n), dim3(256), 0, st, (const BatchInst *)b->d_inst);
+    hipLaunchKernelGGL(k_gather_batch, dim3((unsigned)b->goff[n]), dim3(256), 0, st, (const BatchInst *)b->d_inst,
+                       off + (size_t)28 * (n + 1), n);
     HIPCHK(hipGetLastError());
     int rc = batch_tangent(b, b->qc, st);
     if (rc) return rc;
-    if (b->maxnd > 0) {
-        hipLaunchKernelGGL(k_dirichlet_diag_batch, dim3((unsigned)((b->maxnd + 255) / 256), n), dim3(256), 0, st,
-                           (const BatchInst *)b->d_inst, b->vals_f32);
+    if (b->doff[n] > 0) {
+        hipLaunchKernelGGL(k_dirichlet_diag_batch, dim3((unsigned)b->doff[n]), dim3(256), 0, st, (const BatchInst *)b->d_inst,
+                           off + (size_t)30 * (n + 1), n, b->vals_f32);
         HIPCHK(hipGetLastError());
     }
     return 0;
@@ -2253,6 +2267,9 @@ extern "C" int igap4h_batch_create(igap4h_ctx *const *ctxs, int n, const double 
     b->qc = prec == IGAP4H_FP32 ? GeQC<float>::v : GeQC<double>::v;
     b->eoff.assign(n + 1, 0);
     b->coff.assign((size_t)27 * (n + 1), 0);
+    b->goff.assign(n + 1, 0);
+    b->foff.assign(n + 1, 0);
+    b->doff.assign(n + 1, 0);
     for (int i = 0; i < n; i++) {
         igap4h_ctx *c = ctxs[i];
         if (!c || !d_U_hist[i] || !d_R[i] || !d_vals[i]) {
@@ -2299,20 +2316,23 @@ extern "C" int igap4h_batch_create(igap4h_ctx *const *ctxs, int n, const double 
             long long cnt = (mx > 0 && my > 0 && mz > 0) ? mx * my * mz : 0;
             b->coff[(size_t)col * (n + 1) + i + 1] = b->coff[(size_t)col * (n + 1) + i] + cnt;
         }
-        b->maxdof = std::max<long long>(b->maxdof, c->mesh.ndof());
-        b->maxnnz = std::max<long long>(b->maxnnz, c->nnz);
-        b->maxnd = std::max<int>(b->maxnd, (int)c->drows.size());
+        b->goff[i + 1] = b->goff[i] + (c->mesh.ndof() + 255) / 256;
+        b->foff[i + 1] = b->foff[i] + ((b->vals_f32 ? c->nnz : 2 * c->nnz) + kFillWords - 1) / kFillWords;
+        b->doff[i + 1] = b->doff[i] + ((long long)c->drows.size() + 255) / 256;
     }
-    if (b->eoff[n] > 0x7fffffffLL) {
+    if (b->eoff[n] > 0x7fffffffLL || b->foff[n] > 0x7fffffffLL) {
         igap4h_batch_destroy(b);
         return fail(IGAP4H_EINVAL, "batch has %lld elements; at most 2^31 - 1", b->eoff[n]);
     }
     hipError_t e;
     if ((e = hipSetDevice(b->device)) != hipSuccess || (e = dev_malloc((void **)&b->d_inst, sizeof(BatchInst) * n)) != hipSuccess ||
-        (e = dev_malloc((void **)&b->d_off, sizeof(long long) * 28 * (n + 1))) != hipSuccess ||
+        (e = dev_malloc((void **)&b->d_off, sizeof(long long) * 31 * (n + 1))) != hipSuccess ||
         (e = hipMemcpy(b->d_inst, b->h_inst.data(), sizeof(BatchInst) * n, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(b->d_off, b->eoff.data(), sizeof(long long) * (n + 1), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(b->d_off + (n + 1), b->coff.data(), sizeof(long long) * 27 * (n + 1), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->d_off + (size_t)28 * (n + 1), b->goff.data(), sizeof(long long) * (n + 1), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->d_off + (size_t)29 * (n + 1), b->foff.data(), sizeof(long long) * (n + 1), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->d_off + (size_t)30 * (n + 1), b->doff.data(), sizeof(long long) * (n + 1), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking)) != hipSuccess) {
         igap4h_batch_destroy(b);
         return fail(IGAP4H_EHIP, "batch set-up: %s", hipGetErrorString(e));
