@@ -143,14 +143,27 @@ static LineGeom line_geom(const int nb[3])
 }
 static size_t line_lds_bytes(const LineGeom &g) { return (size_t)25 * (g.seglen + 4) * 3 * sizeof(double); }
 
-// stage x; returns the segment's node range.  sx[(l0*5 + l1) * SW3 + (z - zb) * 3 + j]
+// sum of the stencil widths w2(z') = min(n - 1, z' + 2) - max(0, z' - 2) + 1
+// over z' < z, for n >= 3 nodes in direction 2: the row offset of node z of
+// a line from the line's first node is 9 w0 w1 W(z), with w0, w1 the line's
+// widths in directions 0 and 1 (three rows of 3 w0 w1 w2 entries per node)
+__device__ __forceinline__ long long line_W(int z, int n)
+{
+    const long long m = min(z, n - 2), zz = z;
+    const long long lo = z >= 3 ? (zz - 3) * (zz - 2) / 2 : 0;
+    return m * (m - 1) / 2 + 3 * m + (zz - m) * n - lo;
+}
+
+// stage x; returns the segment's node range and the row offset of the
+// line's first node.  sx[(l0*5 + l1) * SW3 + (z - zb) * 3 + j]
 __device__ __forceinline__ void line_stage(const SpArgs &s, const LineGeom g, long long blk, const double *__restrict__ x,
-                                           double *sx, int &a0, int &a1, int &z0, int &z1)
+                                           double *sx, int &a0, int &a1, int &z0, int &z1, long long &rbase)
 {
     const int seg = (int)(blk % g.nseg);
     const long long line = blk / g.nseg;
     a1 = (int)(line % s.nb[1]);
     a0 = (int)(line / s.nb[1]);
+    rbase = s.rowptr[3 * line * s.nb[2]];  // in flight while x is staged
     z0 = seg * g.seglen;
     z1 = min(s.nb[2], z0 + g.seglen);
     const int zb = z0 - 2, SW3 = 3 * (g.seglen + 4);
@@ -198,9 +211,13 @@ __global__ __launch_bounds__(LW * 64) void k_spmv_line(SpArgs s, LineGeom g, con
 {
     extern __shared__ double sx[];
     __shared__ double sd[LW];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // the wave index in an SGPR: the node loop, row addresses and stencil
+    // widths are then scalar work
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     int a0, a1, z0, z1;
-    line_stage(s, g, xcd_remap(blockIdx.x, gridDim.x), x, sx, a0, a1, z0, z1);
+    long long rbase;
+    line_stage(s, g, xcd_remap(blockIdx.x, gridDim.x), x, sx, a0, a1, z0, z1, rbase);
+    const long long w01 = 9LL * (min(s.nb[0] - 1, a0 + 2) - max(0, a0 - 2) + 1) * (min(s.nb[1] - 1, a1 + 2) - max(0, a1 - 2) + 1);
     const int zb = z0 - 2, SW3 = 3 * (g.seglen + 4);
     // LDS index of the lane's columns for a node whose 5 x 5 x 5 box lies
     // inside the mesh, at z = z0 (+ 3 (z - z0) for the others)
@@ -216,7 +233,7 @@ __global__ __launch_bounds__(LW * 64) void k_spmv_line(SpArgs s, LineGeom g, con
         int a[3], lo[3], w[3];
         line_box(s, a0, a1, z, a, lo, w);
         const long long node = ((long long)a0 * s.nb[1] + a1) * s.nb[2] + z;
-        const VT *row = vals + s.rowptr[3 * node];
+        const VT *row = vals + rbase + w01 * line_W(z, s.nb[2]);
         const int L = 3 * w[0] * w[1] * w[2];
         int cx[6];
         if (in01 && z >= 2 && z + 2 < s.nb[2]) {
