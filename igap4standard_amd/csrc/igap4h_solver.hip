@@ -37,22 +37,34 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// the three wave sums of s0, s1, s2, returned in every lane: a transposed
-// reduction (7 exchanges instead of 18; used by the line SpMV, where it saves
-// 4 % of the FP32 pass — in the coarse-level row kernels it cost time).  After the xor-32 step the lower half
-// carries (s0, s1), the upper half (s2, 0); after xor-16 each quarter carries
-// one of s0, s1, s2, 0; four more steps sum within the quarters, and lanes 16
-// and 32 hand over s1 and s2.
-__device__ __forceinline__ void wave_sum3(double &s0, double &s1, double &s2, int lane)
+// v of lane 15 - l (row_mirror) or 7 - l (row_half_mirror) of the lane's
+// 16-lane row, or of lane l ^ 2, l ^ 1 (quad_perm): DPP moves, no LDS.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+constexpr int DPP_MIRROR = 0x140, DPP_HALF_MIRROR = 0x141, DPP_XOR2 = 0x4E, DPP_XOR1 = 0xB1;
+
+// The three wave sums of s0, s1, s2, transposed: after the exchange over 32
+// lanes the lower half carries (s0, s1), the upper half (s2, 0); after the
+// one over 16 each 16-lane row carries one of s0, s1, s2, 0, which four DPP
+// steps (mirror, half mirror, xor 2, xor 1: every lane of the row gathers
+// all 16 once) sum within the row.  Returns the sum of s_q in every lane of
+// row q = lane / 16 (row 3: 0).  2 LDS exchanges instead of 18.
+__device__ __forceinline__ double wave_sum3(double s0, double s1, double s2, int lane)
 {
     const bool h5 = lane & 32, h4 = lane & 16;
     const double r0 = __shfl_xor(h5 ? s0 : s2, 32, 64), r1 = __shfl_xor(h5 ? s1 : 0.0, 32, 64);
     const double A = (h5 ? s2 : s0) + r0, B = (h5 ? 0.0 : s1) + r1;
     double c = (h4 ? B : A) + __shfl_xor(h4 ? A : B, 16, 64);
-    for (int o = 8; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-    s0 = __shfl(c, 0, 64);
-    s1 = __shfl(c, 16, 64);
-    s2 = __shfl(c, 32, 64);
+    c += dpp_f64<DPP_MIRROR>(c);
+    c += dpp_f64<DPP_HALF_MIRROR>(c);
+    c += dpp_f64<DPP_XOR2>(c);
+    c += dpp_f64<DPP_XOR1>(c);
+    return c;
 }
 
 // node -> grid coordinates and the stencil box [lo, lo + w) per direction
@@ -265,17 +277,16 @@ __global__ __launch_bounds__(LW * 64) void k_spmv_line(SpArgs s, LineGeom g, con
             s1 += (double)v1[it] * xv[it];
             s2 += (double)v2[it] * xv[it];
         }
-        wave_sum3(s0, s1, s2, lane);
-        if (lane == 0) {
-            const double acc[3] = {s0, s1, s2};
-            for (int i = 0; i < 3; i++) {
-                const double v = RES ? b[3 * node + i] - acc[i] : acc[i];
-                y[3 * node + i] = v;
-                if (DOT) dsum += x[3 * node + i] * v;
-            }
+        const double c = wave_sum3(s0, s1, s2, lane);
+        if ((lane & 15) == 0 && lane < 48) {  // lanes 0, 16, 32: rows 3 node + 0, 1, 2
+            const long long r = 3 * node + (lane >> 4);
+            const double v = RES ? b[r] - c : c;
+            y[r] = v;
+            if (DOT) dsum += x[r] * v;
         }
     }
     if (DOT) {
+        dsum = wave_sum(dsum);
         if (lane == 0) sd[wv] = dsum;
         __syncthreads();
         if (threadIdx.x == 0) {
