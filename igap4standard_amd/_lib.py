@@ -63,6 +63,7 @@ IGAP4H_PC_NONE, IGAP4H_PC_BJACOBI3, IGAP4H_PC_SGS, IGAP4H_PC_MG = 0, 1, 2, 3
 IGAP4H_KSP_GMRES, IGAP4H_KSP_CG = 0, 1
 IGAP4H_MG_GS, IGAP4H_MG_CHEBYSHEV = 0, 1
 IGAP4H_SMOOTH_SAME, IGAP4H_SMOOTH_BF16, IGAP4H_SMOOTH_F16 = 0, 1, 2
+IGAP4H_TANGENT_AUTO, IGAP4H_TANGENT_TSTRIP, IGAP4H_TANGENT_SV = 0, 1, 2
 
 
 class NewtonOpts(ctypes.Structure):
@@ -100,7 +101,8 @@ SYMBOLS = ("igap4h_create", "igap4h_destroy", "igap4h_last_error", "igap4h_sizes
            "igap4h_newton_default_opts", "igap4h_newton_solve_dev", "igap4h_spmv_dev", "igap4h_linear_solve_dev",
            "igap4h_residual_vjp_coef_dev", "igap4h_batch_create", "igap4h_batch_destroy", "igap4h_batch_sizes",
            "igap4h_batch_tune", "igap4h_batch_set_variant", "igap4h_batch_assemble", "igap4h_write_solution",
-           "igap4h_read_solution", "igap4h_knot_insert", "igap4h_update_bc", "igap4h_update_neumann")
+           "igap4h_read_solution", "igap4h_knot_insert", "igap4h_update_bc", "igap4h_update_neumann",
+           "igap4h_set_tangent_form")
 
 _lib = None
 
@@ -149,6 +151,7 @@ def lib():
         L.igap4h_batch_assemble.argtypes = [vp, vp]
         L.igap4h_update_bc.argtypes = [vp, ctypes.POINTER(BC)]
         L.igap4h_update_neumann.argtypes = [vp]
+        L.igap4h_set_tangent_form.argtypes = [vp, ctypes.c_int]
         L.igap4h_write_solution.argtypes = [ctypes.c_char_p, c_double_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int]
         L.igap4h_read_solution.argtypes = [ctypes.c_char_p, c_double_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int]
         L.igap4h_knot_insert.argtypes = [c_double_p, c_int_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,

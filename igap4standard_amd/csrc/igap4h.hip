@@ -1114,6 +1114,466 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GtWaves<ACC
     tangent_gt_body<ACC, QC>(m, Uh, coef, mass, rowptr, dmask, vals, vals_f32, ev);
 }
 
+// ---------------------------------------------------------------- strain-variation tangent
+// The same element tangent as tangent_apply, factored through the strain
+// variation of each dof:
+//   K_ij[A,B] = sum_q Y_i(A)^T (w M) Y_j(B) + delta_ij sum_q n(A)^T (w Gamma) n(B)
+// phys/ge: Y_j(B) has 12 components: beta = sym(F_j^T grad n) (normal 00,
+// 11, 22, shear 01, 02, 12) and the trace-free part of nu_IK = H_j,IK n_,I
+// + F_jI n_,IK over I for each K, in the orthonormal basis (1,-1,0)/sqrt2,
+// (1,1,-2)/sqrt6 of trace-free 3-vectors; M = diag(P, 2 e345 I3, 2 le^2 I6)
+// with P = d2Psi/dE_II dE_JJ.  Gamma is the stress form on the gradient and
+// second-derivative slots: grad n(A)^T S grad n(B) + sum tau_IK (n(A)_,I
+// n(B)_,IK + n(B)_,I n(A)_,IK).
+// phys/getime: Y = (Y(F0, H0), Y(F1, H1)), 24 components, from the jets
+// F(s) = F0 + s F1, H(s) = H0 + s H1 along u(s) = ubar + s (u - u0)/2;
+// with the series M(s) = C0 + s C1 + s^2 C2, the series of Y^T M Y
+// truncated at s^2 and averaged as (1/2)(c0/2 + c1/3 + c2/4) is the block
+// form [[M00, M01], [M01, M11]], M00 = C0/4 + C1/6 + C2/8, M01 = C0/6 + C1/8,
+// M11 = C0/8; Gamma takes the averaged S and tau and the mass rho/dt^2 +
+// c/(2 dt) on the value slot (phys/getime/tangent.c:4127-5026).
+// MFMA: rows R = 32 i + A and columns C = 32 j + B (A, B < 27 used) in 16 x
+// 16 tiles (rt, ct), ct >= rt (21 tiles, mirrored into the lower triangle),
+// contracted over the NY components of every Gauss point.  The Gamma
+// operands n(A) and w Gamma n(B) sit in columns 96 + A / 96 + B of the same
+// strips (12 slot rows, the rest zero), and its 3 tiles of the 27 x 27 block
+// are added to the three diagonal blocks: 24 tiles, 6 per wave, one code
+// path for every wave (3 A and 6 B fragments per k-step).
+constexpr int SV_RS = 144;  // k-row stride of the operand strips: 128 + 16, the 4 k-rows of a fragment on distinct banks
+
+template <class ACC, int GT>
+struct SvPoint;
+template <class ACC>
+struct SvPoint<ACC, 0> {
+    ACC F[3][3], H[3][6];
+    ACC P[6];       // w P   (P00 P01 P02 P11 P12 P22)
+    ACC S[6];       // w S   (S00 S11 S22 S01 S02 S12)
+    ACC tau[3][3];  // w tau
+    ACC w;
+};
+template <class ACC>
+struct SvPoint<ACC, 1> {
+    ACC F[3][3], H[3][6], F1[3][3], H1[3][6];
+    ACC P00[6], P01[6], P11[6];  // w M_ab on the normal strains
+    ACC S[6], tau[3][3];          // w x averaged series
+    ACC w;
+};
+
+// the 12 strain-variation components of dof j (row F_j., H_j.) along slots n
+template <class R>
+__device__ __forceinline__ void sv_y(const R F[3], const R H[6], const R n[10], R *y)
+{
+    const R half = R(0.5), s2 = R(IGAP4H_S2), s6 = R(IGAP4H_S6);
+    y[0] = F[0] * n[1];
+    y[1] = F[1] * n[2];
+    y[2] = F[2] * n[3];
+    y[3] = (F[1] * n[1] + F[0] * n[2]) * half;
+    y[4] = (F[2] * n[1] + F[0] * n[3]) * half;
+    y[5] = (F[2] * n[2] + F[1] * n[3]) * half;
+#pragma unroll
+    for (int K = 0; K < 3; K++) {
+        R nu[3];
+#pragma unroll
+        for (int I = 0; I < 3; I++) nu[I] = H[hidx(I, K)] * n[1 + I] + F[I] * n[4 + hidx(I, K)];
+        y[6 + 2 * K] = (nu[0] - nu[1]) * s2;
+        y[7 + 2 * K] = (nu[0] + nu[1] - R(2) * nu[2]) * s6;
+    }
+}
+
+__device__ __forceinline__ void sym3_apply_add(const float *P, const float *y, float *z)
+{
+    z[0] += P[0] * y[0] + P[1] * y[1] + P[2] * y[2];
+    z[1] += P[1] * y[0] + P[3] * y[1] + P[4] * y[2];
+    z[2] += P[2] * y[0] + P[4] * y[1] + P[5] * y[2];
+}
+__device__ __forceinline__ void sym3_apply_add(const double *P, const double *y, double *z)
+{
+    z[0] += P[0] * y[0] + P[1] * y[1] + P[2] * y[2];
+    z[1] += P[1] * y[0] + P[3] * y[1] + P[4] * y[2];
+    z[2] += P[2] * y[0] + P[4] * y[1] + P[5] * y[2];
+}
+
+// Y_j(B) and Z = w M Y_j(B) of one (point, B, j) into k-rows 0..NY-1 of the
+// strips at column 32 j + B (y, z point there); ce = 2 e345, cl = 2 le^2
+template <class ACC>
+__device__ __forceinline__ void sv_item_yz(const SvPoint<ACC, 0> &p, ACC ce, ACC cl, const ACC n[10], int j, ACC *y, ACC *z)
+{
+    ACC Y[12], Z[12];
+    sv_y<ACC>(p.F[j], p.H[j], n, Y);
+    const ACC we = p.w * ce, wl = p.w * cl;
+    Z[0] = Z[1] = Z[2] = ACC(0);
+    sym3_apply_add(p.P, Y, Z);
+#pragma unroll
+    for (int k = 3; k < 6; k++) Z[k] = we * Y[k];
+#pragma unroll
+    for (int k = 6; k < 12; k++) Z[k] = wl * Y[k];
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+        y[k * SV_RS] = Y[k];
+        z[k * SV_RS] = Z[k];
+    }
+}
+template <class ACC>
+__device__ __forceinline__ void sv_item_yz(const SvPoint<ACC, 1> &p, ACC ce, ACC cl, const ACC n[10], int j, ACC *y, ACC *z)
+{
+    ACC Y[24], Z[24];
+    sv_y<ACC>(p.F[j], p.H[j], n, Y);
+    sv_y<ACC>(p.F1[j], p.H1[j], n, Y + 12);
+#pragma unroll
+    for (int k = 0; k < 3; k++) Z[k] = Z[12 + k] = ACC(0);
+    sym3_apply_add(p.P00, Y, Z);
+    sym3_apply_add(p.P01, Y + 12, Z);
+    sym3_apply_add(p.P01, Y, Z + 12);
+    sym3_apply_add(p.P11, Y + 12, Z + 12);
+    const ACC q4 = ACC(0.25), q6 = ACC(1.0 / 6.0), q8 = ACC(0.125);
+#pragma unroll
+    for (int k = 3; k < 12; k++) {
+        const ACC cw = p.w * (k < 6 ? ce : cl);
+        const ACC a00 = cw * q4, a01 = cw * q6, a11 = cw * q8;
+        Z[k] = a00 * Y[k] + a01 * Y[12 + k];
+        Z[12 + k] = a01 * Y[k] + a11 * Y[12 + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 24; k++) {
+        y[k * SV_RS] = Y[k];
+        z[k * SV_RS] = Z[k];
+    }
+}
+
+// n(B) and w Gamma n(B) on the 12 Gamma rows (ge: slots 1..9, getime: 0..9,
+// the rest zero) at column B (ga, gb point there); wmass = w x mass
+template <class ACC, int GT>
+__device__ __forceinline__ void sv_item_gamma(const SvPoint<ACC, GT> &p, ACC wmass, const ACC n[10], ACC *ga, ACC *gb)
+{
+    const ACC *S = p.S;
+    const ACC(&t)[3][3] = p.tau;
+    ACC g[10];
+    g[0] = wmass * n[0];
+    g[1] = S[0] * n[1] + S[3] * n[2] + S[4] * n[3] + t[0][0] * n[4] + t[0][1] * n[5] + t[0][2] * n[6];
+    g[2] = S[3] * n[1] + S[1] * n[2] + S[5] * n[3] + t[1][0] * n[5] + t[1][1] * n[7] + t[1][2] * n[8];
+    g[3] = S[4] * n[1] + S[5] * n[2] + S[2] * n[3] + t[2][0] * n[6] + t[2][1] * n[8] + t[2][2] * n[9];
+    g[4] = t[0][0] * n[1];
+    g[5] = t[0][1] * n[1] + t[1][0] * n[2];
+    g[6] = t[0][2] * n[1] + t[2][0] * n[3];
+    g[7] = t[1][1] * n[2];
+    g[8] = t[1][2] * n[2] + t[2][1] * n[3];
+    g[9] = t[2][2] * n[3];
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+        const int sl = GT ? k : k + 1;
+        ga[k * SV_RS] = sl < 10 ? n[sl] : ACC(0);
+        gb[k * SV_RS] = sl < 10 ? g[sl] : ACC(0);
+    }
+}
+
+// The wave's 6 tiles: A fragments from 3 row offsets (tiles 0-2 use the
+// first, 3-4 the second, 5 the third), B fragments from 6 column offsets;
+// offsets are strip columns (96, 112: the Gamma block).
+//   wave 0: (0,0) (0,1) (0,2) (0,3) (0,4) (0,5)
+//   wave 1: (1,1) (1,2) (1,3) (1,4) (1,5) (5,5)
+//   wave 2: (2,2) (2,3) (2,4) (4,4) (4,5) (2,5)
+//   wave 3: (3,3) (3,4) (3,5) (G0,G0) (G0,G1) (G1,G1)
+struct SvTiles {
+    int ao[3], bo[6];
+};
+__device__ __forceinline__ SvTiles sv_tiles(int wave)
+{
+    SvTiles t;
+    if (wave == 0) t = SvTiles{{0, 0, 0}, {0, 16, 32, 48, 64, 80}};
+    else if (wave == 1) t = SvTiles{{16, 16, 80}, {16, 32, 48, 64, 80, 80}};
+    else if (wave == 2) t = SvTiles{{32, 64, 32}, {32, 48, 64, 64, 80, 80}};
+    else t = SvTiles{{48, 96, 112}, {48, 64, 80, 96, 112, 112}};
+    return t;
+}
+__device__ __forceinline__ int sv_arow(const SvTiles &T, int t) { return T.ao[t < 3 ? 0 : (t < 5 ? 1 : 2)]; }
+
+// the chunk's qcnt points, the NY k-rows of each (y: A strip, z: B strip)
+template <class ACC, int NY>
+__device__ __forceinline__ void sv_mfma_chunk(const ACC *y, const ACC *z, int qcnt, int lane, const SvTiles &T,
+                                              typename V4<ACC>::t acc[6])
+{
+    const int off = (lane >> 4) * SV_RS + (lane & 15);
+    const ACC *ya = y + off, *zb = z + off;
+    for (int ql = 0; ql < qcnt; ql++) {
+#pragma unroll
+        for (int ks = 0; ks < NY / 4; ks++) {
+            const ACC *yk = ya + 4 * ks * SV_RS, *zk = zb + 4 * ks * SV_RS;
+            const ACC a0 = yk[T.ao[0]], a1 = yk[T.ao[1]], a2 = yk[T.ao[2]];
+            ACC b[6];
+#pragma unroll
+            for (int t = 0; t < 6; t++) b[t] = zk[T.bo[t]];
+            acc[0] = mfma16(a0, b[0], acc[0]);
+            acc[1] = mfma16(a0, b[1], acc[1]);
+            acc[2] = mfma16(a0, b[2], acc[2]);
+            acc[3] = mfma16(a1, b[3], acc[3]);
+            acc[4] = mfma16(a1, b[4], acc[4]);
+            acc[5] = mfma16(a2, b[5], acc[5]);
+        }
+        ya += NY * SV_RS;
+        zb += NY * SV_RS;
+    }
+}
+
+// the wave's material tiles into the staged 81 x 81 element matrix (both
+// triangles); GAMMA: its Gamma tiles added to the three diagonal blocks
+template <class ACC, bool GAMMA>
+__device__ __forceinline__ void sv_stage(ACC *sKe, int lane, const SvTiles &T, const typename V4<ACC>::t acc[6])
+{
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+        const int ar = sv_arow(T, t), bc = T.bo[t];
+        if ((ar >= 96) != GAMMA) continue;
+        const int C = bc + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int R = ar + acc_row((ACC)0, lane, r);
+            const int i = (R >> 5) & 3, A = R & 31, j = (C >> 5) & 3, B = C & 31;
+            if (A < 27 && B < 27) {
+                if (GAMMA) {
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        sKe[(3 * A + d) * 81 + 3 * B + d] += acc[t][r];
+                        if (ar != bc) sKe[(3 * B + d) * 81 + 3 * A + d] += acc[t][r];
+                    }
+                } else {
+                    sKe[(3 * A + i) * 81 + 3 * B + j] = acc[t][r];
+                    if (ar != bc) sKe[(3 * B + j) * 81 + 3 * A + i] = acc[t][r];
+                }
+            }
+        }
+    }
+}
+
+template <class ACC, int QC, int GT>
+struct SvDims {
+    static constexpr int NY = GT ? 24 : 12;
+    static constexpr int YSZ = QC * NY * SV_RS;   // one A (or B) strip of a chunk, elements
+    static constexpr int OPB = 2 * YSZ * (int)sizeof(ACC);
+    static constexpr int PTB = 64 * (int)sizeof(SvPoint<ACC, GT>);
+    // prologue use of the operand part: interpolation scratch, U, basis tables
+    static constexpr int PRO = (SF_SCRATCH + 2 * 81 + 3 * 4 * 9) * 8;
+    static constexpr int PART2 = OPB > PRO ? OPB : PRO;
+    static constexpr int KEB = 81 * 81 * (int)sizeof(ACC);
+    static constexpr int REGION = (PTB + PART2) > KEB ? (PTB + PART2) : KEB;
+};
+
+template <class ACC, int QC, int GT>
+__device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double *__restrict__ Uh, const double *__restrict__ coef,
+                                                double mass_d, const long long *__restrict__ rowptr,
+                                                const unsigned char *__restrict__ dmask, void *__restrict__ vals, int vals_f32,
+                                                const int ev[3])
+{
+    using D = SvDims<ACC, QC, GT>;
+    using Pt = SvPoint<ACC, GT>;
+    using V = typename V4<ACC>::t;
+    constexpr int NY = D::NY;
+    // [point data | operand strips]: the interpolation scratch of the
+    // prologue sits in the operand part, the element matrix of the epilogue
+    // at the start
+    __shared__ __attribute__((aligned(16))) unsigned char sRegion[D::REGION];
+    __shared__ ACC sphiA[3 * 4 * 9];
+    __shared__ int sQ[64];
+    __shared__ long long sRowBase[81];
+    __shared__ int sRowSx[81], sRowSy[81];
+    __shared__ unsigned char sMask[81];
+    Pt *sPt = reinterpret_cast<Pt *>(sRegion);
+    ACC *sY = reinterpret_cast<ACC *>(sRegion + D::PTB);
+    ACC *sZ = sY + D::YSZ;
+    // prologue only: ge U / getime (ubar, (u - u0)/2), and the basis tables, past the interpolation scratch
+    double *sU = reinterpret_cast<double *>(sRegion + D::PTB) + SF_SCRATCH;
+    double *sphi = sU + 2 * 81;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const long long e = ((long long)ev[0] * m.ne[1] + ev[1]) * m.ne[2] + ev[2];
+    const int nq = m.nq, nqp = nq * nq * nq;
+
+    if (tid < 81) {
+        int A = tid / 3, i = tid % 3;
+        int a0 = ev[0] + A / 9, a1 = ev[1] + (A / 3) % 3, a2 = ev[2] + A % 3;
+        long long grow = 3 * node_of(m, a0, a1, a2) + i;
+        if constexpr (GT) {
+            // u = slice 0 (t_{n+1}), u1 = slice 1 (t_n), u0 = slice 2 (t_{n-1})
+            const double u = Uh[grow], u1 = Uh[m.ndof + grow], u0 = Uh[2 * m.ndof + grow];
+            sU[tid] = (u0 + u1) / 2.0;
+            sU[81 + tid] = (u - u0) / 2.0;
+        } else {
+            sU[tid] = Uh[grow];
+        }
+        int lo0 = max(0, a0 - 2), lo1 = max(0, a1 - 2), lo2 = max(0, a2 - 2);
+        int w1 = min(m.nb[1] - 1, a1 + 2) - lo1 + 1, w2 = min(m.nb[2] - 1, a2 + 2) - lo2 + 1;
+        sRowBase[tid] = rowptr[grow] + 3LL * (((long long)(ev[0] - lo0) * w1 + (ev[1] - lo1)) * w2 + (ev[2] - lo2));
+        sRowSx[tid] = 3 * w1 * w2;
+        sRowSy[tid] = 3 * w2;
+        sMask[tid] = dmask[grow];
+    } else if (tid >= 128 && tid < 192) {
+        int qp = tid - 128;
+        sQ[qp] = qp < nqp ? ((qp / (nq * nq)) | (((qp / nq) % nq) << 4) | ((qp % nq) << 8)) : 0;
+    }
+    fill_phi(m, ev, sphi, tid, 256);
+    __syncthreads();
+    const Coef c = load_coef(coef, e);
+
+    // ---- per-point data
+    double *sScr = reinterpret_cast<double *>(sRegion + D::PTB);
+    int qx = 0, qy = 0, qz = 0, qi = 0;
+    if (tid < nqp) {
+        const int q = sQ[tid];
+        qx = q & 15; qy = (q >> 4) & 15; qz = (q >> 8) & 15; qi = qx * 16 + qy * 4 + qz;
+    }
+    double ub[30];
+    interp_slots_sf<256>(nq, sU, sphi, sScr, tid);
+    if (tid < nqp) {
+#pragma unroll
+        for (int sl = 0; sl < 30; sl++) ub[sl] = sScr[324 + 864 + sl * 64 + qi];
+    }
+    if constexpr (GT) {
+        __syncthreads();
+        interp_slots_sf<256>(nq, sU + 81, sphi, sScr, tid);
+    }
+    if (tid < nqp) {
+        double w;
+        point_affine(m, ev, qx, qy, qz, ub, &w);
+        Pt &P = sPt[tid];
+        if constexpr (GT) {
+            Jet<3> uj[30];
+#pragma unroll
+            for (int sl = 0; sl < 30; sl++) uj[sl] = Num<Jet<3>>::lin(ub[sl], sScr[324 + 864 + sl * 64 + qi]);
+            Compact<Jet<3>> cp;
+            compact_of<Jet<3>>(uj, c, cp);
+            SvPoint<ACC, 1> &G = reinterpret_cast<SvPoint<ACC, 1> &>(P);
+            for (int a = 0; a < 3; a++) {
+                for (int J = 0; J < 3; J++) {
+                    G.F[a][J] = (ACC)cp.F[a][J].c[0];
+                    G.F1[a][J] = (ACC)cp.F[a][J].c[1];
+                    G.tau[a][J] = (ACC)(w * (cp.tau[a][J].c[0] / 4.0 + cp.tau[a][J].c[1] / 6.0 + cp.tau[a][J].c[2] / 8.0));
+                }
+                for (int h = 0; h < 6; h++) {
+                    G.H[a][h] = (ACC)cp.H[a][h].c[0];
+                    G.H1[a][h] = (ACC)cp.H[a][h].c[1];
+                }
+            }
+            for (int k = 0; k < 6; k++) {
+                const double *pc = cp.P[k].c, *sc = cp.S[k].c;
+                G.P00[k] = (ACC)(w * (pc[0] / 4.0 + pc[1] / 6.0 + pc[2] / 8.0));
+                G.P01[k] = (ACC)(w * (pc[0] / 6.0 + pc[1] / 8.0));
+                G.P11[k] = (ACC)(w * (pc[0] / 8.0));
+                G.S[k] = (ACC)(w * (sc[0] / 4.0 + sc[1] / 6.0 + sc[2] / 8.0));
+            }
+            G.w = (ACC)w;
+        } else {
+            Compact<double> cp;
+            compact_of<double>(ub, c, cp);
+            SvPoint<ACC, 0> &G = reinterpret_cast<SvPoint<ACC, 0> &>(P);
+            for (int a = 0; a < 3; a++) {
+                for (int J = 0; J < 3; J++) {
+                    G.F[a][J] = (ACC)cp.F[a][J];
+                    G.tau[a][J] = (ACC)(w * cp.tau[a][J]);
+                }
+                for (int h = 0; h < 6; h++) G.H[a][h] = (ACC)cp.H[a][h];
+            }
+            for (int k = 0; k < 6; k++) {
+                G.P[k] = (ACC)(w * cp.P[k]);
+                G.S[k] = (ACC)(w * cp.S[k]);
+            }
+            G.w = (ACC)w;
+        }
+    } else if (tid >= 128 && tid < 128 + 3 * 4 * 9) {
+        sphiA[tid - 128] = (ACC)sphi[tid - 128];
+    }
+    __syncthreads();
+    // unused operand columns (27..31 of every 32) and the Gamma block's k-rows
+    // past 12 stay zero from here on
+    for (int idx = tid; idx < QC * NY * 20; idx += 256) {
+        const int r = idx / 20, x = idx % 20, col = 32 * (x / 5) + 27 + x % 5;
+        sY[r * SV_RS + col] = (ACC)0;
+        sZ[r * SV_RS + col] = (ACC)0;
+    }
+    if (NY > 12) {
+        for (int idx = tid; idx < QC * (NY - 12) * 32; idx += 256) {
+            const int r = idx / 32, ql = r / (NY - 12), k = 12 + r % (NY - 12);
+            sY[(ql * NY + k) * SV_RS + 96 + idx % 32] = (ACC)0;
+            sZ[(ql * NY + k) * SV_RS + 96 + idx % 32] = (ACC)0;
+        }
+    }
+    V acc[6];
+#pragma unroll
+    for (int t = 0; t < 6; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[t][r] = (ACC)0;
+    double accd[6][4];
+#pragma unroll
+    for (int t = 0; t < 6; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) accd[t][r] = 0.0;
+    const SvTiles T = sv_tiles(wave);
+    const ACC ce = (ACC)(2.0 * c.e345), cl = (ACC)(2.0 * c.lsq), mass = (ACC)mass_d;
+
+    const int nchunk = (nqp + QC - 1) / QC;
+    for (int ch = 0; ch < nchunk; ch++) {
+        const int qcnt = min(QC, nqp - ch * QC);
+        // operands: wave j < 3 builds Y_j, Z_j, wave 3 the Gamma pair, lane = (point, B)
+        for (int t = lane; t < 27 * qcnt; t += 64) {
+            const int ql = t / 27, B = t - 27 * ql, qp = ch * QC + ql, q = sQ[qp];
+            ACC n[10];
+            basis_slots<ACC>(sphiA, q & 15, (q >> 4) & 15, (q >> 8) & 15, B, n);
+            const Pt &P = sPt[qp];
+            if (wave < 3)
+                sv_item_yz<ACC>(P, ce, cl, n, wave, sY + ql * NY * SV_RS + 32 * wave + B, sZ + ql * NY * SV_RS + 32 * wave + B);
+            else
+                sv_item_gamma<ACC, GT>(P, P.w * mass, n, sY + ql * NY * SV_RS + 96 + B, sZ + ql * NY * SV_RS + 96 + B);
+        }
+        __syncthreads();
+        sv_mfma_chunk<ACC, NY>(sY, sZ, qcnt, lane, T, acc);
+        // FP32: the partial sums of every 16 Gauss points go into FP64
+        if (sizeof(ACC) == 4 && (((ch + 1) * QC) % 16 == 0 || ch == nchunk - 1)) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+#pragma unroll
+                for (int t = 0; t < 6; t++) {
+                    accd[t][r] += (double)acc[t][r];
+                    acc[t][r] = (ACC)0;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (sizeof(ACC) == 4) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+#pragma unroll
+            for (int t = 0; t < 6; t++) acc[t][r] = (ACC)accd[t][r];
+        }
+    }
+
+    // ---- element matrix: material tiles (both triangles), then Gamma on the diagonal blocks
+    ACC *sKe = reinterpret_cast<ACC *>(sRegion);
+    sv_stage<ACC, false>(sKe, lane, T, acc);
+    __syncthreads();
+    if (wave == 3) sv_stage<ACC, true>(sKe, lane, T, acc);
+    __syncthreads();
+    if (vals_f32)
+        scatter_rows<float, ACC, 256>(sKe, sRowBase, sRowSx, sRowSy, sMask, (float *)vals, tid);
+    else
+        scatter_rows<double, ACC, 256>(sKe, sRowBase, sRowSx, sRowSy, sMask, (double *)vals, tid);
+}
+
+// two work-groups per CU (one for getime FP64, whose LDS allows one)
+template <class ACC, int GT>
+struct SvWaves { static constexpr int v = 2; };
+template <>
+struct SvWaves<double, 1> { static constexpr int v = 1; };
+
+template <class ACC, int QC, int GT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SvWaves<ACC, GT>::v))) void k_tangent_sv(
+    MeshArgs m, const double *__restrict__ Uh, const double *__restrict__ coef, double mass,
+    const long long *__restrict__ rowptr, const unsigned char *__restrict__ dmask, void *__restrict__ vals, int vals_f32,
+    int cx, int cy, int cz, int my, int mz)
+{
+    const int b = blockIdx.x;
+    const int ev[3] = {cx + 3 * (b / (mz * my)), cy + 3 * ((b / mz) % my), cz + 3 * (b % mz)};
+    tangent_sv_body<ACC, QC, GT>(m, Uh, coef, mass, rowptr, dmask, vals, vals_f32, ev);
+}
+
 // diagonal of Dirichlet rows = 1 (MatZeroRowsColumns(..., 1.0, ...))
 __global__ void k_dirichlet_diag(MeshArgs m, const int *__restrict__ rows, int n, const long long *__restrict__ rowptr,
                                  void *vals, int vals_f32)
@@ -1859,6 +2319,33 @@ static int launch_tangent_gt(igap4h_ctx *c, const MeshArgs &m, const double *d_U
     return 0;
 }
 
+// Gauss points per chunk of the strain-variation tangent: the largest that
+// keeps two work-groups per CU in LDS (getime FP64: one)
+template <class ACC, int GT>
+struct SvQC { static constexpr int v = 1; };
+template <>
+struct SvQC<float, 0> { static constexpr int v = 4; };
+template <>
+struct SvQC<float, 1> { static constexpr int v = 2; };
+
+
+template <class ACC, int GT>
+static int launch_tangent_sv(igap4h_ctx *c, const MeshArgs &m, const double *d_Uh, double mass, void *d_vals, int f32, hipStream_t st)
+{
+    for (int cx = 0; cx < 3; cx++)
+        for (int cy = 0; cy < 3; cy++)
+            for (int cz = 0; cz < 3; cz++) {
+                int mx = (m.ne[0] - cx + 2) / 3, my = (m.ne[1] - cy + 2) / 3, mz = (m.ne[2] - cz + 2) / 3;
+                long long nb = (long long)mx * my * mz;
+                if (nb <= 0) continue;
+                hipLaunchKernelGGL((k_tangent_sv<ACC, SvQC<ACC, GT>::v, GT>), dim3((unsigned)nb), dim3(256), 0, st, m, d_Uh,
+                                   (const double *)c->d_coef, mass, (const long long *)c->d_rowptr,
+                                   (const unsigned char *)c->d_mask, d_vals, f32, cx, cy, cz, my, mz);
+                HIPCHK(hipGetLastError());
+            }
+    return 0;
+}
+
 extern "C" int igap4h_tangent_dev(igap4h_ctx *c, const double *d_Uh, int prec, int vals_type, void *d_vals, void *stream)
 {
     if (!c || !d_Uh || !d_vals) return fail(IGAP4H_EINVAL, "NULL argument");
@@ -1871,13 +2358,24 @@ extern "C" int igap4h_tangent_dev(igap4h_ctx *c, const double *d_Uh, int prec, i
     hipLaunchKernelGGL(k_fill_zero, dim3(2048), dim3(256), 0, st, (unsigned int *)d_vals, n32);
     HIPCHK(hipGetLastError());
     double mass = c->physics == IGAP4H_PHYS_GETIME ? c->par_mat[5] / (c->par_mat[7] * c->par_mat[7]) + c->par_mat[6] / (2. * c->par_mat[7]) : 0.0;
+    // AUTO: the faster form per physics and precision (64^3 ge, 48^3 getime,
+    // nq 4, ms per tangent, T strip / strain variation): ge FP64 63.3 / 74.2,
+    // FP32 34.6 / 40.3; getime FP64 67.4 / 77.0, FP32 32.9 / 27.6
+    const bool gt = c->physics == IGAP4H_PHYS_GETIME, f64 = prec == IGAP4H_FP64;
+    const bool sv = c->tangent_form == IGAP4H_TANGENT_SV || (c->tangent_form == IGAP4H_TANGENT_AUTO && gt && !f64);
     int rc;
-    if (c->physics == IGAP4H_PHYS_GE)
-        rc = prec == IGAP4H_FP64 ? launch_tangent_ge<double>(c, m, d_Uh, d_vals, f32, st)
-                                 : launch_tangent_ge<float>(c, m, d_Uh, d_vals, f32, st);
-    else
-        rc = prec == IGAP4H_FP64 ? launch_tangent_gt<double>(c, m, d_Uh, mass, d_vals, f32, st)
-                                 : launch_tangent_gt<float>(c, m, d_Uh, mass, d_vals, f32, st);
+    if (sv) {
+        if (gt) rc = f64 ? launch_tangent_sv<double, 1>(c, m, d_Uh, mass, d_vals, f32, st)
+                         : launch_tangent_sv<float, 1>(c, m, d_Uh, mass, d_vals, f32, st);
+        else rc = f64 ? launch_tangent_sv<double, 0>(c, m, d_Uh, 0.0, d_vals, f32, st)
+                      : launch_tangent_sv<float, 0>(c, m, d_Uh, 0.0, d_vals, f32, st);
+    } else if (gt) {
+        rc = f64 ? launch_tangent_gt<double>(c, m, d_Uh, mass, d_vals, f32, st)
+                 : launch_tangent_gt<float>(c, m, d_Uh, mass, d_vals, f32, st);
+    } else {
+        rc = f64 ? launch_tangent_ge<double>(c, m, d_Uh, d_vals, f32, st)
+                 : launch_tangent_ge<float>(c, m, d_Uh, d_vals, f32, st);
+    }
     if (rc) return rc;
     int nd = (int)c->drows.size();
     if (nd > 0) {
@@ -1885,6 +2383,15 @@ extern "C" int igap4h_tangent_dev(igap4h_ctx *c, const double *d_Uh, int prec, i
                            (const long long *)c->d_rowptr, d_vals, f32);
         HIPCHK(hipGetLastError());
     }
+    return 0;
+}
+
+extern "C" int igap4h_set_tangent_form(igap4h_ctx *c, int form)
+{
+    if (!c) return fail(IGAP4H_EINVAL, "NULL argument");
+    if (form != IGAP4H_TANGENT_AUTO && form != IGAP4H_TANGENT_TSTRIP && form != IGAP4H_TANGENT_SV)
+        return fail(IGAP4H_EINVAL, "bad tangent form %d", form);
+    c->tangent_form = form;
     return 0;
 }
 

@@ -59,6 +59,7 @@ struct igap4h_ctx {
     // direction; gidx = globalIdx of the local window
     bool partitioned = false;
     std::vector<int> gidx;
+    int tangent_form = IGAP4H_TANGENT_AUTO;  // igap4h_set_tangent_form
 };
 
 namespace igap4h {

@@ -182,6 +182,10 @@ class App:
         self._coef = np.ascontiguousarray(coef, dtype=np.float64)
         L.check(L.lib().igap4h_set_coef(self.ctx, _dp(self._coef)))
 
+    def set_tangent_form(self, form):
+        """IGAP4H_TANGENT_AUTO / _TSTRIP / _SV (include/igap4h.h)."""
+        L.check(L.lib().igap4h_set_tangent_form(self.ctx, form))
+
     # ---- host-array assembly
     def residual(self, U, R=None):
         U = np.ascontiguousarray(U, dtype=np.float64)

@@ -245,6 +245,20 @@ int igap4h_tangent_dev(igap4h_ctx *ctx, const double *d_U_hist, int prec, int va
  * so calls that overlap on one device (concurrent streams or threads) race. */
 int igap4h_residual_dev_noalloc(igap4h_ctx *ctx, const double *d_U_hist, double *d_R, void *stream);
 
+/* No counterpart in the reference (a kernel choice of this library).  The
+ * element tangent has two implementations with the same results (FP64 within
+ * rounding, FP32 within its 1e-5 bar): IGAP4H_TANGENT_TSTRIP builds the
+ * product of the pointwise tangent with the basis slots on the vector units
+ * and contracts it with the slots on MFMA; IGAP4H_TANGENT_SV factors the
+ * tangent through the strain variations of the dofs and contracts those on
+ * MFMA (less vector work, more MFMA work).  IGAP4H_TANGENT_AUTO (the
+ * default) takes the faster one per physics and precision as measured on
+ * MI355X: SV for the phys/getime FP32 tangent, TSTRIP otherwise.          */
+#define IGAP4H_TANGENT_AUTO 0
+#define IGAP4H_TANGENT_TSTRIP 1
+#define IGAP4H_TANGENT_SV 2
+int igap4h_set_tangent_form(igap4h_ctx *ctx, int form);
+
 /* ---- material coefficients ------------------------------------------- */
 /* Replace the per-element coefficients (host array n_elem x 6).  The device
  * is synchronized first; a lagged tangent of the Newton driver is discarded. */

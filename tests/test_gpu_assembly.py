@@ -241,3 +241,48 @@ def test_tangent_periodic_F0(oracle, physics):
     assert rel(v, v0) < TOL64, rel(v, v0)
     v32 = app.tangent(U[:pb.ndof], prec=L.IGAP4H_FP32)
     assert rel(v32, v0) < TOL32, rel(v32, v0)
+
+
+def _gt_state(pb, seed):
+    rng = np.random.default_rng(seed)
+    n = pb.ndof
+    U0 = P.smooth_state(pb, amp=0.05, noise=0.0, nhist=1) + rng.uniform(-1, 1, n) * 0.01
+    U1 = U0 + rng.uniform(-1, 1, n) * 0.002
+    U = U1 + rng.uniform(-1, 1, n) * 0.002
+    return np.concatenate([U, U1, U0])
+
+
+@pytest.mark.parametrize("form", [L.IGAP4H_TANGENT_TSTRIP, L.IGAP4H_TANGENT_SV])
+@pytest.mark.parametrize("case", ["ge_323_q4", "ge_132_q4", "ge_211_q2", "ge_111_q1", "ge_321_q3", "ge_F0",
+                                  "gt_444_q4", "gt_212_q2", "gt_F0", "gt_321_q3"])
+def test_tangent_forms(oracle, form, case):
+    """Both element-tangent kernels (T strip; strain variation, chunks of 1,
+    2 or 4 Gauss points with ragged last chunks) against the oracle, FP64 and
+    FP32 (FP64 and FP32 values), ge and getime, Gauss orders 1..4, with and
+    without the periodic F0."""
+    say("tangent form", form, case)
+    phys, dims, nq = case.split("_")[0], case.split("_")[1], case.split("_")[-1]
+    if dims == "F0":
+        pb = P.stretch_problem((3, 2, 4), a=0.01, noise=0.1)
+        pb.par_periodic = np.array([0.001, 0.0, 0.0, 0.0, -0.002, 0.0, 0.0, 0.0, 0.0005])
+    else:
+        pb = P.stretch_problem(tuple(int(x) for x in dims), a=0.01, noise=0.1, nquad=int(nq[1]))
+    if phys == "gt":
+        pb.physics = P.GETIME
+        pb.par_mat = pb.par_mat.copy()
+        pb.par_mat[7] = 0.5
+        Uh = _gt_state(pb, 13)
+    else:
+        Uh = state(pb, seed=17)
+    app = App(pb)
+    app.set_tangent_form(form)
+    app.U_hist[:] = Uh
+    rowptr, col = app.pattern()
+    _, _, v0 = oracle.tangent(pb, Uh, rowptr, col)
+    U = Uh[:pb.ndof]
+    v = app.tangent(U, prec=L.IGAP4H_FP64)
+    assert rel(v, v0) < TOL64, rel(v, v0)
+    v32 = app.tangent(U, prec=L.IGAP4H_FP32)
+    assert rel(v32, v0) < TOL32, rel(v32, v0)
+    v32f = app.tangent(U, prec=L.IGAP4H_FP32, vals_type=L.IGAP4H_VALS_F32)
+    assert v32f.dtype == np.float32 and rel(v32f.astype(np.float64), v0) < TOL32
