@@ -248,7 +248,7 @@ def config_legs(say):
     c4 = batch.run(256, reps=3, fp64=False, say=lambda *a: say("  ", *a))
     best = min(c4["autotuner_ms"].values())
     flop = c4["nelem"] * FLOP_KE_PER_QP * 27
-    c4["roofline_tangent_f32"] = {"bound": "mfma", "kernel": "k_tangent_ge_batch<float,QC> (tuned variant)",
+    c4["roofline_tangent_f32"] = {"bound": "mfma", "kernel": "k_tangent_ge_batch<float,QC> / k_tangent_sv_batch<float,QC> (tuned variant)",
                                   "achieved": flop / (best * 1e-3) / 1e12, "peak": PEAK["f32"], "unit": "TFLOP/s",
                                   "frac": flop / (best * 1e-3) / 1e12 / PEAK["f32"]}
     c4["assembly_tflops_f32_tangent_share"] = flop / (c4["graph_ms_per_assembly"] * 1e-3) / 1e12

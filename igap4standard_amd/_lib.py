@@ -64,6 +64,7 @@ IGAP4H_KSP_GMRES, IGAP4H_KSP_CG = 0, 1
 IGAP4H_MG_GS, IGAP4H_MG_CHEBYSHEV = 0, 1
 IGAP4H_SMOOTH_SAME, IGAP4H_SMOOTH_BF16, IGAP4H_SMOOTH_F16 = 0, 1, 2
 IGAP4H_TANGENT_AUTO, IGAP4H_TANGENT_TSTRIP, IGAP4H_TANGENT_SV = 0, 1, 2
+IGAP4H_BATCH_MAX_VARIANTS = 8
 
 
 class NewtonOpts(ctypes.Structure):

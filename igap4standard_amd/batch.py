@@ -72,10 +72,11 @@ class Batch:
 
     def tune(self, reps=3):
         best = ctypes.c_int()
-        ms = (ctypes.c_double * 3)()
-        qs = (ctypes.c_int * 3)()
+        nv = L.IGAP4H_BATCH_MAX_VARIANTS
+        ms = (ctypes.c_double * nv)()
+        qs = (ctypes.c_int * nv)()
         L.check(L.lib().igap4h_batch_tune(self.b, reps, ctypes.byref(best), ms, qs))
-        return best.value, {int(qs[k]): float(ms[k]) for k in range(3)}
+        return best.value, {int(qs[k]): float(ms[k]) for k in range(nv) if qs[k]}
 
     def assemble(self, stream=None):
         L.check(L.lib().igap4h_batch_assemble(self.b, ctypes.c_void_p(stream or 0)))
@@ -106,7 +107,7 @@ def run(count=256, reps=5, fp64=False, seed=0x5EED0004, sequential=True, say=Non
     t_setup = time.perf_counter() - t0
     say("%d instances, %d elements, set-up %.1f s" % (len(pbs), B.nelem, t_setup))
     best, table = B.tune(reps)
-    say("autotuner (Gauss points per MFMA chunk -> tangent ms):", table, "-> %d" % best)
+    say("autotuner (variant = 10 form + Gauss points per MFMA chunk -> tangent ms):", table, "-> %d" % best)
     stream = torch.cuda.current_stream()
     B.assemble(stream.cuda_stream)
     torch.cuda.synchronize()
@@ -134,7 +135,9 @@ def run(count=256, reps=5, fp64=False, seed=0x5EED0004, sequential=True, say=Non
     sizes = {n: sum(1 for pb in pbs if pb.nelem[0] == n) for n in SIZES}
     out = {"instances": len(pbs), "nelem": B.nelem, "sizes": sizes, "nquad": 3, "setup_s": t_setup,
            "precision": "fp64" if fp64 else "fp32-mfma tangent (fp64 chunk accumulation), fp32 values, fp64 residual",
-           "autotuner_ms": table, "variant": best, "graph_ms_per_assembly": ms,
+           "autotuner_ms": table, "variant": best,
+           "variant_key": "10 x form (0 T strip, 1 strain variation) + Gauss points per MFMA chunk",
+           "graph_ms_per_assembly": ms,
            "elements_per_s": B.nelem / (ms * 1e-3), "sequential_ms_per_assembly": ms_seq,
            "graph_speedup_vs_sequential": (ms_seq / ms) if ms_seq else None}
     B.close()

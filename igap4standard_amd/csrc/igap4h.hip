@@ -2640,8 +2640,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     tangent_ge_body<ACC, QC>(m, b.Uh, b.coef, b.rowptr, b.mask, b.vals, vals_f32, ev);
 }
 
-// the tuning space: Gauss points per MFMA chunk (contraction depth 9 QC per pass)
-static const int kBatchQC[2][3] = {{1, 2, 3}, {2, 3, 4}};  // [prec FP64, FP32]
+// the same with the strain-variation form of the tangent
+template <class ACC, int QC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_tangent_sv_batch(
+    const BatchInst *__restrict__ I, const long long *__restrict__ coff, int n, int vals_f32, int cx, int cy, int cz)
+{
+    const int i = batch_find(coff, n, blockIdx.x);
+    const BatchInst &b = I[i];
+    const MeshArgs &m = b.m;
+    const long long t = (long long)blockIdx.x - coff[i];
+    const int my = (m.ne[1] - cy + 2) / 3, mz = (m.ne[2] - cz + 2) / 3;
+    const int ev[3] = {cx + 3 * (int)(t / ((long long)mz * my)), cy + 3 * (int)((t / mz) % my), cz + 3 * (int)(t % mz)};
+    tangent_sv_body<ACC, QC, 0>(m, b.Uh, b.coef, 0.0, b.rowptr, b.mask, b.vals, vals_f32, ev);
+}
+
+// The tuning space, variant = 10 x form + Gauss points per MFMA chunk: form
+// 0 = T strip (contraction depth 9 QC per pass), 1 = strain variation (12 QC
+// per pass, 21 + 3 tiles); 0 ends a list.
+constexpr int kBatchNV = 8;
+static const int kBatchVariants[2][kBatchNV] = {{1, 2, 3, 11, 0, 0, 0, 0}, {2, 3, 4, 12, 14, 0, 0, 0}};  // [FP64, FP32]
+static bool batch_variant_ok(int prec, int v)
+{
+    for (int k = 0; k < kBatchNV; k++)
+        if (v && kBatchVariants[prec == IGAP4H_FP32][k] == v) return true;
+    return false;
+}
 
 struct igap4h_batch {
     int n = 0, prec = 0, vals_f32 = 0, device = 0;
@@ -2693,32 +2716,40 @@ static int batch_refresh(igap4h_batch *b)
     return 0;
 }
 
-template <class ACC, int QC>
+template <class ACC, int QC, int SV>
 static int batch_tangent_launches(igap4h_batch *b, hipStream_t st)
 {
     for (int c = 0; c < 27; c++) {
         const long long *off = b->coff.data() + (size_t)c * (b->n + 1);
         long long nb = off[b->n];
         if (nb <= 0) continue;
-        hipLaunchKernelGGL((k_tangent_ge_batch<ACC, QC>), dim3((unsigned)nb), dim3(256), 0, st, (const BatchInst *)b->d_inst,
-                           (const long long *)(b->d_off + (b->n + 1) * (1 + c)), b->n, b->vals_f32, c / 9, (c / 3) % 3, c % 3);
+        const long long *d_coff = b->d_off + (b->n + 1) * (1 + c);
+        if constexpr (SV)
+            hipLaunchKernelGGL((k_tangent_sv_batch<ACC, QC>), dim3((unsigned)nb), dim3(256), 0, st, (const BatchInst *)b->d_inst,
+                               d_coff, b->n, b->vals_f32, c / 9, (c / 3) % 3, c % 3);
+        else
+            hipLaunchKernelGGL((k_tangent_ge_batch<ACC, QC>), dim3((unsigned)nb), dim3(256), 0, st, (const BatchInst *)b->d_inst,
+                               d_coff, b->n, b->vals_f32, c / 9, (c / 3) % 3, c % 3);
         HIPCHK(hipGetLastError());
     }
     return 0;
 }
 
-static int batch_tangent(igap4h_batch *b, int qc, hipStream_t st)
+static int batch_tangent(igap4h_batch *b, int v, hipStream_t st)
 {
     if (b->prec == IGAP4H_FP32) {
-        if (qc == 2) return batch_tangent_launches<float, 2>(b, st);
-        if (qc == 3) return batch_tangent_launches<float, 3>(b, st);
-        if (qc == 4) return batch_tangent_launches<float, 4>(b, st);
+        if (v == 2) return batch_tangent_launches<float, 2, 0>(b, st);
+        if (v == 3) return batch_tangent_launches<float, 3, 0>(b, st);
+        if (v == 4) return batch_tangent_launches<float, 4, 0>(b, st);
+        if (v == 12) return batch_tangent_launches<float, 2, 1>(b, st);
+        if (v == 14) return batch_tangent_launches<float, 4, 1>(b, st);
     } else {
-        if (qc == 1) return batch_tangent_launches<double, 1>(b, st);
-        if (qc == 2) return batch_tangent_launches<double, 2>(b, st);
-        if (qc == 3) return batch_tangent_launches<double, 3>(b, st);
+        if (v == 1) return batch_tangent_launches<double, 1, 0>(b, st);
+        if (v == 2) return batch_tangent_launches<double, 2, 0>(b, st);
+        if (v == 3) return batch_tangent_launches<double, 3, 0>(b, st);
+        if (v == 11) return batch_tangent_launches<double, 1, 1>(b, st);
     }
-    return fail(IGAP4H_EINVAL, "no batched tangent variant with %d Gauss points per chunk for this precision", qc);
+    return fail(IGAP4H_EINVAL, "no batched tangent variant %d for this precision", v);
 }
 
 static int batch_sequence(igap4h_batch *b, hipStream_t st)
@@ -2859,8 +2890,7 @@ extern "C" int igap4h_batch_sizes(const igap4h_batch *b, int64_t *nelem, int *qc
 extern "C" int igap4h_batch_set_variant(igap4h_batch *b, int qc)
 {
     if (!b) return fail(IGAP4H_EINVAL, "NULL batch");
-    const int *cand = kBatchQC[b->prec == IGAP4H_FP32];
-    if (qc != cand[0] && qc != cand[1] && qc != cand[2]) return fail(IGAP4H_EINVAL, "variant %d is not in the tuning space", qc);
+    if (!batch_variant_ok(b->prec, qc)) return fail(IGAP4H_EINVAL, "variant %d is not in the tuning space", qc);
     if (qc != b->qc) batch_drop_graph(b);
     b->qc = qc;
     return 0;
@@ -2868,7 +2898,8 @@ extern "C" int igap4h_batch_set_variant(igap4h_batch *b, int qc)
 
 // Autotuner: time the batched tangent of every variant (reps launches each,
 // HIP events on the batch's own stream), keep the fastest.  ms[k] receives
-// the mean time of candidate k (3 entries), qcs[k] its chunk size.
+// the mean time of candidate k, qcs[k] the variant (kBatchNV entries, 0 past
+// the space).
 extern "C" int igap4h_batch_tune(igap4h_batch *b, int reps, int *best_qc, double *ms, int *qcs)
 {
     if (!b) return fail(IGAP4H_EINVAL, "NULL batch");
@@ -2876,13 +2907,17 @@ extern "C" int igap4h_batch_tune(igap4h_batch *b, int reps, int *best_qc, double
     HIPCHK(hipSetDevice(b->device));
     int rc0 = batch_refresh(b);
     if (rc0) return rc0;
-    const int *cand = kBatchQC[b->prec == IGAP4H_FP32];
+    const int *cand = kBatchVariants[b->prec == IGAP4H_FP32];
+    if (ms)
+        for (int k = 0; k < kBatchNV; k++) ms[k] = 0.0;
+    if (qcs)
+        for (int k = 0; k < kBatchNV; k++) qcs[k] = 0;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     double best = 1e300;
     int bq = b->qc, rc = 0;
-    for (int k = 0; k < 3 && !rc; k++) {
+    for (int k = 0; k < kBatchNV && cand[k] && !rc; k++) {
         if ((rc = batch_tangent(b, cand[k], b->cap))) break;  // warm-up
         if (hipEventRecord(e0, b->cap) != hipSuccess) { rc = fail(IGAP4H_EHIP, "hipEventRecord failed"); break; }
         for (int r = 0; r < reps && !rc; r++) rc = batch_tangent(b, cand[k], b->cap);

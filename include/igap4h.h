@@ -325,9 +325,12 @@ int igap4h_batch_create(igap4h_ctx *const *ctxs, int n, const double *const *d_U
                         void *const *d_vals, int prec, int vals_type, igap4h_batch **batch);
 void igap4h_batch_destroy(igap4h_batch *batch);
 int igap4h_batch_sizes(const igap4h_batch *batch, int64_t *nelem, int *variant);
-/* Autotuner: times each tangent variant of the tuning space (Gauss points per
- * MFMA chunk: FP64 {1, 2, 3}, FP32 {2, 3, 4}) on this batch, reps launches
- * each, keeps the fastest.  ms[3] / variants[3] (may be NULL) get the table. */
+/* Autotuner: times each tangent variant of the tuning space on this batch,
+ * reps launches each, and keeps the fastest.  variant = 10 x form + Gauss
+ * points per MFMA chunk, form 0 = T strip, 1 = strain variation (see
+ * igap4h_set_tangent_form): FP64 {1, 2, 3, 11}, FP32 {2, 3, 4, 12, 14}.
+ * ms[8] / variants[8] (may be NULL) get the table, variant 0 past its end.  */
+#define IGAP4H_BATCH_MAX_VARIANTS 8
 int igap4h_batch_tune(igap4h_batch *batch, int reps, int *best_variant, double *ms, int *variants);
 int igap4h_batch_set_variant(igap4h_batch *batch, int variant);
 int igap4h_batch_assemble(igap4h_batch *batch, void *stream);
