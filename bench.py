@@ -49,7 +49,10 @@ FLOP_KE_PER_QP = 9 * (2 * 10 * 10 * 27 + 2 * 27 * 10 * 27)
 # projection 1,620 + basis products 540 + the pointwise stress 291 (ge,
 # counted on igap4h_physics.hpp's residual_grad: 121 adds, 170 multiplies)
 FLOP_RE_PER_QP = 1620 + 1620 + 540 + 291
-PEAK = {"f64": 78.6, "f32": 157.3}   # MI355X dense MFMA TFLOP/s
+PEAK = {"f64": 78.6, "f32": 157.3}   # MI355X dense MFMA TFLOP/s (the vector FP64 / FP32 peaks are the same)
+# MFMA flops the strain-variation tangent executes per Gauss point: 24 tiles
+# of 16 x 16 x 4 per 4 strain components (ge 12, getime 24 per dof)
+SV_MFMA_FLOP_PER_QP = {"ge": 24 * 3 * 2 * 16 * 16 * 4, "getime": 24 * 6 * 2 * 16 * 16 * 4}
 PEAK_HBM = 8000.0                    # GB/s
 
 
@@ -225,7 +228,7 @@ def config_legs(say):
     dR = torch.empty(app.ndof, device="cuda:0", dtype=torch.float64)
     c3["residual_ms"] = _events(torch, stream, lambda: app.residual_dev(ts.Uh.data_ptr(), dR.data_ptr(),
                                                                         stream.cuda_stream), 5)
-    for prec, vt, dt, nm, peak in ((L.IGAP4H_FP32, L.IGAP4H_VALS_F32, torch.float32, "k_tangent_gt<float,4>",
+    for prec, vt, dt, nm, peak in ((L.IGAP4H_FP32, L.IGAP4H_VALS_F32, torch.float32, "k_tangent_sv<float,2,1>",
                                     PEAK["f32"]),
                                    (L.IGAP4H_FP64, L.IGAP4H_VALS_F64, torch.float64, "k_tangent_gt<double,2>",
                                     PEAK["f64"])):
@@ -235,7 +238,11 @@ def config_legs(say):
         tf = pb.n_elem * FLOP_KE_PER_QP * pb.nquad ** 3 / (ms * 1e-3) / 1e12
         c3["roofline_tangent_" + ("f32" if prec == L.IGAP4H_FP32 else "f64")] = {
             "bound": "mfma", "kernel": nm + " (27 colour launches per tangent)", "tangent_ms": ms,
-            "achieved": tf, "peak": peak, "unit": "TFLOP/s", "frac": tf / peak}
+            "achieved": tf, "peak": peak, "unit": "TFLOP/s", "frac": tf / peak,
+            "algorithmic": "11,508,480 flop per element"}
+        if prec == L.IGAP4H_FP32:
+            ex = pb.n_elem * SV_MFMA_FLOP_PER_QP["getime"] * pb.nquad ** 3 / (ms * 1e-3) / 1e12
+            c3["roofline_tangent_f32"].update({"mfma_executed_tflops": ex, "mfma_executed_frac": ex / peak})
         del V
     ts.app.close()
     c3["workload"] = ("48^3 getime (torder 2), all faces clamped, rho 1, lambda 1, mu 0.5, l 0.05, dt 1e-3, v0 a "
@@ -278,6 +285,8 @@ def main():
                          "of the FP32 tangent)")
     ap.add_argument("--no-newton", action="store_true", help="assembly and SpMV only")
     ap.add_argument("--no-extras", action="store_true", help="only the 64^3 headline workload")
+    ap.add_argument("--tangent-form", choices=("auto", "tstrip", "sv"), default="auto",
+                    help="element tangent kernel (igap4h_set_tangent_form); auto: the faster per precision")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -317,6 +326,8 @@ def main():
     torch.cuda.set_device(dev)
     say("building context for %d^3 (%d dofs)" % (n, pb.ndof))
     app = App(pb)
+    app.set_tangent_form({"auto": L.IGAP4H_TANGENT_AUTO, "tstrip": L.IGAP4H_TANGENT_TSTRIP,
+                          "sv": L.IGAP4H_TANGENT_SV}[args.tangent_form])
     stream = torch.cuda.current_stream(dev)
     sp = stream.cuda_stream
     dUh = torch.tensor(U, device=dev, dtype=torch.float64)
@@ -459,11 +470,14 @@ def main():
         "spmv": spmv,
     }
     if "mixed" in asm:
+        ex = pb.n_elem * SV_MFMA_FLOP_PER_QP["ge"] * pb.nquad ** 3 / (asm["mixed"]["tangent_ms"] * 1e-3) / 1e12
         line["roofline_tangent_f32"] = {"bound": "mfma", "achieved": asm["mixed"]["tangent_tflops"], "peak": PEAK["f32"],
                                         "unit": "TFLOP/s", "frac": asm["mixed"]["tangent_tflops"] / PEAK["f32"],
-                                        "traffic": profiled_traffic("k_tangent_ge<float,4>"),
-                                        "kernel": "k_tangent_ge<float,4> (27 colour launches per tangent)",
-                                        "algorithmic": "11,508,480 flop per element"}
+                                        "traffic": profiled_traffic("k_tangent_sv<float,2,0>"),
+                                        "kernel": "k_tangent_sv<float,2,0> (strain-variation form, 27 colour "
+                                                  "launches per tangent)",
+                                        "algorithmic": "11,508,480 flop per element",
+                                        "mfma_executed_tflops": ex, "mfma_executed_frac": ex / PEAK["f32"]}
     if "mixed" in asm and "fp64" in asm:
         line["assembly"]["mixed"]["roofline_frac_f32"] = asm["mixed"]["tangent_tflops"] / PEAK["f32"]
         line["assembly"]["mixed"]["tangent_speedup_vs_fp64"] = asm["fp64"]["tangent_ms"] / asm["mixed"]["tangent_ms"]

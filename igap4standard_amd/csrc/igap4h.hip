@@ -2319,12 +2319,13 @@ static int launch_tangent_gt(igap4h_ctx *c, const MeshArgs &m, const double *d_U
     return 0;
 }
 
-// Gauss points per chunk of the strain-variation tangent: the largest that
-// keeps two work-groups per CU in LDS (getime FP64: one)
+// Gauss points per chunk of the strain-variation tangent, as measured (64^3
+// ge FP32: 37.5 / 32.5 / 40.3 ms with 1 / 2 / 4; 48^3 getime FP32: 31.9 /
+// 27.6 with 1 / 2); FP64 keeps two work-groups per CU with 1 (getime: one)
 template <class ACC, int GT>
 struct SvQC { static constexpr int v = 1; };
 template <>
-struct SvQC<float, 0> { static constexpr int v = 4; };
+struct SvQC<float, 0> { static constexpr int v = 2; };
 template <>
 struct SvQC<float, 1> { static constexpr int v = 2; };
 
@@ -2358,11 +2359,11 @@ extern "C" int igap4h_tangent_dev(igap4h_ctx *c, const double *d_Uh, int prec, i
     hipLaunchKernelGGL(k_fill_zero, dim3(2048), dim3(256), 0, st, (unsigned int *)d_vals, n32);
     HIPCHK(hipGetLastError());
     double mass = c->physics == IGAP4H_PHYS_GETIME ? c->par_mat[5] / (c->par_mat[7] * c->par_mat[7]) + c->par_mat[6] / (2. * c->par_mat[7]) : 0.0;
-    // AUTO: the faster form per physics and precision (64^3 ge, 48^3 getime,
-    // nq 4, ms per tangent, T strip / strain variation): ge FP64 63.3 / 74.2,
-    // FP32 34.6 / 40.3; getime FP64 67.4 / 77.0, FP32 32.9 / 27.6
+    // AUTO: the faster form per precision (64^3 ge, 48^3 getime, nq 4, ms per
+    // tangent, T strip / strain variation): ge FP64 63.4 / 74.4, FP32 34.7 /
+    // 32.5; getime FP64 67.4 / 77.0, FP32 32.9 / 27.6
     const bool gt = c->physics == IGAP4H_PHYS_GETIME, f64 = prec == IGAP4H_FP64;
-    const bool sv = c->tangent_form == IGAP4H_TANGENT_SV || (c->tangent_form == IGAP4H_TANGENT_AUTO && gt && !f64);
+    const bool sv = c->tangent_form == IGAP4H_TANGENT_SV || (c->tangent_form == IGAP4H_TANGENT_AUTO && !f64);
     int rc;
     if (sv) {
         if (gt) rc = f64 ? launch_tangent_sv<double, 1>(c, m, d_Uh, mass, d_vals, f32, st)
@@ -2802,7 +2803,8 @@ extern "C" int igap4h_batch_create(igap4h_ctx *const *ctxs, int n, const double 
     b->n = n;
     b->prec = prec;
     b->vals_f32 = vals_type == IGAP4H_VALS_F32;
-    b->qc = prec == IGAP4H_FP32 ? GeQC<float>::v : GeQC<double>::v;
+    // the single-instance default (IGAP4H_TANGENT_AUTO): FP32 strain variation, FP64 T strip
+    b->qc = prec == IGAP4H_FP32 ? 10 + SvQC<float, 0>::v : GeQC<double>::v;
     b->eoff.assign(n + 1, 0);
     b->coff.assign((size_t)27 * (n + 1), 0);
     b->goff.assign(n + 1, 0);

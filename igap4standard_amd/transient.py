@@ -56,7 +56,7 @@ class TimeStepper:
         return info
 
 
-def run(n=48, steps=100, dt=1e-3, fp64=False, lag=1, persist=False, say=None):
+def run(n=48, steps=100, dt=1e-3, fp64=False, lag=1, persist=False, say=None, tangent_form=L.IGAP4H_TANGENT_AUTO):
     """Config-3 run: n^3 getime, all faces clamped, v0 a Gaussian random field
     (std 1e-3, correlation 0.2), `steps` steps of dt.  Returns (summary dict,
     the TimeStepper, whose context stays open)."""
@@ -65,6 +65,7 @@ def run(n=48, steps=100, dt=1e-3, fp64=False, lag=1, persist=False, say=None):
     pb = P.transient_problem(n, dt=dt)
     v0 = P.initial_velocity(pb)
     ts = TimeStepper(pb, mixed=not fp64, lag_jacobian=lag, lag_persist=int(persist))
+    ts.app.set_tangent_form(tangent_form)
     ts.set_state(np.zeros(pb.ndof), -dt * v0)
     say("%d^3 getime, %d dofs, dt %g, %s tangent" % (n, pb.ndof, dt, "FP64" if fp64 else "FP32"))
     infos, norms = [], []
@@ -106,9 +107,12 @@ def main(argv=None):
     ap.add_argument("--fp64", action="store_true", help="FP64 tangent (default: FP32-MFMA tangent, FP32 values)")
     ap.add_argument("--lag", type=int, default=1, help="rebuild the tangent every LAG Newton steps")
     ap.add_argument("--persist", action="store_true", help="count the lag across time steps")
+    ap.add_argument("--tangent-form", choices=("auto", "tstrip", "sv"), default="auto",
+                    help="element tangent kernel (igap4h_set_tangent_form)")
     args = ap.parse_args(argv)
+    form = {"auto": L.IGAP4H_TANGENT_AUTO, "tstrip": L.IGAP4H_TANGENT_TSTRIP, "sv": L.IGAP4H_TANGENT_SV}[args.tangent_form]
     out, ts = run(args.n, args.steps, args.dt, args.fp64, args.lag, args.persist,
-                  say=lambda *a: print("transient:", *a, file=sys.stderr, flush=True))
+                  say=lambda *a: print("transient:", *a, file=sys.stderr, flush=True), tangent_form=form)
     ts.app.close()
     print(json.dumps(out), flush=True)
 
