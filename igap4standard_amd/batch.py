@@ -95,9 +95,10 @@ class Batch:
             pass
 
 
-def run(count=256, reps=5, fp64=False, seed=0x5EED0004, sequential=True, say=None):
-    """Config-4 run: count instances, autotuner, timed graph replays of the
-    whole assembly (and, optionally, the same work instance by instance)."""
+def run(count=256, reps=5, fp64=False, seed=0x5EED0004, sequential=True, say=None, variant=None):
+    """Config-4 run: count instances, autotuner (or the given variant), timed
+    graph replays of the whole assembly (and, optionally, the same work
+    instance by instance)."""
     import torch
     say = say or (lambda *a: None)
     pbs = batch_problems(count, seed=seed)
@@ -106,8 +107,12 @@ def run(count=256, reps=5, fp64=False, seed=0x5EED0004, sequential=True, say=Non
     B = Batch(pbs, prec, vt)
     t_setup = time.perf_counter() - t0
     say("%d instances, %d elements, set-up %.1f s" % (len(pbs), B.nelem, t_setup))
-    best, table = B.tune(reps)
-    say("autotuner (variant = 10 form + Gauss points per MFMA chunk -> tangent ms):", table, "-> %d" % best)
+    if variant is None:
+        best, table = B.tune(reps)
+        say("autotuner (variant = 10 form + Gauss points per MFMA chunk -> tangent ms):", table, "-> %d" % best)
+    else:
+        B.set_variant(variant)
+        best, table = variant, {}
     stream = torch.cuda.current_stream()
     B.assemble(stream.cuda_stream)
     torch.cuda.synchronize()
@@ -150,8 +155,11 @@ def main(argv=None):
     ap.add_argument("--count", type=int, default=256)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--fp64", action="store_true")
+    ap.add_argument("--variant", type=int, default=None, help="skip the autotuner, use this variant")
+    ap.add_argument("--no-sequential", action="store_true", help="graph replays only")
     args = ap.parse_args(argv)
-    out = run(args.count, args.reps, args.fp64, say=lambda *a: print("batch:", *a, file=sys.stderr, flush=True))
+    out = run(args.count, args.reps, args.fp64, sequential=not args.no_sequential,
+              say=lambda *a: print("batch:", *a, file=sys.stderr, flush=True), variant=args.variant)
     print(json.dumps(out), flush=True)
 
 
