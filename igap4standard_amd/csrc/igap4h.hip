@@ -1132,13 +1132,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GtWaves<ACC
 // form [[M00, M01], [M01, M11]], M00 = C0/4 + C1/6 + C2/8, M01 = C0/6 + C1/8,
 // M11 = C0/8; Gamma takes the averaged S and tau and the mass rho/dt^2 +
 // c/(2 dt) on the value slot (phys/getime/tangent.c:4127-5026).
-// MFMA: rows R = 32 i + A and columns C = 32 j + B (A, B < 27 used) in 16 x
-// 16 tiles (rt, ct), ct >= rt (21 tiles, mirrored into the lower triangle),
-// contracted over the NY components of every Gauss point.  The Gamma
-// operands n(A) and w Gamma n(B) sit in columns 96 + A / 96 + B of the same
-// strips (12 slot rows, the rest zero), and its 3 tiles of the 27 x 27 block
-// are added to the three diagonal blocks: 24 tiles, 6 per wave, one code
-// path for every wave (3 A and 6 B fragments per k-step).
+// MFMA: rows R = 27 i + A and columns C = 27 j + B; dofs 0..79 fill five
+// 16-row tiles exactly, and the 15 tiles (rt, ct), ct >= rt, are mirrored
+// into the lower triangle.  The last dof (A = 26, i = 2) is taken by the
+// vector units: its column K[., 80] = sum_q Y(.)^T Z(80) is a dot product
+// the lane that builds Y(.) forms with Z(80) of the point, built one chunk
+// ahead by spare lanes of wave 3.  The Gamma operands n(A) and w Gamma n(B)
+// sit in columns 96 + A / 96 + B of the same strips (12 slot rows, the rest
+// zero), and its 3 tiles of the 27 x 27 block are added to the three
+// diagonal blocks.  5 tile slots per wave, one code path for every wave (3 A
+// and 5 B fragments per k-step; wave 3 has two empty slots).
 constexpr int SV_RS = 144;  // k-row stride of the operand strips: 128 + 16, the 4 k-rows of a fragment on distinct banks
 
 template <class ACC, int GT>
@@ -1193,12 +1196,10 @@ __device__ __forceinline__ void sym3_apply_add(const double *P, const double *y,
     z[2] += P[2] * y[0] + P[4] * y[1] + P[5] * y[2];
 }
 
-// Y_j(B) and Z = w M Y_j(B) of one (point, B, j) into k-rows 0..NY-1 of the
-// strips at column 32 j + B (y, z point there); ce = 2 e345, cl = 2 le^2
+// Y_j(B) and Z = w M Y_j(B) of one (point, B, j) in registers; ce = 2 e345, cl = 2 le^2
 template <class ACC>
-__device__ __forceinline__ void sv_item_yz(const SvPoint<ACC, 0> &p, ACC ce, ACC cl, const ACC n[10], int j, ACC *y, ACC *z)
+__device__ __forceinline__ void sv_yz(const SvPoint<ACC, 0> &p, ACC ce, ACC cl, const ACC n[10], int j, ACC Y[12], ACC Z[12])
 {
-    ACC Y[12], Z[12];
     sv_y<ACC>(p.F[j], p.H[j], n, Y);
     const ACC we = p.w * ce, wl = p.w * cl;
     Z[0] = Z[1] = Z[2] = ACC(0);
@@ -1207,16 +1208,10 @@ __device__ __forceinline__ void sv_item_yz(const SvPoint<ACC, 0> &p, ACC ce, ACC
     for (int k = 3; k < 6; k++) Z[k] = we * Y[k];
 #pragma unroll
     for (int k = 6; k < 12; k++) Z[k] = wl * Y[k];
-#pragma unroll
-    for (int k = 0; k < 12; k++) {
-        y[k * SV_RS] = Y[k];
-        z[k * SV_RS] = Z[k];
-    }
 }
 template <class ACC>
-__device__ __forceinline__ void sv_item_yz(const SvPoint<ACC, 1> &p, ACC ce, ACC cl, const ACC n[10], int j, ACC *y, ACC *z)
+__device__ __forceinline__ void sv_yz(const SvPoint<ACC, 1> &p, ACC ce, ACC cl, const ACC n[10], int j, ACC Y[24], ACC Z[24])
 {
-    ACC Y[24], Z[24];
     sv_y<ACC>(p.F[j], p.H[j], n, Y);
     sv_y<ACC>(p.F1[j], p.H1[j], n, Y + 12);
 #pragma unroll
@@ -1233,11 +1228,25 @@ __device__ __forceinline__ void sv_item_yz(const SvPoint<ACC, 1> &p, ACC ce, ACC
         Z[k] = a00 * Y[k] + a01 * Y[12 + k];
         Z[12 + k] = a01 * Y[k] + a11 * Y[12 + k];
     }
+}
+
+// Y_j(B), Z_j(B) into k-rows 0..NY-1 of the strips at the column y, z point
+// to; returns Y_j(B) . z80, the dof pair's entry K[(B, j), 80] at this point
+template <class ACC, int GT>
+__device__ __forceinline__ ACC sv_item_yz(const SvPoint<ACC, GT> &p, ACC ce, ACC cl, const ACC n[10], int j, ACC *y, ACC *z,
+                                          const ACC *z80)
+{
+    constexpr int NY = GT ? 24 : 12;
+    ACC Y[NY], Z[NY];
+    sv_yz<ACC>(p, ce, cl, n, j, Y, Z);
+    ACC d = ACC(0);
 #pragma unroll
-    for (int k = 0; k < 24; k++) {
+    for (int k = 0; k < NY; k++) {
         y[k * SV_RS] = Y[k];
         z[k * SV_RS] = Z[k];
+        d += Y[k] * z80[k];
     }
+    return d;
 }
 
 // n(B) and w Gamma n(B) on the 12 Gamma rows (ge: slots 1..9, getime: 0..9,
@@ -1266,31 +1275,34 @@ __device__ __forceinline__ void sv_item_gamma(const SvPoint<ACC, GT> &p, ACC wma
     }
 }
 
-// The wave's 6 tiles: A fragments from 3 row offsets (tiles 0-2 use the
-// first, 3-4 the second, 5 the third), B fragments from 6 column offsets;
-// offsets are strip columns (96, 112: the Gamma block).
-//   wave 0: (0,0) (0,1) (0,2) (0,3) (0,4) (0,5)
-//   wave 1: (1,1) (1,2) (1,3) (1,4) (1,5) (5,5)
-//   wave 2: (2,2) (2,3) (2,4) (4,4) (4,5) (2,5)
-//   wave 3: (3,3) (3,4) (3,5) (G0,G0) (G0,G1) (G1,G1)
+// The wave's 5 tile slots: A fragments from 3 row offsets (slots 0-2 use the
+// first, 3 the second, 4 the third), B fragments from 5 column offsets;
+// offsets are strip columns (96, 112: the Gamma block; 128: an empty slot,
+// computed and discarded).
+//   wave 0: (0,0) (0,1) (0,2) (0,3) (0,4)
+//   wave 1: (1,1) (1,2) (1,3) (1,4) (4,4)
+//   wave 2: (2,2) (2,3) (2,4) (3,3) (3,4)
+//   wave 3: (G0,G0) (G0,G1) -  (G1,G1) -
+// (Fragments repeated within a wave are read again: skipping the repeats
+// with wave-uniform branches split the k-step loop and cost 15 %.)
 struct SvTiles {
-    int ao[3], bo[6];
+    int ao[3], bo[5];
 };
 __device__ __forceinline__ SvTiles sv_tiles(int wave)
 {
     SvTiles t;
-    if (wave == 0) t = SvTiles{{0, 0, 0}, {0, 16, 32, 48, 64, 80}};
-    else if (wave == 1) t = SvTiles{{16, 16, 80}, {16, 32, 48, 64, 80, 80}};
-    else if (wave == 2) t = SvTiles{{32, 64, 32}, {32, 48, 64, 64, 80, 80}};
-    else t = SvTiles{{48, 96, 112}, {48, 64, 80, 96, 112, 112}};
+    if (wave == 0) t = SvTiles{{0, 0, 0}, {0, 16, 32, 48, 64}};
+    else if (wave == 1) t = SvTiles{{16, 16, 64}, {16, 32, 48, 64, 64}};
+    else if (wave == 2) t = SvTiles{{32, 48, 48}, {32, 48, 64, 48, 64}};
+    else t = SvTiles{{96, 112, 112}, {96, 112, 128, 112, 128}};
     return t;
 }
-__device__ __forceinline__ int sv_arow(const SvTiles &T, int t) { return T.ao[t < 3 ? 0 : (t < 5 ? 1 : 2)]; }
+__device__ __forceinline__ int sv_arow(const SvTiles &T, int t) { return T.ao[t < 3 ? 0 : (t < 4 ? 1 : 2)]; }
 
 // the chunk's qcnt points, the NY k-rows of each (y: A strip, z: B strip)
 template <class ACC, int NY>
 __device__ __forceinline__ void sv_mfma_chunk(const ACC *y, const ACC *z, int qcnt, int lane, const SvTiles &T,
-                                              typename V4<ACC>::t acc[6])
+                                              typename V4<ACC>::t acc[5])
 {
     const int off = (lane >> 4) * SV_RS + (lane & 15);
     const ACC *ya = y + off, *zb = z + off;
@@ -1299,15 +1311,14 @@ __device__ __forceinline__ void sv_mfma_chunk(const ACC *y, const ACC *z, int qc
         for (int ks = 0; ks < NY / 4; ks++) {
             const ACC *yk = ya + 4 * ks * SV_RS, *zk = zb + 4 * ks * SV_RS;
             const ACC a0 = yk[T.ao[0]], a1 = yk[T.ao[1]], a2 = yk[T.ao[2]];
-            ACC b[6];
+            ACC b[5];
 #pragma unroll
-            for (int t = 0; t < 6; t++) b[t] = zk[T.bo[t]];
+            for (int t = 0; t < 5; t++) b[t] = zk[T.bo[t]];
             acc[0] = mfma16(a0, b[0], acc[0]);
             acc[1] = mfma16(a0, b[1], acc[1]);
             acc[2] = mfma16(a0, b[2], acc[2]);
             acc[3] = mfma16(a1, b[3], acc[3]);
-            acc[4] = mfma16(a1, b[4], acc[4]);
-            acc[5] = mfma16(a2, b[5], acc[5]);
+            acc[4] = mfma16(a2, b[4], acc[4]);
         }
         ya += NY * SV_RS;
         zb += NY * SV_RS;
@@ -1317,28 +1328,29 @@ __device__ __forceinline__ void sv_mfma_chunk(const ACC *y, const ACC *z, int qc
 // the wave's material tiles into the staged 81 x 81 element matrix (both
 // triangles); GAMMA: its Gamma tiles added to the three diagonal blocks
 template <class ACC, bool GAMMA>
-__device__ __forceinline__ void sv_stage(ACC *sKe, int lane, const SvTiles &T, const typename V4<ACC>::t acc[6])
+__device__ __forceinline__ void sv_stage(ACC *sKe, int lane, const SvTiles &T, const typename V4<ACC>::t acc[5])
 {
 #pragma unroll
-    for (int t = 0; t < 6; t++) {
+    for (int t = 0; t < 5; t++) {
         const int ar = sv_arow(T, t), bc = T.bo[t];
-        if ((ar >= 96) != GAMMA) continue;
+        if (bc >= 128 || (ar >= 96) != GAMMA) continue;
         const int C = bc + (lane & 15);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int R = ar + acc_row((ACC)0, lane, r);
-            const int i = (R >> 5) & 3, A = R & 31, j = (C >> 5) & 3, B = C & 31;
-            if (A < 27 && B < 27) {
-                if (GAMMA) {
+            if (GAMMA) {
+                const int A = R - 96, B = C - 96;
+                if (A < 27 && B < 27) {
 #pragma unroll
                     for (int d = 0; d < 3; d++) {
                         sKe[(3 * A + d) * 81 + 3 * B + d] += acc[t][r];
                         if (ar != bc) sKe[(3 * B + d) * 81 + 3 * A + d] += acc[t][r];
                     }
-                } else {
-                    sKe[(3 * A + i) * 81 + 3 * B + j] = acc[t][r];
-                    if (ar != bc) sKe[(3 * B + j) * 81 + 3 * A + i] = acc[t][r];
                 }
+            } else {
+                const int i = R / 27, A = R - 27 * i, j = C / 27, B = C - 27 * j;
+                sKe[(3 * A + i) * 81 + 3 * B + j] = acc[t][r];
+                if (ar != bc) sKe[(3 * B + j) * 81 + 3 * A + i] = acc[t][r];
             }
         }
     }
@@ -1353,8 +1365,12 @@ struct SvDims {
     // prologue use of the operand part: interpolation scratch, U, basis tables
     static constexpr int PRO = (SF_SCRATCH + 2 * 81 + 3 * 4 * 9) * 8;
     static constexpr int PART2 = OPB > PRO ? OPB : PRO;
-    static constexpr int KEB = 81 * 81 * (int)sizeof(ACC);
-    static constexpr int REGION = (PTB + PART2) > KEB ? (PTB + PART2) : KEB;
+    static constexpr int KEB = (81 * 81 * (int)sizeof(ACC) + 15) / 16 * 16;
+    // per (dof block j < 3, item (point, B)) sums of the dof-80 column, FP64,
+    // at the end of the region (the element matrix takes the start)
+    static constexpr int S80B = 3 * 27 * QC * 8;
+    static constexpr int REGION = (PTB + PART2) > KEB + S80B ? (PTB + PART2) : KEB + S80B;
+    static constexpr int S80OFF = REGION - S80B;
 };
 
 template <class ACC, int QC, int GT>
@@ -1372,6 +1388,7 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
     // at the start
     __shared__ __attribute__((aligned(16))) unsigned char sRegion[D::REGION];
     __shared__ ACC sphiA[3 * 4 * 9];
+    __shared__ ACC sZ80[2][QC][NY];  // Z of dof 80 per point of this chunk and the next
     __shared__ int sQ[64];
     __shared__ long long sRowBase[81];
     __shared__ int sRowSx[81], sRowSy[81];
@@ -1481,13 +1498,7 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
         sphiA[tid - 128] = (ACC)sphi[tid - 128];
     }
     __syncthreads();
-    // unused operand columns (27..31 of every 32) and the Gamma block's k-rows
-    // past 12 stay zero from here on
-    for (int idx = tid; idx < QC * NY * 20; idx += 256) {
-        const int r = idx / 20, x = idx % 20, col = 32 * (x / 5) + 27 + x % 5;
-        sY[r * SV_RS + col] = (ACC)0;
-        sZ[r * SV_RS + col] = (ACC)0;
-    }
+    // the Gamma block's k-rows past 12 (getime) stay zero from here on
     if (NY > 12) {
         for (int idx = tid; idx < QC * (NY - 12) * 32; idx += 256) {
             const int r = idx / 32, ql = r / (NY - 12), k = 12 + r % (NY - 12);
@@ -1495,32 +1506,70 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
             sZ[(ql * NY + k) * SV_RS + 96 + idx % 32] = (ACC)0;
         }
     }
-    V acc[6];
+    const ACC ce = (ACC)(2.0 * c.e345), cl = (ACC)(2.0 * c.lsq), mass = (ACC)mass_d;
+    // Z(80) of the first chunk's points
+    if (tid < min(QC, nqp)) {
+        const int q = sQ[tid];
+        ACC n[10], Y[NY], Z[NY];
+        basis_slots<ACC>(sphiA, q & 15, (q >> 4) & 15, (q >> 8) & 15, 26, n);
+        sv_yz<ACC>(sPt[tid], ce, cl, n, 2, Y, Z);
 #pragma unroll
-    for (int t = 0; t < 6; t++)
+        for (int k = 0; k < NY; k++) sZ80[0][tid][k] = Z[k];
+    }
+    __syncthreads();
+
+    V acc[5];
+#pragma unroll
+    for (int t = 0; t < 5; t++)
 #pragma unroll
         for (int r = 0; r < 4; r++) acc[t][r] = (ACC)0;
-    double accd[6][4];
+    double accd[5][4];
 #pragma unroll
-    for (int t = 0; t < 6; t++)
+    for (int t = 0; t < 5; t++)
 #pragma unroll
         for (int r = 0; r < 4; r++) accd[t][r] = 0.0;
     const SvTiles T = sv_tiles(wave);
-    const ACC ce = (ACC)(2.0 * c.e345), cl = (ACC)(2.0 * c.lsq), mass = (ACC)mass_d;
+    constexpr int NSLOT = (27 * QC + 63) / 64;  // items per lane (waves 0-2)
+    double acc80[NSLOT];
+#pragma unroll
+    for (int k = 0; k < NSLOT; k++) acc80[k] = 0.0;
 
     const int nchunk = (nqp + QC - 1) / QC;
     for (int ch = 0; ch < nchunk; ch++) {
         const int qcnt = min(QC, nqp - ch * QC);
-        // operands: wave j < 3 builds Y_j, Z_j, wave 3 the Gamma pair, lane = (point, B)
-        for (int t = lane; t < 27 * qcnt; t += 64) {
-            const int ql = t / 27, B = t - 27 * ql, qp = ch * QC + ql, q = sQ[qp];
-            ACC n[10];
-            basis_slots<ACC>(sphiA, q & 15, (q >> 4) & 15, (q >> 8) & 15, B, n);
-            const Pt &P = sPt[qp];
-            if (wave < 3)
-                sv_item_yz<ACC>(P, ce, cl, n, wave, sY + ql * NY * SV_RS + 32 * wave + B, sZ + ql * NY * SV_RS + 32 * wave + B);
-            else
-                sv_item_gamma<ACC, GT>(P, P.w * mass, n, sY + ql * NY * SV_RS + 96 + B, sZ + ql * NY * SV_RS + 96 + B);
+        // operands: wave j < 3 builds Y_j, Z_j (lane = (point, B)) and the
+        // dof-80 products; wave 3 the Gamma pair, and Z(80) of the next chunk
+        if (wave < 3) {
+#pragma unroll
+            for (int sl = 0; sl < NSLOT; sl++) {
+                const int t = lane + 64 * sl;
+                if (t < 27 * qcnt) {
+                    const int ql = t / 27, B = t - 27 * ql, q = sQ[ch * QC + ql];
+                    ACC n[10];
+                    basis_slots<ACC>(sphiA, q & 15, (q >> 4) & 15, (q >> 8) & 15, B, n);
+                    const int col = 27 * wave + B;
+                    acc80[sl] += (double)sv_item_yz<ACC, GT>(sPt[ch * QC + ql], ce, cl, n, wave, sY + ql * NY * SV_RS + col,
+                                                             sZ + ql * NY * SV_RS + col, sZ80[ch & 1][ql]);
+                }
+            }
+        } else {
+            const int qnext = ch + 1 < nchunk ? min(QC, nqp - (ch + 1) * QC) : 0;
+            for (int t = lane; t < 27 * qcnt + qnext; t += 64) {
+                if (t < 27 * qcnt) {
+                    const int ql = t / 27, B = t - 27 * ql, qp = ch * QC + ql, q = sQ[qp];
+                    ACC n[10];
+                    basis_slots<ACC>(sphiA, q & 15, (q >> 4) & 15, (q >> 8) & 15, B, n);
+                    const Pt &P = sPt[qp];
+                    sv_item_gamma<ACC, GT>(P, P.w * mass, n, sY + ql * NY * SV_RS + 96 + B, sZ + ql * NY * SV_RS + 96 + B);
+                } else {
+                    const int ql = t - 27 * qcnt, qp = (ch + 1) * QC + ql, q = sQ[qp];
+                    ACC n[10], Y[NY], Z[NY];
+                    basis_slots<ACC>(sphiA, q & 15, (q >> 4) & 15, (q >> 8) & 15, 26, n);
+                    sv_yz<ACC>(sPt[qp], ce, cl, n, 2, Y, Z);
+#pragma unroll
+                    for (int k = 0; k < NY; k++) sZ80[(ch + 1) & 1][ql][k] = Z[k];
+                }
+            }
         }
         __syncthreads();
         sv_mfma_chunk<ACC, NY>(sY, sZ, qcnt, lane, T, acc);
@@ -1529,7 +1578,7 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
 #pragma unroll
             for (int r = 0; r < 4; r++) {
 #pragma unroll
-                for (int t = 0; t < 6; t++) {
+                for (int t = 0; t < 5; t++) {
                     accd[t][r] += (double)acc[t][r];
                     acc[t][r] = (ACC)0;
                 }
@@ -1541,13 +1590,30 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
 #pragma unroll
         for (int r = 0; r < 4; r++) {
 #pragma unroll
-            for (int t = 0; t < 6; t++) acc[t][r] = (ACC)accd[t][r];
+            for (int t = 0; t < 5; t++) acc[t][r] = (ACC)accd[t][r];
         }
     }
 
-    // ---- element matrix: material tiles (both triangles), then Gamma on the diagonal blocks
+    // ---- element matrix: material tiles (both triangles) and the dof-80
+    // row and column, then Gamma on the diagonal blocks
     ACC *sKe = reinterpret_cast<ACC *>(sRegion);
+    double *s80 = reinterpret_cast<double *>(sRegion + D::S80OFF);
+    if (wave < 3) {
+#pragma unroll
+        for (int sl = 0; sl < NSLOT; sl++) {
+            const int t = lane + 64 * sl;
+            if (t < 27 * QC) s80[wave * 27 * QC + t] = acc80[sl];
+        }
+    }
     sv_stage<ACC, false>(sKe, lane, T, acc);
+    __syncthreads();
+    if (tid < 81) {
+        const int j = tid / 27, B = tid - 27 * j;
+        double k80 = 0.0;
+        for (int ql = 0; ql < QC; ql++) k80 += s80[j * 27 * QC + ql * 27 + B];
+        sKe[(3 * B + j) * 81 + 80] = (ACC)k80;
+        sKe[80 * 81 + 3 * B + j] = (ACC)k80;
+    }
     __syncthreads();
     if (wave == 3) sv_stage<ACC, true>(sKe, lane, T, acc);
     __syncthreads();
