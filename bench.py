@@ -50,9 +50,9 @@ FLOP_KE_PER_QP = 9 * (2 * 10 * 10 * 27 + 2 * 27 * 10 * 27)
 # counted on igap4h_physics.hpp's residual_grad: 121 adds, 170 multiplies)
 FLOP_RE_PER_QP = 1620 + 1620 + 540 + 291
 PEAK = {"f64": 78.6, "f32": 157.3}   # MI355X dense MFMA TFLOP/s (the vector FP64 / FP32 peaks are the same)
-# MFMA flops the strain-variation tangent executes per Gauss point: 24 tiles
-# of 16 x 16 x 4 per 4 strain components (ge 12, getime 24 per dof)
-SV_MFMA_FLOP_PER_QP = {"ge": 24 * 3 * 2 * 16 * 16 * 4, "getime": 24 * 6 * 2 * 16 * 16 * 4}
+# MFMA flops the strain-variation tangent executes per Gauss point: 4 waves x
+# 5 tile slots of 16 x 16 x 4 per 4 strain components (ge 12, getime 24 per dof)
+SV_MFMA_FLOP_PER_QP = {"ge": 20 * 3 * 2 * 16 * 16 * 4, "getime": 20 * 6 * 2 * 16 * 16 * 4}
 PEAK_HBM = 8000.0                    # GB/s
 
 
