@@ -1420,7 +1420,17 @@ static SpArgs sp_args(const igap4h_ctx *c)
 // element stays) while the level has more than MG_DENSE dofs; the coarsest
 // level is solved with a dense inverse when it has at most MG_DENSE dofs,
 // else by symmetric Gauss-Seidel sweeps.
-constexpr long long MG_DENSE = 400;
+// The inverse is formed on the host at every set-up (Gauss-Jordan, 2 n^3):
+// 200 stops the coarsening at 2^3 elements (192 dofs, ~3 ms); with 400 a 48^3
+// hierarchy ended at 3^3 elements (375 dofs) and the inverse took 24 ms per
+// Newton step, a quarter of a getime time step, for the same iteration
+// count.  On small meshes such a level can be singular (its Galerkin
+// operator sees the fine level's Dirichlet rows masked out, and a coarse
+// space of mostly boundary functions has more of them than the fine
+// interior has dofs): the set-up then drops it once and inverts the level
+// above, as long as that has at most MG_DENSE_MAX dofs.
+constexpr long long MG_DENSE = 200;
+constexpr long long MG_DENSE_MAX = 400;
 
 static int mg_build(igap4h_ctx *c, Solver *s)
 {
@@ -1444,7 +1454,9 @@ static int mg_build(igap4h_ctx *c, Solver *s)
         MGLevel &F = s->mg.back();
         bool any = false;
         for (int d = 0; d < 3; d++) any = any || F.ne[d] >= 2;
-        if (F.ndof <= MG_DENSE || !any) break;
+        // the fine level (its Dirichlet rows masked in the Galerkin product)
+        // is coarsened above MG_DENSE_MAX dofs, the coarse levels above MG_DENSE
+        if (F.ndof <= (s->mg.size() == 1 ? MG_DENSE_MAX : MG_DENSE) || !any) break;
         MGLevel C;
         std::vector<double> kc[3], P[3];
         for (int d = 0; d < 3; d++) {
@@ -1701,6 +1713,42 @@ struct Op {
         }
         return 0;
     }
+    // dense inverse of the coarsest level B (if it is dense): 0, 1 when the
+    // operator is singular, or an error code
+    int dense_setup(MGLevel &B) const
+    {
+        if (!B.dense) return 0;
+        std::vector<SMT> h((size_t)B.nnode * 1125);
+        std::vector<double> A((size_t)B.ndof * B.ndof, 0.0), inv;
+        HIPCHK(hipMemcpyAsync(h.data(), B.sm, h.size() * sizeof(SMT), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (long long n = 0; n < B.nnode; n++) {
+            int a[3] = {(int)(n / ((long long)B.nb[2] * B.nb[1])), (int)((n / B.nb[2]) % B.nb[1]), (int)(n % B.nb[2])};
+            for (int o = 0; o < 125; o++) {
+                int b0 = a[0] + o / 25 - 2, b1 = a[1] + (o / 5) % 5 - 2, b2 = a[2] + o % 5 - 2;
+                if (b0 < 0 || b0 >= B.nb[0] || b1 < 0 || b1 >= B.nb[1] || b2 < 0 || b2 >= B.nb[2]) continue;
+                long long m = ((long long)b0 * B.nb[1] + b1) * B.nb[2] + b2;
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++)
+                        A[(size_t)(3 * n + i) * B.ndof + 3 * m + j] = h[(size_t)n * 1125 + i * 375 + o * 3 + j];
+            }
+        }
+        for (long long r = 0; r < B.ndof; r++) {  // dofs removed with the Dirichlet rows
+            bool empty = true;
+            for (long long q = 0; q < B.ndof && empty; q++) empty = A[(size_t)r * B.ndof + q] == 0.0;
+            if (empty) A[(size_t)r * B.ndof + r] = 1.0;
+        }
+        if (!dense_inverse(A, (int)B.ndof, inv)) return 1;
+        HIPCHK(hipMemcpyAsync(B.Ainv, inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));  // inv is freed on return
+        return 0;
+    }
+    static void free_level(MGLevel &L)
+    {
+        for (double *p : {L.Dblk, L.Dinv, L.x, L.b, L.r, L.xfer_w, L.Ainv, L.d[0], L.d[1], L.ev}) (void)hipFree(p);
+        for (SMT *p : {L.sm, L.stage[0], L.stage[1]}) (void)hipFree(p);
+        (void)hipFree(L.xfer_i);
+    }
     int mg_setup() const
     {
         int rc = 0;
@@ -1734,32 +1782,17 @@ struct Op {
         if (smoother == 1)
             for (size_t l = 0; l + 1 < s->mg.size(); l++)
                 if ((rc = estimate_lmax(l))) return rc;
-        MGLevel &B = s->mg.back();
-        if (B.dense) {
-            std::vector<SMT> h((size_t)B.nnode * 1125);
-            std::vector<double> A((size_t)B.ndof * B.ndof, 0.0), inv;
-            HIPCHK(hipMemcpyAsync(h.data(), B.sm, h.size() * sizeof(SMT), hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            for (long long n = 0; n < B.nnode; n++) {
-                int a[3] = {(int)(n / ((long long)B.nb[2] * B.nb[1])), (int)((n / B.nb[2]) % B.nb[1]), (int)(n % B.nb[2])};
-                for (int o = 0; o < 125; o++) {
-                    int b0 = a[0] + o / 25 - 2, b1 = a[1] + (o / 5) % 5 - 2, b2 = a[2] + o % 5 - 2;
-                    if (b0 < 0 || b0 >= B.nb[0] || b1 < 0 || b1 >= B.nb[1] || b2 < 0 || b2 >= B.nb[2]) continue;
-                    long long m = ((long long)b0 * B.nb[1] + b1) * B.nb[2] + b2;
-                    for (int i = 0; i < 3; i++)
-                        for (int j = 0; j < 3; j++)
-                            A[(size_t)(3 * n + i) * B.ndof + 3 * m + j] = h[(size_t)n * 1125 + i * 375 + o * 3 + j];
-                }
-            }
-            for (long long r = 0; r < B.ndof; r++) {  // dofs removed with the Dirichlet rows
-                bool empty = true;
-                for (long long q = 0; q < B.ndof && empty; q++) empty = A[(size_t)r * B.ndof + q] == 0.0;
-                if (empty) A[(size_t)r * B.ndof + r] = 1.0;
-            }
-            if (!dense_inverse(A, (int)B.ndof, inv)) return fail(IGAP4H_EINVAL, "multigrid: coarsest operator is singular");
-            HIPCHK(hipMemcpyAsync(B.Ainv, inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice, st));
-            HIPCHK(hipStreamSynchronize(st));  // inv is freed on return
+        int dr = dense_setup(s->mg.back());
+        if (dr == 1 && s->mg.size() > 2 && s->mg[s->mg.size() - 2].ndof <= MG_DENSE_MAX) {
+            free_level(s->mg.back());
+            s->mg.pop_back();
+            MGLevel &N = s->mg.back();
+            if ((rc = dev_alloc(&N.Ainv, (size_t)N.ndof * N.ndof))) return rc;
+            N.dense = true;
+            dr = dense_setup(N);
         }
+        if (dr == 1) return fail(IGAP4H_EINVAL, "multigrid: coarsest operator is singular");
+        if (dr) return dr;
         (void)rc;
         return 0;
     }
