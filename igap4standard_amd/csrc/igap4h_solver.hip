@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <future>
 
 #include "igap4h_internal.hpp"
 
@@ -1559,14 +1560,13 @@ static bool dense_inverse(std::vector<double> &A, int n, std::vector<double> &in
             A[(size_t)k * n + c2] *= ip;
             inv[(size_t)k * n + c2] *= ip;
         }
+        // columns c < k of A are unit vectors by now (zero in row k): only c >= k change
         for (int r = 0; r < n; r++) {
             if (r == k) continue;
             double f = A[(size_t)r * n + k];
             if (f == 0.0) continue;
-            for (int c2 = 0; c2 < n; c2++) {
-                A[(size_t)r * n + c2] -= f * A[(size_t)k * n + c2];
-                inv[(size_t)r * n + c2] -= f * inv[(size_t)k * n + c2];
-            }
+            for (int c2 = k; c2 < n; c2++) A[(size_t)r * n + c2] -= f * A[(size_t)k * n + c2];
+            for (int c2 = 0; c2 < n; c2++) inv[(size_t)r * n + c2] -= f * inv[(size_t)k * n + c2];
         }
     }
     return true;
@@ -1713,15 +1713,18 @@ struct Op {
         }
         return 0;
     }
-    // dense inverse of the coarsest level B (if it is dense): 0, 1 when the
-    // operator is singular, or an error code
-    int dense_setup(MGLevel &B) const
+    // the coarsest level's operator on the host (stencil storage)
+    int dense_fetch(const MGLevel &B, std::vector<SMT> &h) const
     {
-        if (!B.dense) return 0;
-        std::vector<SMT> h((size_t)B.nnode * 1125);
-        std::vector<double> A((size_t)B.ndof * B.ndof, 0.0), inv;
+        h.resize((size_t)B.nnode * 1125);
         HIPCHK(hipMemcpyAsync(h.data(), B.sm, h.size() * sizeof(SMT), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        return 0;
+    }
+    // its dense form and inverse (host only); false when it is singular
+    static bool dense_invert(const MGLevel &B, const std::vector<SMT> &h, std::vector<double> &inv)
+    {
+        std::vector<double> A((size_t)B.ndof * B.ndof, 0.0);
         for (long long n = 0; n < B.nnode; n++) {
             int a[3] = {(int)(n / ((long long)B.nb[2] * B.nb[1])), (int)((n / B.nb[2]) % B.nb[1]), (int)(n % B.nb[2])};
             for (int o = 0; o < 125; o++) {
@@ -1738,10 +1741,24 @@ struct Op {
             for (long long q = 0; q < B.ndof && empty; q++) empty = A[(size_t)r * B.ndof + q] == 0.0;
             if (empty) A[(size_t)r * B.ndof + r] = 1.0;
         }
-        if (!dense_inverse(A, (int)B.ndof, inv)) return 1;
+        return dense_inverse(A, (int)B.ndof, inv);
+    }
+    int dense_upload(MGLevel &B, const std::vector<double> &inv) const
+    {
         HIPCHK(hipMemcpyAsync(B.Ainv, inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice, st));
-        HIPCHK(hipStreamSynchronize(st));  // inv is freed on return
+        HIPCHK(hipStreamSynchronize(st));  // inv is freed by the caller
         return 0;
+    }
+    // dense inverse of level B, in order: 0, 1 when the operator is singular,
+    // or an error code
+    int dense_setup(MGLevel &B) const
+    {
+        std::vector<SMT> h;
+        std::vector<double> inv;
+        int rc;
+        if ((rc = dense_fetch(B, h))) return rc;
+        if (!dense_invert(B, h, inv)) return 1;
+        return dense_upload(B, inv);
     }
     static void free_level(MGLevel &L)
     {
@@ -1779,10 +1796,27 @@ struct Op {
             hipLaunchKernelGGL(k_block_diag_sm, dim3(grid_for(C.nnode, 256)), dim3(256), 0, st, C.nnode, (const SMT *)C.sm, C.Dblk, C.Dinv);
             HIPCHK(hipGetLastError());
         }
+        // the coarsest level's inverse is formed on a host thread while the
+        // device runs the eigenvalue estimates
+        MGLevel &B0 = s->mg.back();
+        std::vector<SMT> hB;
+        std::vector<double> hinv;
+        std::future<bool> job;
+        if (B0.dense) {
+            if ((rc = dense_fetch(B0, hB))) return rc;
+            try {
+                job = std::async(std::launch::async, [&B0, &hB, &hinv] { return dense_invert(B0, hB, hinv); });
+            } catch (...) {  // no thread: inverted below, in order
+            }
+        }
         if (smoother == 1)
             for (size_t l = 0; l + 1 < s->mg.size(); l++)
-                if ((rc = estimate_lmax(l))) return rc;
-        int dr = dense_setup(s->mg.back());
+                if ((rc = estimate_lmax(l))) return rc;  // the job's future waits for it
+        int dr = 0;
+        if (B0.dense) {
+            const bool ok = job.valid() ? job.get() : dense_invert(B0, hB, hinv);
+            dr = ok ? dense_upload(B0, hinv) : 1;
+        }
         if (dr == 1 && s->mg.size() > 2 && s->mg[s->mg.size() - 2].ndof <= MG_DENSE_MAX) {
             free_level(s->mg.back());
             s->mg.pop_back();
