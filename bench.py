@@ -7,9 +7,12 @@ A step is one full Newton solve on the GPU from the affine guess u = 0.01 X
 (Dirichlet rows set to their values) to ||R|| / ||R0|| < 1e-10: per Newton
 iteration the FP64 residual, the tangent on the AIJ pattern, GMRES(300) with a
 multigrid V-cycle and the backtracking line search, all with inputs resident
-in HBM.  Two forms: plain FP64 (the primary line) and mixed (FP32-MFMA
-tangent with FP64 accumulation over Gauss-point chunks, FP32 matrix values
-in the Krylov solve and the multigrid hierarchy's fine level).
+in HBM.  Forms: plain FP64 (the primary line); mixed (FP32-MFMA tangent
+with FP64 accumulation over Gauss-point chunks, FP32 matrix values in the
+Krylov solve and the multigrid hierarchy's fine level); mixed_f16s (mixed,
+the fine-level smoothing passes on a row-scaled binary16 copy of the
+tangent); fp64_f16s (FP64 tangent, residual and Krylov solve, only the
+smoothing passes on the binary16 copy).
 
 Also timed: the assembly alone (residual + tangent at a non-equilibrium state
 U = 1e-3 sin(pi X) sin(pi Y) sin(pi Z) (1, .5, .25) + 1e-6 noise) and the
@@ -283,10 +286,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of work per CPU baseline process")
     ap.add_argument("--cpu-procs", type=int, default=0, help="CPU baseline processes (default: cores, at most 16)")
-    ap.add_argument("--forms", default="fp64,mixed,mixed_f16s",
+    ap.add_argument("--forms", default="fp64,mixed,mixed_f16s,fp64_f16s",
                     help="comma list of fp64, mixed, mixed_f16s / mixed_bf16s (the mixed form with the "
                          "fine-level multigrid smoothing passes on a row-scaled binary16 / bfloat16 copy "
-                         "of the FP32 tangent)")
+                         "of the FP32 tangent), fp64_f16s (FP64 tangent and Krylov, smoothing passes on the "
+                         "binary16 copy of the FP64 tangent)")
     ap.add_argument("--no-newton", action="store_true", help="assembly and SpMV only")
     ap.add_argument("--no-extras", action="store_true", help="only the 64^3 headline workload")
     ap.add_argument("--tangent-form", choices=("auto", "tstrip", "sv"), default="auto",
@@ -340,8 +344,9 @@ def main():
     dK32 = torch.empty(app.nnz, device=dev, dtype=torch.float32)
     forms = {"fp64": (L.IGAP4H_FP64, L.IGAP4H_VALS_F64, dK64), "mixed": (L.IGAP4H_FP32, L.IGAP4H_VALS_F32, dK32),
              "mixed_f16s": (L.IGAP4H_FP32, L.IGAP4H_VALS_F32, dK32),
-             "mixed_bf16s": (L.IGAP4H_FP32, L.IGAP4H_VALS_F32, dK32)}
-    smooth = {"mixed_f16s": L.IGAP4H_SMOOTH_F16, "mixed_bf16s": L.IGAP4H_SMOOTH_BF16}
+             "mixed_bf16s": (L.IGAP4H_FP32, L.IGAP4H_VALS_F32, dK32),
+             "fp64_f16s": (L.IGAP4H_FP64, L.IGAP4H_VALS_F64, dK64)}
+    smooth = {"mixed_f16s": L.IGAP4H_SMOOTH_F16, "mixed_bf16s": L.IGAP4H_SMOOTH_BF16, "fp64_f16s": L.IGAP4H_SMOOTH_F16}
     wanted = [f for f in args.forms.split(",") if f in forms]
     wanted_asm = [f for f in wanted if f not in smooth] or ["mixed"]  # the 16-bit forms assemble as mixed
 
@@ -489,7 +494,7 @@ def main():
     if "mixed" in asm and "fp64" in asm:
         line["assembly"]["mixed"]["roofline_frac_f32"] = asm["mixed"]["tangent_tflops"] / PEAK["f32"]
         line["assembly"]["mixed"]["tangent_speedup_vs_fp64"] = asm["fp64"]["tangent_ms"] / asm["mixed"]["tangent_ms"]
-    for name in ("mixed", "mixed_f16s", "mixed_bf16s"):
+    for name in ("mixed", "mixed_f16s", "mixed_bf16s", "fp64_f16s"):
         if name in newton and "fp64" in newton:
             line["newton"][name + "_speedup_vs_fp64"] = newton["fp64"]["solve_s"] / newton[name]["solve_s"]
     line["cpu_baseline"] = cpu

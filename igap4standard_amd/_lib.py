@@ -61,7 +61,7 @@ IGAP4H_VALS_F64, IGAP4H_VALS_F32 = 0, 1
 IGAP4H_PHYS_GE, IGAP4H_PHYS_GETIME, IGAP4H_PHYS_FROM_TORDER = 0, 1, -1
 IGAP4H_PC_NONE, IGAP4H_PC_BJACOBI3, IGAP4H_PC_SGS, IGAP4H_PC_MG = 0, 1, 2, 3
 IGAP4H_KSP_GMRES, IGAP4H_KSP_CG = 0, 1
-IGAP4H_MG_GS, IGAP4H_MG_CHEBYSHEV = 0, 1
+IGAP4H_MG_GS, IGAP4H_MG_CHEBYSHEV, IGAP4H_MG_CHEB4 = 0, 1, 2
 IGAP4H_SMOOTH_SAME, IGAP4H_SMOOTH_BF16, IGAP4H_SMOOTH_F16 = 0, 1, 2
 IGAP4H_TANGENT_AUTO, IGAP4H_TANGENT_TSTRIP, IGAP4H_TANGENT_SV = 0, 1, 2
 IGAP4H_BATCH_MAX_VARIANTS = 8

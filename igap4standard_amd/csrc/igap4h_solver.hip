@@ -1601,13 +1601,14 @@ struct Op {
     hipStream_t st;
     int nu_pre = 1, nu_post = 1;  // Gauss-Seidel sweeps before / after the coarse correction
     int mg_graph = 1;             // replay the V-cycle from a captured graph
-    int smoother = 0;             // 0 block Gauss-Seidel, 1 Chebyshev (degree nu_pre / nu_post)
+    int smoother = 0;             // 0 block Gauss-Seidel, 1 Chebyshev (degree nu_pre / nu_post), 2 fourth-kind Chebyshev
     int svt = IGAP4H_SMOOTH_SAME; // matrix of the fine-level Chebyshev passes
     int cdeg = 0;                 // Chebyshev degree cap below the finest level (0: none)
     double cratio = 200.0;        // Chebyshev interval [lmax' / cratio, lmax'], lmax' = 1.1 lmax
 
     // smoothing degree (Chebyshev) or sweeps (Gauss-Seidel) nu on level l
-    int deg(size_t l, int nu) const { return l && smoother == 1 && cdeg > 0 ? std::min(nu, cdeg) : nu; }
+    bool chebyshev() const { return smoother == IGAP4H_MG_CHEBYSHEV || smoother == IGAP4H_MG_CHEB4; }
+    int deg(size_t l, int nu) const { return l && chebyshev() && cdeg > 0 ? std::min(nu, cdeg) : nu; }
 
     // fine-level pass on the 16-bit copy (k_cheb16): MODE 0 is d = c2 Dinv K~ x,
     // MODE 1 starts Chebyshev from x (r = b - K~ x, d = c2 Dinv r), MODE 2 is one step on din
@@ -1809,7 +1810,7 @@ struct Op {
             } catch (...) {  // no thread: inverted below, in order
             }
         }
-        if (smoother == 1)
+        if (chebyshev())
             for (size_t l = 0; l + 1 < s->mg.size(); l++)
                 if ((rc = estimate_lmax(l))) return rc;  // the job's future waits for it
         int dr = 0;
@@ -1879,32 +1880,33 @@ struct Op {
     // Chebyshev smoothing of degree m on level l for K x = b (from x = 0 if
     // from_zero; started: that start was made by the restriction); with
     // want_res, L.r = b - K x on return.  With pending, the last update
-    // x += d is left to the caller: *pending = d.
+    // x += d is left to the caller: *pending = d.  Both kinds run
+    //   d_0 = s0 Dinv r_0;  x += d_{k-1}, r -= K d_{k-1}, d_k = c1_k d_{k-1} + c2_k Dinv r_k
+    // with the coefficients of cheb_s0 / cheb_c.
     int cheb(size_t l, const double *b, double *x, int m, bool from_zero, bool want_res, bool started = false,
              const double **pending = nullptr) const
     {
         const MGLevel &L = s->mg[l];
         const double *Dinv = l ? L.Dinv : s->Dinv;
-        const double bmax = 1.1 * L.lmax, amin = bmax / cratio;
-        const double theta = cheb_theta(l), delta = 0.5 * (bmax - amin), sigma = theta / delta;
-        double rho = 1.0 / sigma;
+        const double s0 = cheb_s0(l);
         double *r = L.r, *d0 = L.d[0], *d1 = L.d[1];
         const dim3 gn(grid_for(L.nnode, 256));
         int rc;
         if (from_zero) {
             if (!started) {
-                hipLaunchKernelGGL(k_cheb_zero, gn, dim3(256), 0, st, L.nnode, Dinv, b, r, d0, x, 1.0 / theta);
+                hipLaunchKernelGGL(k_cheb_zero, gn, dim3(256), 0, st, L.nnode, Dinv, b, r, d0, x, s0);
                 HIPCHK(hipGetLastError());
             }
         } else if (l == 0 && svt != IGAP4H_SMOOTH_SAME) {
-            if ((rc = cheb16<1>(x, b, r, nullptr, d0, 0.0, 1.0 / theta, 1))) return rc;
+            if ((rc = cheb16<1>(x, b, r, nullptr, d0, 0.0, s0, 1))) return rc;
         } else if (l == 0) {  // fine level: the row pass is the SpMV kernel
             if ((rc = spmv(x, r, b))) return rc;
-            hipLaunchKernelGGL(k_cheb_dfromr, gn, dim3(256), 0, st, L.nnode, Dinv, (const double *)r, d0, 1.0 / theta);
+            hipLaunchKernelGGL(k_cheb_dfromr, gn, dim3(256), 0, st, L.nnode, Dinv, (const double *)r, d0, s0);
             HIPCHK(hipGetLastError());
         } else {
-            LEVEL_LAUNCH(l, k_cheb_init, Dinv, b, (const double *)x, r, d0, 1.0 / theta);
+            LEVEL_LAUNCH(l, k_cheb_init, Dinv, b, (const double *)x, r, d0, s0);
         }
+        double rho = 0.0;
         for (int k = 1; k <= m; k++) {
             const bool last = k == m;
             if (last && !want_res && pending) {
@@ -1916,27 +1918,50 @@ struct Op {
                 HIPCHK(hipGetLastError());
                 break;
             }
-            const double rn = 1.0 / (2.0 * sigma - rho);
+            double c1, c2;
+            cheb_c(l, k, &rho, &c1, &c2);
             if (l == 0 && svt != IGAP4H_SMOOTH_SAME) {
-                if ((rc = cheb16<2>(d0, nullptr, r, x, d1, rn * rho, 2.0 * rn / delta, last ? 0 : 1))) return rc;
+                if ((rc = cheb16<2>(d0, nullptr, r, x, d1, c1, c2, last ? 0 : 1))) return rc;
             } else if (l == 0) {
                 if ((rc = spmv(d0, L.x))) return rc;
                 hipLaunchKernelGGL(k_cheb_update, gn, dim3(256), 0, st, L.nnode, Dinv, (const double *)L.x, (const double *)d0, d1, r, x,
-                                   rn * rho, 2.0 * rn / delta, last ? 0 : 1);
+                                   c1, c2, last ? 0 : 1);
                 HIPCHK(hipGetLastError());
             } else {
-                LEVEL_LAUNCH(l, k_cheb_step, Dinv, (const double *)d0, d1, r, x, rn * rho, 2.0 * rn / delta, last ? 0 : 1);
+                LEVEL_LAUNCH(l, k_cheb_step, Dinv, (const double *)d0, d1, r, x, c1, c2, last ? 0 : 1);
             }
-            rho = rn;
             std::swap(d0, d1);
         }
         return 0;
     }
-    // Chebyshev's theta = (bmax + amin) / 2 on level l
-    double cheb_theta(size_t l) const
+    // the first direction's factor s0 on level l: 1 / theta, theta = (bmax +
+    // amin) / 2 over the interval [amin, bmax] = [1.1 lmax / ratio, 1.1 lmax]
+    // (first kind); 4 / (3 bmax) (fourth kind)
+    double cheb_s0(size_t l) const
     {
         const double bmax = 1.1 * s->mg[l].lmax, amin = bmax / cratio;
-        return 0.5 * (bmax + amin);
+        return smoother == IGAP4H_MG_CHEB4 ? 4.0 / (3.0 * bmax) : 2.0 / (bmax + amin);
+    }
+    // step k's (c1, c2); rho carries the first kind's recurrence (0 before step 1).
+    // First kind, delta = (bmax - amin) / 2, sigma = theta / delta:
+    //   rho_0 = 1 / sigma, rho_k = 1 / (2 sigma - rho_{k-1}), c1 = rho_k rho_{k-1}, c2 = 2 rho_k / delta.
+    // Fourth kind (Lottes, "Optimal polynomial smoothers for multigrid
+    // V-cycles", Numer. Linear Algebra Appl. 2023, unit weights):
+    //   c1 = (2k - 1) / (2k + 3), c2 = (8k + 4) / ((2k + 3) bmax).
+    void cheb_c(size_t l, int k, double *rho, double *c1, double *c2) const
+    {
+        const double bmax = 1.1 * s->mg[l].lmax, amin = bmax / cratio;
+        if (smoother == IGAP4H_MG_CHEB4) {
+            *c1 = (2.0 * k - 1.0) / (2.0 * k + 3.0);
+            *c2 = (8.0 * k + 4.0) / ((2.0 * k + 3.0) * bmax);
+            return;
+        }
+        const double theta = 0.5 * (bmax + amin), delta = 0.5 * (bmax - amin), sigma = theta / delta;
+        if (k == 1) *rho = 1.0 / sigma;
+        const double rn = 1.0 / (2.0 * sigma - *rho);
+        *c1 = rn * *rho;
+        *c2 = 2.0 * rn / delta;
+        *rho = rn;
     }
     // x = V-cycle(b) on level l (x overwritten); started and pending as in cheb
     int vcycle(size_t l, const double *b, double *x, bool started = false, const double **pending = nullptr) const
@@ -1957,7 +1982,7 @@ struct Op {
             return 0;
         }
         const MGLevel &C = s->mg[l + 1];
-        if (smoother == 1) {
+        if (chebyshev()) {
             const double *res = b;
             const int pre = deg(l, nu_pre), post = deg(l, nu_post);
             if (pre > 0) {
@@ -1971,7 +1996,7 @@ struct Op {
             // (unless that level is the coarsest), and the prolongation adds
             // the coarse smoother's last update
             const bool cstart = deg(l + 1, nu_pre) > 0 && l + 2 < s->mg.size();
-            const double cinv = cstart ? 1.0 / cheb_theta(l + 1) : 0.0;
+            const double cinv = cstart ? cheb_s0(l + 1) : 0.0;
             hipLaunchKernelGGL(k_restrict, dim3(grid_for(C.nnode, 4)), dim3(256), 0, st, L.T, res, C.b,
                                l ? (const unsigned char *)nullptr : (const unsigned char *)c->d_mask,
                                cstart ? (const double *)C.Dinv : nullptr, C.r, C.x, C.d[0], cinv);
@@ -2075,7 +2100,7 @@ struct Op {
         if (pc == IGAP4H_PC_MG) {
             int rc = mg_build(c, s);
             if (rc) return rc;
-            if (smoother == 1 && svt != IGAP4H_SMOOTH_SAME && (rc = pack_16())) return rc;
+            if (chebyshev() && svt != IGAP4H_SMOOTH_SAME && (rc = pack_16())) return rc;
             // the V-cycle graph holds this set-up's Chebyshev coefficients (from
             // lmax) as kernel arguments: capture it again
             s->key_vals = nullptr;
@@ -2328,7 +2353,8 @@ int check_opts(const igap4h_newton_opts *o)
     if (o->ksp_max_it < 1) return fail(IGAP4H_EINVAL, "ksp_max_it must be >= 1");
     if (o->pc == IGAP4H_PC_MG && (o->mg_pre < 0 || o->mg_pre > 8 || o->mg_post < 0 || o->mg_post > 8 || o->mg_pre + o->mg_post < 1))
         return fail(IGAP4H_EINVAL, "multigrid sweeps pre %d / post %d outside 0..8 (sum >= 1)", o->mg_pre, o->mg_post);
-    if (o->mg_smoother != IGAP4H_MG_GS && o->mg_smoother != IGAP4H_MG_CHEBYSHEV) return fail(IGAP4H_EINVAL, "bad mg_smoother %d", o->mg_smoother);
+    if (o->mg_smoother != IGAP4H_MG_GS && o->mg_smoother != IGAP4H_MG_CHEBYSHEV && o->mg_smoother != IGAP4H_MG_CHEB4)
+        return fail(IGAP4H_EINVAL, "bad mg_smoother %d", o->mg_smoother);
     if (o->mg_smooth_vals != IGAP4H_SMOOTH_SAME && o->mg_smooth_vals != IGAP4H_SMOOTH_BF16 && o->mg_smooth_vals != IGAP4H_SMOOTH_F16)
         return fail(IGAP4H_EINVAL, "bad mg_smooth_vals %d", o->mg_smooth_vals);
     if (o->mg_coarse_deg < 0 || o->mg_coarse_deg > 8) return fail(IGAP4H_EINVAL, "mg_coarse_deg %d outside 0..8", o->mg_coarse_deg);

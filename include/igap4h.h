@@ -350,7 +350,7 @@ int igap4h_batch_assemble(igap4h_batch *batch, void *stream);
  * solve is bitwise reproducible on one device.                            */
 enum { IGAP4H_PC_NONE = 0, IGAP4H_PC_BJACOBI3 = 1, IGAP4H_PC_SGS = 2, IGAP4H_PC_MG = 3 };
 enum { IGAP4H_KSP_GMRES = 0, IGAP4H_KSP_CG = 1 };
-enum { IGAP4H_MG_GS = 0, IGAP4H_MG_CHEBYSHEV = 1 };
+enum { IGAP4H_MG_GS = 0, IGAP4H_MG_CHEBYSHEV = 1, IGAP4H_MG_CHEB4 = 2 };
 enum { IGAP4H_SMOOTH_SAME = 0, IGAP4H_SMOOTH_BF16 = 1, IGAP4H_SMOOTH_F16 = 2 };
 
 typedef struct {
@@ -387,14 +387,17 @@ typedef struct {
                           (0..8 each, sum >= 1; default 5 / 5)                     */
     int mg_graph;      /* IGAP4H_PC_MG: 1 replays the V-cycle from a captured HIP graph */
     int mg_smoother;   /* IGAP4H_PC_MG: IGAP4H_MG_GS (block Gauss-Seidel, mg_pre/mg_post
-                          sweeps) or IGAP4H_MG_CHEBYSHEV (block-Jacobi Chebyshev of
-                          degree mg_pre/mg_post, largest eigenvalue by power iteration) */
+                          sweeps), IGAP4H_MG_CHEBYSHEV (block-Jacobi Chebyshev of
+                          degree mg_pre/mg_post over [1.1 lmax / mg_cheb_ratio,
+                          1.1 lmax], lmax by power iteration) or IGAP4H_MG_CHEB4
+                          (Chebyshev polynomials of the fourth kind on [0, 1.1 lmax],
+                          Lottes 2023; no lower bound, mg_cheb_ratio unused)       */
     int lag_jacobian;  /* rebuild the tangent and preconditioner every lag_jacobian-th
                           Newton step (PETSc -snes_lag_jacobian; 1 = every step, the
                           reference)                                              */
     int lag_persist;   /* 1: the lag counts across solves (-snes_lag_jacobian_persists),
                           so a time stepper reuses the tangent of earlier steps    */
-    int mg_smooth_vals;/* IGAP4H_PC_MG with IGAP4H_MG_CHEBYSHEV: matrix of the fine-level
+    int mg_smooth_vals;/* IGAP4H_PC_MG with a Chebyshev smoother: matrix of the fine-level
                           smoothing passes.  IGAP4H_SMOOTH_SAME: the Krylov matrix
                           (vals_type); IGAP4H_SMOOTH_BF16: a bfloat16 copy made when
                           the preconditioner is set up (half the bytes of FP32; the
@@ -403,7 +406,7 @@ typedef struct {
                           power of two each (entries in (-1, 1), 2^-11 rounding).
                           The 16-bit copies need at most 617 nodes in direction 2
                           (else the solve returns IGAP4H_EINVAL)                   */
-    int mg_coarse_deg; /* IGAP4H_MG_CHEBYSHEV: Chebyshev degree before / after the coarse
+    int mg_coarse_deg; /* Chebyshev smoothers: degree before / after the coarse
                           correction on the levels below the finest, at most
                           mg_pre / mg_post (0..8; 0: mg_pre / mg_post as on the
                           finest level; default 1)                                 */
