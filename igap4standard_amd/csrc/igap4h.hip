@@ -558,7 +558,6 @@ __global__ __launch_bounds__(256) void k_gather_finish(MeshArgs m, const double 
 // ============================================================== tangent
 // One work-group of 256 threads (4 waves) per element of one colour; each
 // wave owns 6 of the 24 16x16 output tiles of the i <= j dof blocks.
-constexpr int TNT = 256;  // threads per element
 
 // Column tiles: row-block i of K holds columns (j >= i, B) padded to 32 per
 // j-block: widths 96 | 64 | 32 at offsets 0 | 96 | 160 of a 192-wide strip.
@@ -566,34 +565,76 @@ __device__ __forceinline__ int ctile_block(int ct) { return ct < 6 ? 0 : (ct < 1
 __device__ __forceinline__ int block_off(int i) { return i == 0 ? 0 : (i == 1 ? 96 : 160); }
 
 // Read-modify-write of the staged 81x81 element matrix into the CSR values,
-// all loads of a batch issued before its stores.
-template <class VAL, class ACC, int NTHR = TNT>
-__device__ __forceinline__ void scatter_rows(const ACC *sKe, const long long *sRowBase, const int *sRowSx,
-                                             const int *sRowSy, const unsigned char *sMask, VAL *vals, int tid)
+// in two steps so that the loads can be issued before the element matrix is
+// staged (or earlier): thread t < 243 owns column cc = t mod 81
+// of the element matrix and rows t / 81 + 3 k (k = 0..26); the column's offset
+// within a CSR row (x and y node steps, 3 z + dof) is decoded once, a row adds
+// its base and strides.  No other element of a colour touches these values, so
+// they may be read any time after the previous colour's launch.
+struct ScatterCol {
+    int cc, r0, cx, cy, cz;
+    bool act;
+};
+__device__ __forceinline__ ScatterCol scatter_col(int tid, const unsigned char *sMask)
 {
-    constexpr int NB = 13; // values per thread per batch (13 x 512 >= 81 x 81)
-    for (int base = 0; base < 81 * 81; base += NB * NTHR) {
-        long long pos[NB];
-        VAL add[NB], old[NB];
+    ScatterCol s;
+    s.act = tid < 243;
+    s.cc = tid % 81;
+    s.r0 = s.act ? tid / 81 : 0;
+    const int B = s.cc / 3;
+    s.cx = B / 9;
+    s.cy = (B / 3) % 3;
+    s.cz = 3 * (B % 3) + s.cc % 3;
+    s.act = s.act && !sMask[s.cc];
+    return s;
+}
+template <class VAL>
+__device__ __forceinline__ void scatter_load(const ScatterCol &s, const long long *sRowBase, const int *sRowSx,
+                                             const int *sRowSy, const unsigned char *sMask, const VAL *vals, VAL old[27])
+{
 #pragma unroll
-        for (int m = 0; m < NB; m++) {
-            int idx = base + m * NTHR + tid;
-            pos[m] = -1;
-            if (idx < 81 * 81) {
-                int r = idx / 81, cc = idx % 81;
-                if (!(sMask[r] | sMask[cc])) {
-                    int B = cc / 3, j = cc % 3;
-                    pos[m] = sRowBase[r] + (B / 9) * sRowSx[r] + ((B / 3) % 3) * sRowSy[r] + 3 * (B % 3) + j;
-                    add[m] = (VAL)sKe[idx];
-                }
-            }
-        }
+    for (int k = 0; k < 27; k++) {
+        const int r = s.r0 + 3 * k;
+        if (s.act && !sMask[r]) old[k] = vals[sRowBase[r] + (s.cx * sRowSx[r] + s.cy * sRowSy[r] + s.cz)];
+    }
+}
+template <class VAL, class ACC>
+__device__ __forceinline__ void scatter_store(const ScatterCol &s, const ACC *sKe, const long long *sRowBase,
+                                              const int *sRowSx, const int *sRowSy, const unsigned char *sMask, VAL *vals,
+                                              const VAL old[27])
+{
+    // every sum first, then the stores back to back: loads and stores share one
+    // counter, so a load still pending between two stores would make each store
+    // wait for the ones before it
+    VAL nv[27];
 #pragma unroll
-        for (int m = 0; m < NB; m++)
-            if (pos[m] >= 0) old[m] = vals[pos[m]];
+    for (int k = 0; k < 27; k++) nv[k] = old[k] + (VAL)sKe[(s.r0 + 3 * k) * 81 + s.cc];
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the loads are in, none left between the stores
 #pragma unroll
-        for (int m = 0; m < NB; m++)
-            if (pos[m] >= 0) vals[pos[m]] = old[m] + add[m];
+    for (int k = 0; k < 27; k++) {
+        const int r = s.r0 + 3 * k;
+        if (s.act && !sMask[r]) vals[sRowBase[r] + (s.cx * sRowSx[r] + s.cy * sRowSy[r] + s.cz)] = nv[k];
+    }
+}
+
+// Loads of the element's CSR values, then stage() (which stages the element
+// matrix in sKe and ends with a barrier), then the stores.
+template <class ACC, class STAGE>
+__device__ __forceinline__ void scatter_staged(int vals_f32, void *vals, const ACC *sKe, const long long *sRowBase,
+                                               const int *sRowSx, const int *sRowSy, const unsigned char *sMask, int tid,
+                                               STAGE stage)
+{
+    const ScatterCol sc = scatter_col(tid, sMask);
+    if (vals_f32) {
+        float old[27];
+        scatter_load<float>(sc, sRowBase, sRowSx, sRowSy, sMask, (const float *)vals, old);
+        stage();
+        scatter_store<float, ACC>(sc, sKe, sRowBase, sRowSx, sRowSy, sMask, (float *)vals, old);
+    } else {
+        double old[27];
+        scatter_load<double>(sc, sRowBase, sRowSx, sRowSy, sMask, (const double *)vals, old);
+        stage();
+        scatter_store<double, ACC>(sc, sKe, sRowBase, sRowSx, sRowSy, sMask, (double *)vals, old);
     }
 }
 
@@ -816,28 +857,26 @@ __device__ __forceinline__ void tangent_ge_body(const MeshArgs &m, const double 
             for (int r = 0; r < 4; r++) acc[t][r] = (ACC)accd[t][r];
     }
 
-    // stage the 81x81 element matrix (mirror the i < j blocks), then scatter
+    // stage the 81x81 element matrix (mirror the i < j blocks) while the CSR values load, then scatter
     ACC *sKe = reinterpret_cast<ACC *>(sRegion);
+    scatter_staged<ACC>(vals_f32, vals, sKe, sRowBase, sRowSx, sRowSy, sMask, tid, [&]() {
 #pragma unroll
-    for (int t = 0; t < 6; t++) {
-        int ct = 3 * wave + t / 2, at = t & 1;
-        int i = ctile_block(ct);
-        int cl = ct * 16 + (lane & 15) - block_off(i);
-        if (cl >= (3 - i) * 27) continue;
-        int j = i + cl / 27, B = cl % 27;
+        for (int t = 0; t < 6; t++) {
+            int ct = 3 * wave + t / 2, at = t & 1;
+            int i = ctile_block(ct);
+            int cl = ct * 16 + (lane & 15) - block_off(i);
+            if (cl >= (3 - i) * 27) continue;
+            int j = i + cl / 27, B = cl % 27;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            int A = at * 16 + acc_row((ACC)0, lane, r);
-            if (A >= 27) continue;
-            sKe[(3 * A + i) * 81 + 3 * B + j] = acc[t][r];
-            if (i != j) sKe[(3 * B + j) * 81 + 3 * A + i] = acc[t][r];
+            for (int r = 0; r < 4; r++) {
+                int A = at * 16 + acc_row((ACC)0, lane, r);
+                if (A >= 27) continue;
+                sKe[(3 * A + i) * 81 + 3 * B + j] = acc[t][r];
+                if (i != j) sKe[(3 * B + j) * 81 + 3 * A + i] = acc[t][r];
+            }
         }
-    }
-    __syncthreads();
-    if (vals_f32)
-        scatter_rows<float, ACC, 256>(sKe, sRowBase, sRowSx, sRowSy, sMask, (float *)vals, tid);
-    else
-        scatter_rows<double, ACC, 256>(sKe, sRowBase, sRowSx, sRowSy, sMask, (double *)vals, tid);
+        __syncthreads();
+    });
 }
 
 template <class ACC, int QC>
@@ -1071,26 +1110,24 @@ __device__ __forceinline__ void tangent_gt_body(const MeshArgs &m, const double 
     }
 
     ACC *sKe = reinterpret_cast<ACC *>(sRegion);
+    scatter_staged<ACC>(vals_f32, vals, sKe, sRowBase, sRowSx, sRowSy, sMask, tid, [&]() {
 #pragma unroll
-    for (int t = 0; t < 6; t++) {
-        int ct = 3 * wave + t / 2, at = t & 1;
-        int i = ctile_block(ct);
-        int cl = ct * 16 + (lane & 15) - block_off(i);
-        if (cl >= (3 - i) * 27) continue;
-        int j = i + cl / 27, B = cl % 27;
+        for (int t = 0; t < 6; t++) {
+            int ct = 3 * wave + t / 2, at = t & 1;
+            int i = ctile_block(ct);
+            int cl = ct * 16 + (lane & 15) - block_off(i);
+            if (cl >= (3 - i) * 27) continue;
+            int j = i + cl / 27, B = cl % 27;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            int A = at * 16 + acc_row((ACC)0, lane, r);
-            if (A >= 27) continue;
-            sKe[(3 * A + i) * 81 + 3 * B + j] = acc[t][r];
-            if (i != j) sKe[(3 * B + j) * 81 + 3 * A + i] = acc[t][r];
+            for (int r = 0; r < 4; r++) {
+                int A = at * 16 + acc_row((ACC)0, lane, r);
+                if (A >= 27) continue;
+                sKe[(3 * A + i) * 81 + 3 * B + j] = acc[t][r];
+                if (i != j) sKe[(3 * B + j) * 81 + 3 * A + i] = acc[t][r];
+            }
         }
-    }
-    __syncthreads();
-    if (vals_f32)
-        scatter_rows<float, ACC, 256>(sKe, sRowBase, sRowSx, sRowSy, sMask, (float *)vals, tid);
-    else
-        scatter_rows<double, ACC, 256>(sKe, sRowBase, sRowSx, sRowSy, sMask, (double *)vals, tid);
+        __syncthreads();
+    });
 }
 
 // Gauss points per chunk of the getime tangent
@@ -1356,12 +1393,13 @@ __device__ __forceinline__ void sv_stage(ACC *sKe, int lane, const SvTiles &T, c
     }
 }
 
-template <class ACC, int QC, int GT>
+// NQM: Gauss points per direction the point data are sized for
+template <class ACC, int QC, int GT, int NQM = 4>
 struct SvDims {
     static constexpr int NY = GT ? 24 : 12;
     static constexpr int YSZ = QC * NY * SV_RS;   // one A (or B) strip of a chunk, elements
     static constexpr int OPB = 2 * YSZ * (int)sizeof(ACC);
-    static constexpr int PTB = 64 * (int)sizeof(SvPoint<ACC, GT>);
+    static constexpr int PTB = NQM * NQM * NQM * (int)sizeof(SvPoint<ACC, GT>);
     // prologue use of the operand part: interpolation scratch, U, basis tables
     static constexpr int PRO = (SF_SCRATCH + 2 * 81 + 3 * 4 * 9) * 8;
     static constexpr int PART2 = OPB > PRO ? OPB : PRO;
@@ -1373,13 +1411,13 @@ struct SvDims {
     static constexpr int S80OFF = REGION - S80B;
 };
 
-template <class ACC, int QC, int GT>
+template <class ACC, int QC, int GT, int NQM = 4>
 __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double *__restrict__ Uh, const double *__restrict__ coef,
                                                 double mass_d, const long long *__restrict__ rowptr,
                                                 const unsigned char *__restrict__ dmask, void *__restrict__ vals, int vals_f32,
                                                 const int ev[3])
 {
-    using D = SvDims<ACC, QC, GT>;
+    using D = SvDims<ACC, QC, GT, NQM>;
     using Pt = SvPoint<ACC, GT>;
     using V = typename V4<ACC>::t;
     constexpr int NY = D::NY;
@@ -1534,6 +1572,12 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
 #pragma unroll
     for (int k = 0; k < NSLOT; k++) acc80[k] = 0.0;
 
+    // FP32 values: the element's CSR values are read now and arrive during the
+    // chunk loop (27 registers); FP64 values are read before the staging
+    const ScatterCol sc = scatter_col(tid, sMask);
+    float oldf[27];
+    if (vals_f32) scatter_load<float>(sc, sRowBase, sRowSx, sRowSy, sMask, (const float *)vals, oldf);
+
     const int nchunk = (nqp + QC - 1) / QC;
     for (int ch = 0; ch < nchunk; ch++) {
         const int qcnt = min(QC, nqp - ch * QC);
@@ -1598,29 +1642,36 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
     // row and column, then Gamma on the diagonal blocks
     ACC *sKe = reinterpret_cast<ACC *>(sRegion);
     double *s80 = reinterpret_cast<double *>(sRegion + D::S80OFF);
-    if (wave < 3) {
+    auto stage = [&]() {
+        if (wave < 3) {
 #pragma unroll
-        for (int sl = 0; sl < NSLOT; sl++) {
-            const int t = lane + 64 * sl;
-            if (t < 27 * QC) s80[wave * 27 * QC + t] = acc80[sl];
+            for (int sl = 0; sl < NSLOT; sl++) {
+                const int t = lane + 64 * sl;
+                if (t < 27 * QC) s80[wave * 27 * QC + t] = acc80[sl];
+            }
         }
+        sv_stage<ACC, false>(sKe, lane, T, acc);
+        __syncthreads();
+        if (tid < 81) {
+            const int j = tid / 27, B = tid - 27 * j;
+            double k80 = 0.0;
+            for (int ql = 0; ql < QC; ql++) k80 += s80[j * 27 * QC + ql * 27 + B];
+            sKe[(3 * B + j) * 81 + 80] = (ACC)k80;
+            sKe[80 * 81 + 3 * B + j] = (ACC)k80;
+        }
+        __syncthreads();
+        if (wave == 3) sv_stage<ACC, true>(sKe, lane, T, acc);
+        __syncthreads();
+    };
+    if (vals_f32) {
+        stage();
+        scatter_store<float, ACC>(sc, sKe, sRowBase, sRowSx, sRowSy, sMask, (float *)vals, oldf);
+    } else {
+        double old[27];
+        scatter_load<double>(sc, sRowBase, sRowSx, sRowSy, sMask, (const double *)vals, old);
+        stage();
+        scatter_store<double, ACC>(sc, sKe, sRowBase, sRowSx, sRowSy, sMask, (double *)vals, old);
     }
-    sv_stage<ACC, false>(sKe, lane, T, acc);
-    __syncthreads();
-    if (tid < 81) {
-        const int j = tid / 27, B = tid - 27 * j;
-        double k80 = 0.0;
-        for (int ql = 0; ql < QC; ql++) k80 += s80[j * 27 * QC + ql * 27 + B];
-        sKe[(3 * B + j) * 81 + 80] = (ACC)k80;
-        sKe[80 * 81 + 3 * B + j] = (ACC)k80;
-    }
-    __syncthreads();
-    if (wave == 3) sv_stage<ACC, true>(sKe, lane, T, acc);
-    __syncthreads();
-    if (vals_f32)
-        scatter_rows<float, ACC, 256>(sKe, sRowBase, sRowSx, sRowSy, sMask, (float *)vals, tid);
-    else
-        scatter_rows<double, ACC, 256>(sKe, sRowBase, sRowSx, sRowSy, sMask, (double *)vals, tid);
 }
 
 // two work-groups per CU (one for getime FP64, whose LDS allows one)
@@ -2707,9 +2758,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     tangent_ge_body<ACC, QC>(m, b.Uh, b.coef, b.rowptr, b.mask, b.vals, vals_f32, ev);
 }
 
-// the same with the strain-variation form of the tangent
-template <class ACC, int QC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_tangent_sv_batch(
+// the same with the strain-variation form of the tangent; NQM: point data for
+// up to NQM^3 Gauss points, WPE: work-groups per CU the registers are sized for
+template <class ACC, int QC, int NQM, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_tangent_sv_batch(
     const BatchInst *__restrict__ I, const long long *__restrict__ coff, int n, int vals_f32, int cx, int cy, int cz)
 {
     const int i = batch_find(coff, n, blockIdx.x);
@@ -2718,23 +2770,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const long long t = (long long)blockIdx.x - coff[i];
     const int my = (m.ne[1] - cy + 2) / 3, mz = (m.ne[2] - cz + 2) / 3;
     const int ev[3] = {cx + 3 * (int)(t / ((long long)mz * my)), cy + 3 * (int)((t / mz) % my), cz + 3 * (int)(t % mz)};
-    tangent_sv_body<ACC, QC, 0>(m, b.Uh, b.coef, 0.0, b.rowptr, b.mask, b.vals, vals_f32, ev);
+    tangent_sv_body<ACC, QC, 0, NQM>(m, b.Uh, b.coef, 0.0, b.rowptr, b.mask, b.vals, vals_f32, ev);
 }
 
 // The tuning space, variant = 10 x form + Gauss points per MFMA chunk: form
 // 0 = T strip (contraction depth 9 QC per pass), 1 = strain variation (12 QC
-// per pass, 21 + 3 tiles); 0 ends a list.
+// per pass, 15 + 3 tiles), 2 = strain variation with the point data sized for
+// 3^3 Gauss points and registers for 4 work-groups per CU (instances with
+// nq <= 3 only); 0 ends a list.
 constexpr int kBatchNV = 8;
-static const int kBatchVariants[2][kBatchNV] = {{1, 2, 3, 11, 0, 0, 0, 0}, {2, 3, 4, 12, 14, 0, 0, 0}};  // [FP64, FP32]
-static bool batch_variant_ok(int prec, int v)
+static const int kBatchVariants[2][kBatchNV] = {{1, 2, 3, 11, 0, 0, 0, 0}, {2, 3, 4, 12, 14, 22, 0, 0}};  // [FP64, FP32]
+static bool batch_variant_ok(int prec, int v, int nqmax)
 {
+    if (v / 10 == 2 && nqmax > 3) return false;
     for (int k = 0; k < kBatchNV; k++)
         if (v && kBatchVariants[prec == IGAP4H_FP32][k] == v) return true;
     return false;
 }
 
 struct igap4h_batch {
-    int n = 0, prec = 0, vals_f32 = 0, device = 0;
+    int n = 0, prec = 0, vals_f32 = 0, device = 0, nqmax = 0;
     std::vector<BatchInst> h_inst;
     BatchInst *d_inst = nullptr;
     long long *d_off = nullptr;             // [eoff | coff (27) | goff | foff | doff], n + 1 each
@@ -2783,7 +2838,7 @@ static int batch_refresh(igap4h_batch *b)
     return 0;
 }
 
-template <class ACC, int QC, int SV>
+template <class ACC, int QC, int SV, int NQM = 4, int WPE = 2>
 static int batch_tangent_launches(igap4h_batch *b, hipStream_t st)
 {
     for (int c = 0; c < 27; c++) {
@@ -2792,7 +2847,7 @@ static int batch_tangent_launches(igap4h_batch *b, hipStream_t st)
         if (nb <= 0) continue;
         const long long *d_coff = b->d_off + (b->n + 1) * (1 + c);
         if constexpr (SV)
-            hipLaunchKernelGGL((k_tangent_sv_batch<ACC, QC>), dim3((unsigned)nb), dim3(256), 0, st, (const BatchInst *)b->d_inst,
+            hipLaunchKernelGGL((k_tangent_sv_batch<ACC, QC, NQM, WPE>), dim3((unsigned)nb), dim3(256), 0, st, (const BatchInst *)b->d_inst,
                                d_coff, b->n, b->vals_f32, c / 9, (c / 3) % 3, c % 3);
         else
             hipLaunchKernelGGL((k_tangent_ge_batch<ACC, QC>), dim3((unsigned)nb), dim3(256), 0, st, (const BatchInst *)b->d_inst,
@@ -2810,6 +2865,7 @@ static int batch_tangent(igap4h_batch *b, int v, hipStream_t st)
         if (v == 4) return batch_tangent_launches<float, 4, 0>(b, st);
         if (v == 12) return batch_tangent_launches<float, 2, 1>(b, st);
         if (v == 14) return batch_tangent_launches<float, 4, 1>(b, st);
+        if (v == 22 && b->nqmax <= 3) return batch_tangent_launches<float, 2, 1, 3, 4>(b, st);
     } else {
         if (v == 1) return batch_tangent_launches<double, 1, 0>(b, st);
         if (v == 2) return batch_tangent_launches<double, 2, 0>(b, st);
@@ -2890,6 +2946,7 @@ extern "C" int igap4h_batch_create(igap4h_ctx *const *ctxs, int n, const double 
             igap4h_batch_destroy(b);
             return fail(IGAP4H_EINVAL, "batch instance %d is a partitioned block (whole patches only)", i);
         }
+        b->nqmax = std::max(b->nqmax, c->nquad);
         if (i == 0) b->device = c->device;
         else if (c->device != b->device) {
             igap4h_batch_destroy(b);
@@ -2958,7 +3015,8 @@ extern "C" int igap4h_batch_sizes(const igap4h_batch *b, int64_t *nelem, int *qc
 extern "C" int igap4h_batch_set_variant(igap4h_batch *b, int qc)
 {
     if (!b) return fail(IGAP4H_EINVAL, "NULL batch");
-    if (!batch_variant_ok(b->prec, qc)) return fail(IGAP4H_EINVAL, "variant %d is not in the tuning space", qc);
+    if (!batch_variant_ok(b->prec, qc, b->nqmax))
+        return fail(IGAP4H_EINVAL, "variant %d is not in the tuning space of this batch", qc);
     if (qc != b->qc) batch_drop_graph(b);
     b->qc = qc;
     return 0;
@@ -2986,6 +3044,7 @@ extern "C" int igap4h_batch_tune(igap4h_batch *b, int reps, int *best_qc, double
     double best = 1e300;
     int bq = b->qc, rc = 0;
     for (int k = 0; k < kBatchNV && cand[k] && !rc; k++) {
+        if (!batch_variant_ok(b->prec, cand[k], b->nqmax)) continue;
         if ((rc = batch_tangent(b, cand[k], b->cap))) break;  // warm-up
         if (hipEventRecord(e0, b->cap) != hipSuccess) { rc = fail(IGAP4H_EHIP, "hipEventRecord failed"); break; }
         for (int r = 0; r < reps && !rc; r++) rc = batch_tangent(b, cand[k], b->cap);

@@ -328,8 +328,12 @@ int igap4h_batch_sizes(const igap4h_batch *batch, int64_t *nelem, int *variant);
 /* Autotuner: times each tangent variant of the tuning space on this batch,
  * reps launches each, and keeps the fastest.  variant = 10 x form + Gauss
  * points per MFMA chunk, form 0 = T strip, 1 = strain variation (see
- * igap4h_set_tangent_form): FP64 {1, 2, 3, 11}, FP32 {2, 3, 4, 12, 14}.
- * ms[8] / variants[8] (may be NULL) get the table, variant 0 past its end.
+ * igap4h_set_tangent_form), 2 = strain variation with its per-point data
+ * sized for 3^3 Gauss points, 4 work-groups per CU (batches whose instances
+ * all have nquad <= 3): FP64 {1, 2, 3, 11}, FP32 {2, 3, 4, 12, 14, 22}.
+ * Forms 1 and 2 compute identical values.
+ * ms[8] / variants[8] (may be NULL) get the table, variant 0 where a variant
+ * does not apply and past the end.
  * A new batch starts with the variant of igap4h_tangent_dev's default form
  * (FP32 12, FP64 2), so that its results equal the single-instance ones. */
 #define IGAP4H_BATCH_MAX_VARIANTS 8

@@ -62,7 +62,30 @@ def test_batch_equals_single_instance(prec, vt):
         for v, r in zip(B.vals, ref):
             x = v.cpu().numpy().astype(np.float64)
             assert np.linalg.norm(x - r) / np.linalg.norm(r) < tol, (qc, np.linalg.norm(x - r) / np.linalg.norm(r))
+    if prec == L.IGAP4H_FP32:
+        # form 2 is form 1 with a smaller LDS footprint: the same values bit for bit
+        assert 22 in table
+        out = {}
+        for qc in (12, 22):
+            B.set_variant(qc)
+            B.assemble()
+            torch.cuda.synchronize()
+            out[qc] = [v.cpu().numpy().copy() for v in B.vals]
+        assert all(np.array_equal(a, b) for a, b in zip(out[12], out[22]))
     B.set_variant(default_qc)
+    B.close()
+
+
+def test_batch_compact_variant_needs_nq3():
+    """Variant 22 sizes its point data for 3^3 Gauss points: a batch with an
+    nq = 4 instance leaves it out of the tuning space and refuses it."""
+    from igap4standard_amd.batch import Batch
+    pbs = [P.stretch_problem(2, a=0.005, nquad=3), P.stretch_problem(3, a=0.005, nquad=4)]
+    B = Batch(pbs, L.IGAP4H_FP32, L.IGAP4H_VALS_F32)
+    best, table = B.tune(1)
+    assert 22 not in table and 12 in table
+    with pytest.raises(L.IgapError):
+        B.set_variant(22)
     B.close()
 
 
