@@ -299,17 +299,31 @@ __device__ __forceinline__ void interp_slots_sf2(int nq, const double *sU, const
     }
 }
 
+// the knot span [a_d, b_d] of an element in each direction (per-element
+// constants, loaded with the element's other inputs)
+struct ElemGeo {
+    double a[3], b[3];
+};
+__device__ __forceinline__ ElemGeo elem_geo(const MeshArgs &m, const int e[3])
+{
+    ElemGeo g;
+    for (int d = 0; d < 3; d++) {
+        g.a[d] = m.knot[d][e[d] + 2];
+        g.b[d] = m.knot[d][e[d] + 3];
+    }
+    return g;
+}
+
 // periodic F0 terms of one point (snesiga_Residual.c:125-144) and its weight
-__device__ __forceinline__ void point_affine(const MeshArgs &m, const int e[3], int qx, int qy, int qz, double u[30],
+__device__ __forceinline__ void point_affine(const MeshArgs &m, const ElemGeo &g, int qx, int qy, int qz, double u[30],
                                              double *w)
 {
     double xq[3], hh[3];
     int q[3] = {qx, qy, qz};
     for (int d = 0; d < 3; d++) {
-        const double *k = m.knot[d] + e[d];
         double xi = m.cq[q[d]];
-        xq[d] = (-xi + 1.0) * k[2] + xi * k[3];
-        hh[d] = k[3] - k[2];
+        xq[d] = (-xi + 1.0) * g.a[d] + xi * g.b[d];
+        hh[d] = g.b[d] - g.a[d];
     }
     for (int i = 0; i < 3; i++) {
         u[10 * i] += m.pp[3 * i] * xq[0] + m.pp[3 * i + 1] * xq[1] + m.pp[3 * i + 2] * xq[2];
@@ -317,7 +331,6 @@ __device__ __forceinline__ void point_affine(const MeshArgs &m, const int e[3], 
     }
     *w = m.wq[qx] * m.wq[qy] * m.wq[qz] * hh[0] * hh[1] * hh[2];
 }
-
 __device__ __forceinline__ Coef load_coef(const double *coef, long long e)
 {
     const double *c = coef + 6 * e;
@@ -353,11 +366,12 @@ __device__ __forceinline__ void residual_elem_body(const MeshArgs &m, const doub
         long long nd = node_of(m, ev[0] + A / 9, ev[1] + (A / 3) % 3, ev[2] + A % 3);
         sU[idx] = Uh[h * m.ndof + 3 * nd + i];
     }
+    const Coef c = load_coef(coef, e);
+    const ElemGeo geo = elem_geo(m, ev);
     fill_phi(m, ev, sphi, tid, 64);
     __syncthreads();
     const int nq = m.nq, nqp = nq * nq * nq;
     const bool active = tid < nqp;
-    const Coef c = load_coef(coef, e);
     const int qx = tid / (nq * nq), qy = (tid / nq) % nq, qz = tid % nq, qi = qx * 16 + qy * 4 + qz;
     double u[NH][30];
     for (int h = 0; h < NH; h++) {
@@ -368,7 +382,7 @@ __device__ __forceinline__ void residual_elem_body(const MeshArgs &m, const doub
     for (int s = 0; s < 30; s++) r[s] = 0.0;
     double w = 0.0;
     if (active) {
-        for (int h = 0; h < NH; h++) point_affine(m, ev, qx, qy, qz, u[h], &w);
+        for (int h = 0; h < NH; h++) point_affine(m, geo, qx, qy, qz, u[h], &w);
         if (PK == 0) {
             residual_grad<double>(u[0], c, r);
         } else {
@@ -738,9 +752,11 @@ __device__ __forceinline__ void tangent_ge_body(const MeshArgs &m, const double 
         int qp = tid - 128;
         sQ[qp] = qp < nqp ? ((qp / (nq * nq)) | (((qp / nq) % nq) << 4) | ((qp % nq) << 8)) : 0;
     }
+    // per-element constants issued with the other loads: one memory round trip before the barrier
+    const Coef c = load_coef(coef, e);
+    const ElemGeo geo = elem_geo(m, ev);
     fill_phi(m, ev, sphi, tid, 256);
     __syncthreads();
-    const Coef c = load_coef(coef, e);
 
     double *sScr = reinterpret_cast<double *>(sRegion + CMP_BYTES);
     interp_slots_sf<256>(nq, sU, sphi, sScr, tid);
@@ -750,7 +766,7 @@ __device__ __forceinline__ void tangent_ge_body(const MeshArgs &m, const double 
         double u[30], w;
 #pragma unroll
         for (int sl = 0; sl < 30; sl++) u[sl] = sScr[324 + 864 + sl * 64 + qi];
-        point_affine(m, ev, qx, qy, qz, u, &w);
+        point_affine(m, geo, qx, qy, qz, u, &w);
         Compact<double> cp;
         compact_of<double>(u, c, cp);
         store_compact_aos<ACC>(sCmp, tid, cp);
@@ -985,9 +1001,11 @@ __device__ __forceinline__ void tangent_gt_body(const MeshArgs &m, const double 
         int qp = tid - 128;
         sQ[qp] = qp < nqp ? ((qp / (nq * nq)) | (((qp / nq) % nq) << 4) | ((qp % nq) << 8)) : 0;
     }
+    // per-element constants issued with the other loads: one memory round trip before the barrier
+    const Coef c = load_coef(coef, e);
+    const ElemGeo geo = elem_geo(m, ev);
     fill_phi(m, ev, sphi, tid, 256);
     __syncthreads();
-    const Coef c = load_coef(coef, e);
 
     double *sScr = reinterpret_cast<double *>(sRegion + CMP_BYTES);
     double ub[30], ud[30];
@@ -1007,7 +1025,7 @@ __device__ __forceinline__ void tangent_gt_body(const MeshArgs &m, const double 
 #pragma unroll
         for (int sl = 0; sl < 30; sl++) ud[sl] = sScr[324 + 864 + sl * 64 + qi];
         double w;
-        point_affine(m, ev, qx, qy, qz, ub, &w);
+        point_affine(m, geo, qx, qy, qz, ub, &w);
         Jet<3> uj[30];
 #pragma unroll
         for (int sl = 0; sl < 30; sl++) uj[sl] = Num<Jet<3>>::lin(ub[sl], ud[sl]);
@@ -1173,13 +1191,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GtWaves<ACC
 // 16-row tiles exactly, and the 15 tiles (rt, ct), ct >= rt, are mirrored
 // into the lower triangle.  The last dof (A = 26, i = 2) is taken by the
 // vector units: its column K[., 80] = sum_q Y(.)^T Z(80) is a dot product
-// the lane that builds Y(.) forms with Z(80) of the point, built one chunk
-// ahead by spare lanes of wave 3.  The Gamma operands n(A) and w Gamma n(B)
-// sit in columns 96 + A / 96 + B of the same strips (12 slot rows, the rest
-// zero), and its 3 tiles of the 27 x 27 block are added to the three
-// diagonal blocks.  5 tile slots per wave, one code path for every wave (3 A
-// and 5 B fragments per k-step; wave 3 has two empty slots).
-constexpr int SV_RS = 144;  // k-row stride of the operand strips: 128 + 16, the 4 k-rows of a fragment on distinct banks
+// the lane that builds Y(.) forms with Z(80) of the point, built for every
+// Gauss point in one pass after the point data.  The Gamma operands n(A) and
+// w Gamma n(B) sit in columns 80 + A / 80 + B of the same strips (12 slot
+// rows, the rest zero), and its 3 tiles of the 27 x 27 block are added to the
+// three diagonal blocks.  5 tile slots per wave, one code path for every wave
+// (3 A and 5 B fragments per k-step; wave 3 has two empty slots).
+constexpr int SV_G = 80;    // first Gamma column of the operand strips
+constexpr int SV_RS = 112;  // k-row stride of the strips: 80 + 32; 112 = 48 mod 64 puts the 4 k-rows of a fragment on distinct banks
 
 template <class ACC, int GT>
 struct SvPoint;
@@ -1314,8 +1333,8 @@ __device__ __forceinline__ void sv_item_gamma(const SvPoint<ACC, GT> &p, ACC wma
 
 // The wave's 5 tile slots: A fragments from 3 row offsets (slots 0-2 use the
 // first, 3 the second, 4 the third), B fragments from 5 column offsets;
-// offsets are strip columns (96, 112: the Gamma block; 128: an empty slot,
-// computed and discarded).
+// offsets are strip columns (80, 96: the Gamma block); the empty slots read
+// column 0 and their results are discarded.
 //   wave 0: (0,0) (0,1) (0,2) (0,3) (0,4)
 //   wave 1: (1,1) (1,2) (1,3) (1,4) (4,4)
 //   wave 2: (2,2) (2,3) (2,4) (3,3) (3,4)
@@ -1323,15 +1342,15 @@ __device__ __forceinline__ void sv_item_gamma(const SvPoint<ACC, GT> &p, ACC wma
 // (Fragments repeated within a wave are read again: skipping the repeats
 // with wave-uniform branches split the k-step loop and cost 15 %.)
 struct SvTiles {
-    int ao[3], bo[5];
+    int ao[3], bo[5], empty;  // empty: bit t set for an empty slot t
 };
 __device__ __forceinline__ SvTiles sv_tiles(int wave)
 {
     SvTiles t;
-    if (wave == 0) t = SvTiles{{0, 0, 0}, {0, 16, 32, 48, 64}};
-    else if (wave == 1) t = SvTiles{{16, 16, 64}, {16, 32, 48, 64, 64}};
-    else if (wave == 2) t = SvTiles{{32, 48, 48}, {32, 48, 64, 48, 64}};
-    else t = SvTiles{{96, 112, 112}, {96, 112, 128, 112, 128}};
+    if (wave == 0) t = SvTiles{{0, 0, 0}, {0, 16, 32, 48, 64}, 0};
+    else if (wave == 1) t = SvTiles{{16, 16, 64}, {16, 32, 48, 64, 64}, 0};
+    else if (wave == 2) t = SvTiles{{32, 48, 48}, {32, 48, 64, 48, 64}, 0};
+    else t = SvTiles{{SV_G, SV_G + 16, SV_G + 16}, {SV_G, SV_G + 16, 0, SV_G + 16, 0}, (1 << 2) | (1 << 4)};
     return t;
 }
 __device__ __forceinline__ int sv_arow(const SvTiles &T, int t) { return T.ao[t < 3 ? 0 : (t < 4 ? 1 : 2)]; }
@@ -1370,13 +1389,13 @@ __device__ __forceinline__ void sv_stage(ACC *sKe, int lane, const SvTiles &T, c
 #pragma unroll
     for (int t = 0; t < 5; t++) {
         const int ar = sv_arow(T, t), bc = T.bo[t];
-        if (bc >= 128 || (ar >= 96) != GAMMA) continue;
+        if (((T.empty >> t) & 1) || (ar >= SV_G) != GAMMA) continue;
         const int C = bc + (lane & 15);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int R = ar + acc_row((ACC)0, lane, r);
             if (GAMMA) {
-                const int A = R - 96, B = C - 96;
+                const int A = R - SV_G, B = C - SV_G;
                 if (A < 27 && B < 27) {
 #pragma unroll
                     for (int d = 0; d < 3; d++) {
@@ -1426,7 +1445,7 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
     // at the start
     __shared__ __attribute__((aligned(16))) unsigned char sRegion[D::REGION];
     __shared__ ACC sphiA[3 * 4 * 9];
-    __shared__ ACC sZ80[2][QC][NY];  // Z of dof 80 per point of this chunk and the next
+    __shared__ ACC sZ80[NQM * NQM * NQM][NY];  // Z of dof 80 per Gauss point
     __shared__ int sQ[64];
     __shared__ long long sRowBase[81];
     __shared__ int sRowSx[81], sRowSy[81];
@@ -1464,9 +1483,11 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
         int qp = tid - 128;
         sQ[qp] = qp < nqp ? ((qp / (nq * nq)) | (((qp / nq) % nq) << 4) | ((qp % nq) << 8)) : 0;
     }
+    // per-element constants issued with the other loads: one memory round trip before the barrier
+    const Coef c = load_coef(coef, e);
+    const ElemGeo geo = elem_geo(m, ev);
     fill_phi(m, ev, sphi, tid, 256);
     __syncthreads();
-    const Coef c = load_coef(coef, e);
 
     // ---- per-point data
     double *sScr = reinterpret_cast<double *>(sRegion + D::PTB);
@@ -1487,7 +1508,7 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
     }
     if (tid < nqp) {
         double w;
-        point_affine(m, ev, qx, qy, qz, ub, &w);
+        point_affine(m, geo, qx, qy, qz, ub, &w);
         Pt &P = sPt[tid];
         if constexpr (GT) {
             Jet<3> uj[30];
@@ -1540,19 +1561,19 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
     if (NY > 12) {
         for (int idx = tid; idx < QC * (NY - 12) * 32; idx += 256) {
             const int r = idx / 32, ql = r / (NY - 12), k = 12 + r % (NY - 12);
-            sY[(ql * NY + k) * SV_RS + 96 + idx % 32] = (ACC)0;
-            sZ[(ql * NY + k) * SV_RS + 96 + idx % 32] = (ACC)0;
+            sY[(ql * NY + k) * SV_RS + SV_G + idx % 32] = (ACC)0;
+            sZ[(ql * NY + k) * SV_RS + SV_G + idx % 32] = (ACC)0;
         }
     }
     const ACC ce = (ACC)(2.0 * c.e345), cl = (ACC)(2.0 * c.lsq), mass = (ACC)mass_d;
-    // Z(80) of the first chunk's points
-    if (tid < min(QC, nqp)) {
+    // Z(80) of every Gauss point, one lane per point
+    if (tid < nqp) {
         const int q = sQ[tid];
         ACC n[10], Y[NY], Z[NY];
         basis_slots<ACC>(sphiA, q & 15, (q >> 4) & 15, (q >> 8) & 15, 26, n);
         sv_yz<ACC>(sPt[tid], ce, cl, n, 2, Y, Z);
 #pragma unroll
-        for (int k = 0; k < NY; k++) sZ80[0][tid][k] = Z[k];
+        for (int k = 0; k < NY; k++) sZ80[tid][k] = Z[k];
     }
     __syncthreads();
 
@@ -1582,7 +1603,7 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
     for (int ch = 0; ch < nchunk; ch++) {
         const int qcnt = min(QC, nqp - ch * QC);
         // operands: wave j < 3 builds Y_j, Z_j (lane = (point, B)) and the
-        // dof-80 products; wave 3 the Gamma pair, and Z(80) of the next chunk
+        // dof-80 products; wave 3 the Gamma pair
         if (wave < 3) {
 #pragma unroll
             for (int sl = 0; sl < NSLOT; sl++) {
@@ -1591,28 +1612,20 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
                     const int ql = t / 27, B = t - 27 * ql, q = sQ[ch * QC + ql];
                     ACC n[10];
                     basis_slots<ACC>(sphiA, q & 15, (q >> 4) & 15, (q >> 8) & 15, B, n);
-                    const int col = 27 * wave + B;
+                    // the last dof's column is the first Gamma column: its Y and Z go to
+                    // the last (padding) column, whose products are discarded
+                    const int col = wave == 2 && B == 26 ? SV_RS - 1 : 27 * wave + B;
                     acc80[sl] += (double)sv_item_yz<ACC, GT>(sPt[ch * QC + ql], ce, cl, n, wave, sY + ql * NY * SV_RS + col,
-                                                             sZ + ql * NY * SV_RS + col, sZ80[ch & 1][ql]);
+                                                             sZ + ql * NY * SV_RS + col, sZ80[ch * QC + ql]);
                 }
             }
         } else {
-            const int qnext = ch + 1 < nchunk ? min(QC, nqp - (ch + 1) * QC) : 0;
-            for (int t = lane; t < 27 * qcnt + qnext; t += 64) {
-                if (t < 27 * qcnt) {
-                    const int ql = t / 27, B = t - 27 * ql, qp = ch * QC + ql, q = sQ[qp];
-                    ACC n[10];
-                    basis_slots<ACC>(sphiA, q & 15, (q >> 4) & 15, (q >> 8) & 15, B, n);
-                    const Pt &P = sPt[qp];
-                    sv_item_gamma<ACC, GT>(P, P.w * mass, n, sY + ql * NY * SV_RS + 96 + B, sZ + ql * NY * SV_RS + 96 + B);
-                } else {
-                    const int ql = t - 27 * qcnt, qp = (ch + 1) * QC + ql, q = sQ[qp];
-                    ACC n[10], Y[NY], Z[NY];
-                    basis_slots<ACC>(sphiA, q & 15, (q >> 4) & 15, (q >> 8) & 15, 26, n);
-                    sv_yz<ACC>(sPt[qp], ce, cl, n, 2, Y, Z);
-#pragma unroll
-                    for (int k = 0; k < NY; k++) sZ80[(ch + 1) & 1][ql][k] = Z[k];
-                }
+            for (int t = lane; t < 27 * qcnt; t += 64) {
+                const int ql = t / 27, B = t - 27 * ql, qp = ch * QC + ql, q = sQ[qp];
+                ACC n[10];
+                basis_slots<ACC>(sphiA, q & 15, (q >> 4) & 15, (q >> 8) & 15, B, n);
+                const Pt &P = sPt[qp];
+                sv_item_gamma<ACC, GT>(P, P.w * mass, n, sY + ql * NY * SV_RS + SV_G + B, sZ + ql * NY * SV_RS + SV_G + B);
             }
         }
         __syncthreads();
