@@ -1218,34 +1218,41 @@ struct SvPoint<ACC, 1> {
     ACC w;
 };
 
-// the 12 strain-variation components of dof j (row F_j., H_j.) along slots n
+// the 12 strain-variation components of dof j (row F_j., H_j.) along slots
+// n, each without its constant factor s_k (1/2 for the shear strains, 1/sqrt2
+// and 1/sqrt6 for the trace-free basis): K = sum_k Y_k (m_k Y_k) needs s_k^2
+// only, which the Z coefficients carry (SV_SH, SV_T2, SV_T6)
 template <class R>
 __device__ __forceinline__ void sv_y(const R F[3], const R H[6], const R n[10], R *y)
 {
-    const R half = R(0.5), s2 = R(IGAP4H_S2), s6 = R(IGAP4H_S6);
     y[0] = F[0] * n[1];
     y[1] = F[1] * n[2];
     y[2] = F[2] * n[3];
-    y[3] = (F[1] * n[1] + F[0] * n[2]) * half;
-    y[4] = (F[2] * n[1] + F[0] * n[3]) * half;
-    y[5] = (F[2] * n[2] + F[1] * n[3]) * half;
+    y[3] = F[1] * n[1] + F[0] * n[2];
+    y[4] = F[2] * n[1] + F[0] * n[3];
+    y[5] = F[2] * n[2] + F[1] * n[3];
 #pragma unroll
     for (int K = 0; K < 3; K++) {
         R nu[3];
 #pragma unroll
         for (int I = 0; I < 3; I++) nu[I] = H[hidx(I, K)] * n[1 + I] + F[I] * n[4 + hidx(I, K)];
-        y[6 + 2 * K] = (nu[0] - nu[1]) * s2;
-        y[7 + 2 * K] = (nu[0] + nu[1] - R(2) * nu[2]) * s6;
+        y[6 + 2 * K] = nu[0] - nu[1];
+        y[7 + 2 * K] = nu[0] + nu[1] - R(2) * nu[2];
     }
 }
+constexpr double SV_SH = 0.25, SV_T2 = 0.5, SV_T6 = 1.0 / 6.0;  // s_k^2 of the scaled components
+// s_k^2 of component k >= 3
+__host__ __device__ constexpr double sv_s2(int k) { return k < 6 ? SV_SH : ((k & 1) ? SV_T6 : SV_T2); }
 
-__device__ __forceinline__ void sym3_apply_add(const float *P, const float *y, float *z)
+template <class R>
+__device__ __forceinline__ void sym3_apply(const R *P, const R *y, R *z)
 {
-    z[0] += P[0] * y[0] + P[1] * y[1] + P[2] * y[2];
-    z[1] += P[1] * y[0] + P[3] * y[1] + P[4] * y[2];
-    z[2] += P[2] * y[0] + P[4] * y[1] + P[5] * y[2];
+    z[0] = P[0] * y[0] + P[1] * y[1] + P[2] * y[2];
+    z[1] = P[1] * y[0] + P[3] * y[1] + P[4] * y[2];
+    z[2] = P[2] * y[0] + P[4] * y[1] + P[5] * y[2];
 }
-__device__ __forceinline__ void sym3_apply_add(const double *P, const double *y, double *z)
+template <class R>
+__device__ __forceinline__ void sym3_apply_add(const R *P, const R *y, R *z)
 {
     z[0] += P[0] * y[0] + P[1] * y[1] + P[2] * y[2];
     z[1] += P[1] * y[0] + P[3] * y[1] + P[4] * y[2];
@@ -1257,30 +1264,26 @@ template <class ACC>
 __device__ __forceinline__ void sv_yz(const SvPoint<ACC, 0> &p, ACC ce, ACC cl, const ACC n[10], int j, ACC Y[12], ACC Z[12])
 {
     sv_y<ACC>(p.F[j], p.H[j], n, Y);
-    const ACC we = p.w * ce, wl = p.w * cl;
-    Z[0] = Z[1] = Z[2] = ACC(0);
-    sym3_apply_add(p.P, Y, Z);
+    const ACC we = p.w * ce * ACC(SV_SH), wl2 = p.w * cl * ACC(SV_T2), wl6 = p.w * cl * ACC(SV_T6);
+    sym3_apply(p.P, Y, Z);
 #pragma unroll
     for (int k = 3; k < 6; k++) Z[k] = we * Y[k];
 #pragma unroll
-    for (int k = 6; k < 12; k++) Z[k] = wl * Y[k];
+    for (int k = 6; k < 12; k++) Z[k] = ((k & 1) ? wl6 : wl2) * Y[k];
 }
 template <class ACC>
 __device__ __forceinline__ void sv_yz(const SvPoint<ACC, 1> &p, ACC ce, ACC cl, const ACC n[10], int j, ACC Y[24], ACC Z[24])
 {
     sv_y<ACC>(p.F[j], p.H[j], n, Y);
     sv_y<ACC>(p.F1[j], p.H1[j], n, Y + 12);
-#pragma unroll
-    for (int k = 0; k < 3; k++) Z[k] = Z[12 + k] = ACC(0);
-    sym3_apply_add(p.P00, Y, Z);
+    sym3_apply(p.P00, Y, Z);
     sym3_apply_add(p.P01, Y + 12, Z);
-    sym3_apply_add(p.P01, Y, Z + 12);
+    sym3_apply(p.P01, Y, Z + 12);
     sym3_apply_add(p.P11, Y + 12, Z + 12);
-    const ACC q4 = ACC(0.25), q6 = ACC(1.0 / 6.0), q8 = ACC(0.125);
 #pragma unroll
     for (int k = 3; k < 12; k++) {
-        const ACC cw = p.w * (k < 6 ? ce : cl);
-        const ACC a00 = cw * q4, a01 = cw * q6, a11 = cw * q8;
+        const ACC cw = p.w * (k < 6 ? ce : cl) * ACC(sv_s2(k));
+        const ACC a00 = cw * ACC(0.25), a01 = cw * ACC(1.0 / 6.0), a11 = cw * ACC(0.125);
         Z[k] = a00 * Y[k] + a01 * Y[12 + k];
         Z[12 + k] = a01 * Y[k] + a11 * Y[12 + k];
     }
