@@ -1194,9 +1194,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GtWaves<ACC
 // the lane that builds Y(.) forms with Z(80) of the point, built for every
 // Gauss point in one pass after the point data.  The Gamma operands n(A) and
 // w Gamma n(B) sit in columns 80 + A / 80 + B of the same strips (12 slot
-// rows, the rest zero), and its 3 tiles of the 27 x 27 block are added to the
-// three diagonal blocks.  5 tile slots per wave, one code path for every wave
-// (3 A and 5 B fragments per k-step; wave 3 has two empty slots).
+// rows, the rest of those rows unused), and its 3 tiles of the 27 x 27 block
+// are added to the three diagonal blocks.  5 tile slots per wave, one code
+// path for every wave.
 constexpr int SV_G = 80;    // first Gamma column of the operand strips
 constexpr int SV_RS = 112;  // k-row stride of the strips: 80 + 32; 112 = 48 mod 64 puts the 4 k-rows of a fragment on distinct banks
 
@@ -1334,32 +1334,49 @@ __device__ __forceinline__ void sv_item_gamma(const SvPoint<ACC, GT> &p, ACC wma
     }
 }
 
-// The wave's 5 tile slots: A fragments from 3 row offsets (slots 0-2 use the
-// first, 3 the second, 4 the third), B fragments from 5 column offsets;
-// offsets are strip columns (80, 96: the Gamma block); the empty slots read
-// column 0 and their results are discarded.
+// The wave's 5 tile slots; slot t takes its A fragment from ao[ag(t)] and its
+// B fragment from bo[t] (strip columns; 80, 96: the Gamma block).
+// phys/ge (GT = 0): every slot over the 12 k-rows, ag = (0, 0, 0, 1, 2)
 //   wave 0: (0,0) (0,1) (0,2) (0,3) (0,4)
 //   wave 1: (1,1) (1,2) (1,3) (1,4) (4,4)
 //   wave 2: (2,2) (2,3) (2,4) (3,3) (3,4)
 //   wave 3: (G0,G0) (G0,G1) -  (G1,G1) -
-// (Fragments repeated within a wave are read again: skipping the repeats
-// with wave-uniform branches split the k-step loop and cost 15 %.)
+// phys/getime (GT = 1): 4 material slots over the 24 k-rows and one Gamma slot
+// over the 12 Gamma rows, ag = (0, 1, 1, 1, 2): a wave issues 4 x 6 + 3 = 27
+// MFMAs per point instead of 30
+//   wave 0: (0,0) (0,1) (0,2) (0,3) | (G0,G0)
+//   wave 1: (0,4) (1,1) (1,2) (1,3) | (G0,G1)
+//   wave 2: (1,4) (2,2) (2,3) (2,4) | (G1,G1)
+//   wave 3: (4,4) (3,3) (3,4)   -   |    -
+// (for ge the second assignment measured 1.3 % slower.)  Empty slots read
+// column 0 and are discarded.  (Fragments repeated within a wave are read
+// again: skipping the repeats with wave-uniform branches split the k-step
+// loop and cost 15 %.)
 struct SvTiles {
-    int ao[3], bo[5], empty;  // empty: bit t set for an empty slot t
+    int ao[3], bo[5];
+    unsigned empty, gam;  // bit t: slot t is empty / a Gamma tile
 };
+template <int GT>
+__device__ __forceinline__ constexpr int sv_ag(int t) { return GT ? (t == 0 ? 0 : (t < 4 ? 1 : 2)) : (t < 3 ? 0 : (t < 4 ? 1 : 2)); }
+template <int GT>
 __device__ __forceinline__ SvTiles sv_tiles(int wave)
 {
-    SvTiles t;
-    if (wave == 0) t = SvTiles{{0, 0, 0}, {0, 16, 32, 48, 64}, 0};
-    else if (wave == 1) t = SvTiles{{16, 16, 64}, {16, 32, 48, 64, 64}, 0};
-    else if (wave == 2) t = SvTiles{{32, 48, 48}, {32, 48, 64, 48, 64}, 0};
-    else t = SvTiles{{SV_G, SV_G + 16, SV_G + 16}, {SV_G, SV_G + 16, 0, SV_G + 16, 0}, (1 << 2) | (1 << 4)};
-    return t;
+    if (GT == 0) {
+        if (wave == 0) return SvTiles{{0, 0, 0}, {0, 16, 32, 48, 64}, 0u, 0u};
+        if (wave == 1) return SvTiles{{16, 16, 64}, {16, 32, 48, 64, 64}, 0u, 0u};
+        if (wave == 2) return SvTiles{{32, 48, 48}, {32, 48, 64, 48, 64}, 0u, 0u};
+        return SvTiles{{SV_G, SV_G + 16, SV_G + 16}, {SV_G, SV_G + 16, 0, SV_G + 16, 0}, (1u << 2) | (1u << 4),
+                       (1u << 0) | (1u << 1) | (1u << 3)};
+    }
+    if (wave == 0) return SvTiles{{0, 0, SV_G}, {0, 16, 32, 48, SV_G}, 0u, 1u << 4};
+    if (wave == 1) return SvTiles{{0, 16, SV_G}, {64, 16, 32, 48, SV_G + 16}, 0u, 1u << 4};
+    if (wave == 2) return SvTiles{{16, 32, SV_G + 16}, {64, 32, 48, 64, SV_G + 16}, 0u, 1u << 4};
+    return SvTiles{{64, 48, 0}, {64, 48, 64, 0, 0}, (1u << 3) | (1u << 4), 0u};
 }
-__device__ __forceinline__ int sv_arow(const SvTiles &T, int t) { return T.ao[t < 3 ? 0 : (t < 4 ? 1 : 2)]; }
 
-// the chunk's qcnt points, the NY k-rows of each (y: A strip, z: B strip)
-template <class ACC, int NY>
+// the chunk's qcnt points, the NY k-rows of each (y: A strip, z: B strip);
+// GT: slot 4 (Gamma) over the first 12 k-rows only
+template <class ACC, int NY, int GT>
 __device__ __forceinline__ void sv_mfma_chunk(const ACC *y, const ACC *z, int qcnt, int lane, const SvTiles &T,
                                               typename V4<ACC>::t acc[5])
 {
@@ -1369,15 +1386,17 @@ __device__ __forceinline__ void sv_mfma_chunk(const ACC *y, const ACC *z, int qc
 #pragma unroll
         for (int ks = 0; ks < NY / 4; ks++) {
             const ACC *yk = ya + 4 * ks * SV_RS, *zk = zb + 4 * ks * SV_RS;
-            const ACC a0 = yk[T.ao[0]], a1 = yk[T.ao[1]], a2 = yk[T.ao[2]];
-            ACC b[5];
+            const bool g4 = !GT || ks < 3;
+            ACC a[3], b[5];
+            a[0] = yk[T.ao[0]];
+            a[1] = yk[T.ao[1]];
+            if (g4) a[2] = yk[T.ao[2]];
 #pragma unroll
-            for (int t = 0; t < 5; t++) b[t] = zk[T.bo[t]];
-            acc[0] = mfma16(a0, b[0], acc[0]);
-            acc[1] = mfma16(a0, b[1], acc[1]);
-            acc[2] = mfma16(a0, b[2], acc[2]);
-            acc[3] = mfma16(a1, b[3], acc[3]);
-            acc[4] = mfma16(a2, b[4], acc[4]);
+            for (int t = 0; t < 4; t++) b[t] = zk[T.bo[t]];
+            if (g4) b[4] = zk[T.bo[4]];
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc[t] = mfma16(a[sv_ag<GT>(t)], b[t], acc[t]);
+            if (g4) acc[4] = mfma16(a[2], b[4], acc[4]);
         }
         ya += NY * SV_RS;
         zb += NY * SV_RS;
@@ -1385,14 +1404,15 @@ __device__ __forceinline__ void sv_mfma_chunk(const ACC *y, const ACC *z, int qc
 }
 
 // the wave's material tiles into the staged 81 x 81 element matrix (both
-// triangles); GAMMA: its Gamma tiles added to the three diagonal blocks
-template <class ACC, bool GAMMA>
+// triangles); GAMMA: its Gamma tiles added to the three diagonal blocks (a
+// Gamma tile touches entries no other one does)
+template <class ACC, bool GAMMA, int GT>
 __device__ __forceinline__ void sv_stage(ACC *sKe, int lane, const SvTiles &T, const typename V4<ACC>::t acc[5])
 {
 #pragma unroll
     for (int t = 0; t < 5; t++) {
-        const int ar = sv_arow(T, t), bc = T.bo[t];
-        if (((T.empty >> t) & 1) || (ar >= SV_G) != GAMMA) continue;
+        if (((T.empty >> t) & 1) || (((T.gam >> t) & 1) != 0) != GAMMA) continue;
+        const int ar = T.ao[sv_ag<GT>(t)], bc = T.bo[t];
         const int C = bc + (lane & 15);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
@@ -1560,14 +1580,6 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
         sphiA[tid - 128] = (ACC)sphi[tid - 128];
     }
     __syncthreads();
-    // the Gamma block's k-rows past 12 (getime) stay zero from here on
-    if (NY > 12) {
-        for (int idx = tid; idx < QC * (NY - 12) * 32; idx += 256) {
-            const int r = idx / 32, ql = r / (NY - 12), k = 12 + r % (NY - 12);
-            sY[(ql * NY + k) * SV_RS + SV_G + idx % 32] = (ACC)0;
-            sZ[(ql * NY + k) * SV_RS + SV_G + idx % 32] = (ACC)0;
-        }
-    }
     const ACC ce = (ACC)(2.0 * c.e345), cl = (ACC)(2.0 * c.lsq), mass = (ACC)mass_d;
     // Z(80) of every Gauss point, one lane per point
     if (tid < nqp) {
@@ -1590,7 +1602,7 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
     for (int t = 0; t < 5; t++)
 #pragma unroll
         for (int r = 0; r < 4; r++) accd[t][r] = 0.0;
-    const SvTiles T = sv_tiles(wave);
+    const SvTiles T = sv_tiles<GT>(wave);
     constexpr int NSLOT = (27 * QC + 63) / 64;  // items per lane (waves 0-2)
     double acc80[NSLOT];
 #pragma unroll
@@ -1632,7 +1644,7 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
             }
         }
         __syncthreads();
-        sv_mfma_chunk<ACC, NY>(sY, sZ, qcnt, lane, T, acc);
+        sv_mfma_chunk<ACC, NY, GT>(sY, sZ, qcnt, lane, T, acc);
         // FP32: the partial sums of every 16 Gauss points go into FP64
         if (sizeof(ACC) == 4 && (((ch + 1) * QC) % 16 == 0 || ch == nchunk - 1)) {
 #pragma unroll
@@ -1666,7 +1678,7 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
                 if (t < 27 * QC) s80[wave * 27 * QC + t] = acc80[sl];
             }
         }
-        sv_stage<ACC, false>(sKe, lane, T, acc);
+        sv_stage<ACC, false, GT>(sKe, lane, T, acc);
         __syncthreads();
         if (tid < 81) {
             const int j = tid / 27, B = tid - 27 * j;
@@ -1676,7 +1688,7 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
             sKe[80 * 81 + 3 * B + j] = (ACC)k80;
         }
         __syncthreads();
-        if (wave == 3) sv_stage<ACC, true>(sKe, lane, T, acc);
+        sv_stage<ACC, true, GT>(sKe, lane, T, acc);
         __syncthreads();
     };
     if (vals_f32) {
