@@ -57,9 +57,10 @@ FLOP_RE_PER_QP = 1620 + 1620 + 540 + 291
 # weights): ~71 k of the 260,544 algorithmic
 FLOP_RE_EXEC_PER_ELEM = 71000
 PEAK = {"f64": 78.6, "f32": 157.3}   # MI355X dense MFMA TFLOP/s (the vector FP64 / FP32 peaks are the same)
-# MFMA flops the strain-variation tangent executes per Gauss point: 4 waves x
-# 5 tile slots of 16 x 16 x 4 per 4 strain components (ge 12, getime 24 per dof)
-SV_MFMA_FLOP_PER_QP = {"ge": 20 * 3 * 2 * 16 * 16 * 4, "getime": 20 * 6 * 2 * 16 * 16 * 4}
+# MFMA flops the strain-variation tangent executes per Gauss point, 4 waves x
+# 16 x 16 x 4 tiles: ge 5 slots x 3 k-steps (12 strain components per dof);
+# getime 4 material slots x 6 k-steps (24 components) + 1 Gamma slot x 3
+SV_MFMA_FLOP_PER_QP = {"ge": 4 * 5 * 3 * 2 * 16 * 16 * 4, "getime": 4 * (4 * 6 + 3) * 2 * 16 * 16 * 4}
 PEAK_HBM = 8000.0                    # GB/s
 
 
