@@ -435,7 +435,11 @@ def main():
     main_asm = asm.get("fp64") or asm[wanted_asm[0]]
     main_sp = spmv.get("fp64") or spmv[wanted_asm[0]]
     main_nt = newton.get("fp64") or (newton[wanted[0]] if newton else None)
-    tangent_kernel = "k_tangent_ge<double,2>" if "fp64" in asm else "k_tangent_ge<float,4>"
+    # the tangent kernels of the chosen form (auto: the strain variation for phys/ge in both precisions)
+    tstrip = args.tangent_form == "tstrip"
+    k64 = "k_tangent_ge<double,2>" if tstrip else "k_tangent_sv<double,2,0>"
+    k32 = "k_tangent_ge<float,4>" if tstrip else "k_tangent_sv<float,2,0>"
+    tangent_kernel = k64 if "fp64" in asm else k32
     spmv_kernel = "k_spmv_line<double,0,0>" if "fp64" in spmv else "k_spmv_line<float,0,0>"
     if main_nt:
         value, ms_step = main_nt["elements_per_s"], main_nt["ms_per_newton_step"]
@@ -485,13 +489,17 @@ def main():
     }
     if "mixed" in asm:
         ex = pb.n_elem * SV_MFMA_FLOP_PER_QP["ge"] * pb.nquad ** 3 / (asm["mixed"]["tangent_ms"] * 1e-3) / 1e12
+        if tstrip:
+            ex = None
         line["roofline_tangent_f32"] = {"bound": "mfma", "achieved": asm["mixed"]["tangent_tflops"], "peak": PEAK["f32"],
                                         "unit": "TFLOP/s", "frac": asm["mixed"]["tangent_tflops"] / PEAK["f32"],
-                                        "traffic": profiled_traffic("k_tangent_sv<float,2,0>"),
-                                        "kernel": "k_tangent_sv<float,2,0> (strain-variation form, 27 colour "
-                                                  "launches per tangent)",
+                                        "traffic": profiled_traffic(k32),
+                                        "kernel": k32 + " (27 colour launches per tangent)",
                                         "algorithmic": "11,508,480 flop per element",
-                                        "mfma_executed_tflops": ex, "mfma_executed_frac": ex / PEAK["f32"]}
+                                        "mfma_executed_tflops": ex, "mfma_executed_frac": ex / PEAK["f32"] if ex else None}
+    if "fp64" in asm and not tstrip:
+        ex = pb.n_elem * SV_MFMA_FLOP_PER_QP["ge"] * pb.nquad ** 3 / (asm["fp64"]["tangent_ms"] * 1e-3) / 1e12
+        line["roofline_tangent"].update({"mfma_executed_tflops": ex, "mfma_executed_frac": ex / PEAK["f64"]})
     if "mixed" in asm and "fp64" in asm:
         line["assembly"]["mixed"]["roofline_frac_f32"] = asm["mixed"]["tangent_tflops"] / PEAK["f32"]
         line["assembly"]["mixed"]["tangent_speedup_vs_fp64"] = asm["fp64"]["tangent_ms"] / asm["mixed"]["tangent_ms"]

@@ -2473,6 +2473,8 @@ template <>
 struct SvQC<float, 0> { static constexpr int v = 2; };
 template <>
 struct SvQC<float, 1> { static constexpr int v = 2; };
+template <>
+struct SvQC<double, 0> { static constexpr int v = 2; };
 
 
 template <class ACC, int GT>
@@ -2504,11 +2506,11 @@ extern "C" int igap4h_tangent_dev(igap4h_ctx *c, const double *d_Uh, int prec, i
     hipLaunchKernelGGL(k_fill_zero, dim3(2048), dim3(256), 0, st, (unsigned int *)d_vals, n32);
     HIPCHK(hipGetLastError());
     double mass = c->physics == IGAP4H_PHYS_GETIME ? c->par_mat[5] / (c->par_mat[7] * c->par_mat[7]) + c->par_mat[6] / (2. * c->par_mat[7]) : 0.0;
-    // AUTO: the faster form per precision (64^3 ge, 48^3 getime, nq 4, ms per
-    // tangent, T strip / strain variation): ge FP64 63.4 / 74.4, FP32 34.7 /
-    // 32.5; getime FP64 67.4 / 77.0, FP32 32.9 / 27.6
+    // AUTO: the faster form per physics and precision (64^3 ge, 48^3 getime,
+    // nq 4, ms per tangent, T strip / strain variation): ge FP64 62.2 / 55.4,
+    // FP32 34.5 / 29.1; getime FP64 65.5 / 73.1, FP32 33.1 / 23.2
     const bool gt = c->physics == IGAP4H_PHYS_GETIME, f64 = prec == IGAP4H_FP64;
-    const bool sv = c->tangent_form == IGAP4H_TANGENT_SV || (c->tangent_form == IGAP4H_TANGENT_AUTO && !f64);
+    const bool sv = c->tangent_form == IGAP4H_TANGENT_SV || (c->tangent_form == IGAP4H_TANGENT_AUTO && !(gt && f64));
     int rc;
     if (sv) {
         if (gt) rc = f64 ? launch_tangent_sv<double, 1>(c, m, d_Uh, mass, d_vals, f32, st)
@@ -2807,7 +2809,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
 // 3^3 Gauss points and registers for 4 work-groups per CU (instances with
 // nq <= 3 only); 0 ends a list.
 constexpr int kBatchNV = 8;
-static const int kBatchVariants[2][kBatchNV] = {{1, 2, 3, 11, 0, 0, 0, 0}, {2, 3, 4, 12, 14, 22, 0, 0}};  // [FP64, FP32]
+static const int kBatchVariants[2][kBatchNV] = {{1, 2, 3, 11, 12, 0, 0, 0}, {2, 3, 4, 12, 14, 22, 0, 0}};  // [FP64, FP32]
 static bool batch_variant_ok(int prec, int v, int nqmax)
 {
     if (v / 10 == 2 && nqmax > 3) return false;
@@ -2899,6 +2901,7 @@ static int batch_tangent(igap4h_batch *b, int v, hipStream_t st)
         if (v == 2) return batch_tangent_launches<double, 2, 0>(b, st);
         if (v == 3) return batch_tangent_launches<double, 3, 0>(b, st);
         if (v == 11) return batch_tangent_launches<double, 1, 1>(b, st);
+        if (v == 12) return batch_tangent_launches<double, 2, 1>(b, st);
     }
     return fail(IGAP4H_EINVAL, "no batched tangent variant %d for this precision", v);
 }
@@ -2953,8 +2956,8 @@ extern "C" int igap4h_batch_create(igap4h_ctx *const *ctxs, int n, const double 
     b->n = n;
     b->prec = prec;
     b->vals_f32 = vals_type == IGAP4H_VALS_F32;
-    // the single-instance default (IGAP4H_TANGENT_AUTO): FP32 strain variation, FP64 T strip
-    b->qc = prec == IGAP4H_FP32 ? 10 + SvQC<float, 0>::v : GeQC<double>::v;
+    // the single-instance default for phys/ge (IGAP4H_TANGENT_AUTO): the strain variation
+    b->qc = prec == IGAP4H_FP32 ? 10 + SvQC<float, 0>::v : 10 + SvQC<double, 0>::v;
     b->eoff.assign(n + 1, 0);
     b->coff.assign((size_t)27 * (n + 1), 0);
     b->goff.assign(n + 1, 0);

@@ -252,8 +252,9 @@ int igap4h_residual_dev_noalloc(igap4h_ctx *ctx, const double *d_U_hist, double 
  * and contracts it with the slots on MFMA; IGAP4H_TANGENT_SV factors the
  * tangent through the strain variations of the dofs and contracts those on
  * MFMA (less vector work, more MFMA work).  IGAP4H_TANGENT_AUTO (the
- * default) takes the faster one per precision as measured on MI355X: SV
- * for the FP32 tangent, TSTRIP for FP64.                                  */
+ * default) takes the faster one as measured on MI355X: SV for phys/ge in
+ * both precisions and for the FP32 phys/getime tangent, TSTRIP for the FP64
+ * phys/getime tangent.                                                    */
 #define IGAP4H_TANGENT_AUTO 0
 #define IGAP4H_TANGENT_TSTRIP 1
 #define IGAP4H_TANGENT_SV 2
@@ -330,12 +331,13 @@ int igap4h_batch_sizes(const igap4h_batch *batch, int64_t *nelem, int *variant);
  * points per MFMA chunk, form 0 = T strip, 1 = strain variation (see
  * igap4h_set_tangent_form), 2 = strain variation with its per-point data
  * sized for 3^3 Gauss points, 4 work-groups per CU (batches whose instances
- * all have nquad <= 3): FP64 {1, 2, 3, 11}, FP32 {2, 3, 4, 12, 14, 22}.
+ * all have nquad <= 3): FP64 {1, 2, 3, 11, 12}, FP32 {2, 3, 4, 12, 14, 22}.
  * Forms 1 and 2 compute identical values.
  * ms[8] / variants[8] (may be NULL) get the table, variant 0 where a variant
  * does not apply and past the end.
  * A new batch starts with the variant of igap4h_tangent_dev's default form
- * (FP32 12, FP64 2), so that its results equal the single-instance ones. */
+ * (12 in both precisions), so that its results equal the single-instance
+ * ones. */
 #define IGAP4H_BATCH_MAX_VARIANTS 8
 int igap4h_batch_tune(igap4h_batch *batch, int reps, int *best_variant, double *ms, int *variants);
 int igap4h_batch_set_variant(igap4h_batch *batch, int variant);
