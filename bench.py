@@ -238,7 +238,7 @@ def config_legs(say):
                                                                         stream.cuda_stream), 5)
     for prec, vt, dt, nm, peak in ((L.IGAP4H_FP32, L.IGAP4H_VALS_F32, torch.float32, "k_tangent_sv<float,2,1>",
                                     PEAK["f32"]),
-                                   (L.IGAP4H_FP64, L.IGAP4H_VALS_F64, torch.float64, "k_tangent_gt<double,2>",
+                                   (L.IGAP4H_FP64, L.IGAP4H_VALS_F64, torch.float64, "k_tangent_sv<double,2,1>",
                                     PEAK["f64"])):
         V = torch.empty(app.nnz, device="cuda:0", dtype=dt)
         ms = _events(torch, stream, lambda: app.tangent_dev(ts.Uh.data_ptr(), V.data_ptr(), prec, vt,
@@ -248,9 +248,9 @@ def config_legs(say):
             "bound": "mfma", "kernel": nm + " (27 colour launches per tangent)", "tangent_ms": ms,
             "achieved": tf, "peak": peak, "unit": "TFLOP/s", "frac": tf / peak,
             "algorithmic": "11,508,480 flop per element"}
-        if prec == L.IGAP4H_FP32:
-            ex = pb.n_elem * SV_MFMA_FLOP_PER_QP["getime"] * pb.nquad ** 3 / (ms * 1e-3) / 1e12
-            c3["roofline_tangent_f32"].update({"mfma_executed_tflops": ex, "mfma_executed_frac": ex / peak})
+        ex = pb.n_elem * SV_MFMA_FLOP_PER_QP["getime"] * pb.nquad ** 3 / (ms * 1e-3) / 1e12
+        c3["roofline_tangent_" + ("f32" if prec == L.IGAP4H_FP32 else "f64")].update(
+            {"mfma_executed_tflops": ex, "mfma_executed_frac": ex / peak})
         del V
     ts.app.close()
     c3["workload"] = ("48^3 getime (torder 2), all faces clamped, rho 1, lambda 1, mu 0.5, l 0.05, dt 1e-3, v0 a "

@@ -1702,7 +1702,7 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
     }
 }
 
-// two work-groups per CU (one for getime FP64, whose LDS allows one)
+// two work-groups per CU (one for getime FP64: 146 KB of LDS with 2 points per chunk)
 template <class ACC, int GT>
 struct SvWaves { static constexpr int v = 2; };
 template <>
@@ -2475,6 +2475,8 @@ template <>
 struct SvQC<float, 1> { static constexpr int v = 2; };
 template <>
 struct SvQC<double, 0> { static constexpr int v = 2; };
+template <>
+struct SvQC<double, 1> { static constexpr int v = 2; };
 
 
 template <class ACC, int GT>
@@ -2506,11 +2508,12 @@ extern "C" int igap4h_tangent_dev(igap4h_ctx *c, const double *d_Uh, int prec, i
     hipLaunchKernelGGL(k_fill_zero, dim3(2048), dim3(256), 0, st, (unsigned int *)d_vals, n32);
     HIPCHK(hipGetLastError());
     double mass = c->physics == IGAP4H_PHYS_GETIME ? c->par_mat[5] / (c->par_mat[7] * c->par_mat[7]) + c->par_mat[6] / (2. * c->par_mat[7]) : 0.0;
-    // AUTO: the faster form per physics and precision (64^3 ge, 48^3 getime,
-    // nq 4, ms per tangent, T strip / strain variation): ge FP64 62.2 / 55.4,
-    // FP32 34.5 / 29.1; getime FP64 65.5 / 73.1, FP32 33.1 / 23.2
+    // AUTO: the faster form, the strain variation for both physics and
+    // precisions (64^3 ge, 48^3 getime, nq 4, ms per tangent, T strip / strain
+    // variation): ge FP64 62.2 / 55.4, FP32 34.5 / 29.1; getime FP64 65.5 / 55.3,
+    // FP32 33.1 / 23.2
     const bool gt = c->physics == IGAP4H_PHYS_GETIME, f64 = prec == IGAP4H_FP64;
-    const bool sv = c->tangent_form == IGAP4H_TANGENT_SV || (c->tangent_form == IGAP4H_TANGENT_AUTO && !(gt && f64));
+    const bool sv = c->tangent_form != IGAP4H_TANGENT_TSTRIP;
     int rc;
     if (sv) {
         if (gt) rc = f64 ? launch_tangent_sv<double, 1>(c, m, d_Uh, mass, d_vals, f32, st)

@@ -252,9 +252,8 @@ int igap4h_residual_dev_noalloc(igap4h_ctx *ctx, const double *d_U_hist, double 
  * and contracts it with the slots on MFMA; IGAP4H_TANGENT_SV factors the
  * tangent through the strain variations of the dofs and contracts those on
  * MFMA (less vector work, more MFMA work).  IGAP4H_TANGENT_AUTO (the
- * default) takes the faster one as measured on MI355X: SV for phys/ge in
- * both precisions and for the FP32 phys/getime tangent, TSTRIP for the FP64
- * phys/getime tangent.                                                    */
+ * default) takes the faster one as measured on MI355X: SV for both physics
+ * and precisions.                                                         */
 #define IGAP4H_TANGENT_AUTO 0
 #define IGAP4H_TANGENT_TSTRIP 1
 #define IGAP4H_TANGENT_SV 2
