@@ -258,8 +258,8 @@ def _gt_state(pb, seed):
 def test_tangent_forms(oracle, form, case):
     """Both element-tangent kernels (T strip; strain variation, chunks of 1,
     2 or 4 Gauss points with ragged last chunks) against the oracle, FP64 and
-    FP32 (FP64 and FP32 values), ge and getime, Gauss orders 1..4, with and
-    without the periodic F0."""
+    FP32 (FP64 and FP32 values, and FP64 into FP32 values), ge and getime,
+    Gauss orders 1..4, with and without the periodic F0."""
     say("tangent form", form, case)
     phys, dims, nq = case.split("_")[0], case.split("_")[1], case.split("_")[-1]
     if dims == "F0":
@@ -286,3 +286,6 @@ def test_tangent_forms(oracle, form, case):
     assert rel(v32, v0) < TOL32, rel(v32, v0)
     v32f = app.tangent(U, prec=L.IGAP4H_FP32, vals_type=L.IGAP4H_VALS_F32)
     assert v32f.dtype == np.float32 and rel(v32f.astype(np.float64), v0) < TOL32
+    # FP64 contraction into FP32 values (the colours' contributions add in FP32)
+    v64f = app.tangent(U, prec=L.IGAP4H_FP64, vals_type=L.IGAP4H_VALS_F32)
+    assert v64f.dtype == np.float32 and rel(v64f.astype(np.float64), v0) < TOL32
