@@ -57,10 +57,12 @@ FLOP_RE_PER_QP = 1620 + 1620 + 540 + 291
 # weights): ~71 k of the 260,544 algorithmic
 FLOP_RE_EXEC_PER_ELEM = 71000
 PEAK = {"f64": 78.6, "f32": 157.3}   # MI355X dense MFMA TFLOP/s (the vector FP64 / FP32 peaks are the same)
-# MFMA flops the strain-variation tangent executes per Gauss point, 4 waves x
-# 16 x 16 x 4 tiles: ge 5 slots x 3 k-steps (12 strain components per dof);
-# getime 4 material slots x 6 k-steps (24 components) + 1 Gamma slot x 3
-SV_MFMA_FLOP_PER_QP = {"ge": 4 * 5 * 3 * 2 * 16 * 16 * 4, "getime": 4 * (4 * 6 + 3) * 2 * 16 * 16 * 4}
+# MFMA flops the strain-variation tangent executes per Gauss point, in 16 x 16 x 4
+# tiles: ge waves 0-2 5 material slots x 3 k-steps (12 strain components per
+# dof), wave 3 the 3 Gamma tiles x 3 (54 MFMAs); getime waves 0-2 4 material
+# slots x 6 k-steps (24 components) + 1 Gamma slot x 3, wave 3 3 material slots
+# x 6 (99 MFMAs)
+SV_MFMA_FLOP_PER_QP = {"ge": (3 * 5 * 3 + 3 * 3) * 2 * 16 * 16 * 4, "getime": (3 * (4 * 6 + 3) + 3 * 6) * 2 * 16 * 16 * 4}
 PEAK_HBM = 8000.0                    # GB/s
 
 

@@ -1195,8 +1195,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GtWaves<ACC
 // Gauss point in one pass after the point data.  The Gamma operands n(A) and
 // w Gamma n(B) sit in columns 80 + A / 80 + B of the same strips (12 slot
 // rows, the rest of those rows unused), and its 3 tiles of the 27 x 27 block
-// are added to the three diagonal blocks.  5 tile slots per wave, one code
-// path for every wave.
+// are added to the three diagonal blocks.
 constexpr int SV_G = 80;    // first Gamma column of the operand strips
 constexpr int SV_RS = 112;  // k-row stride of the strips: 80 + 32; 112 = 48 mod 64 puts the 4 k-rows of a fragment on distinct banks
 
@@ -1334,51 +1333,80 @@ __device__ __forceinline__ void sv_item_gamma(const SvPoint<ACC, GT> &p, ACC wma
     }
 }
 
-// The wave's 5 tile slots; slot t takes its A fragment from ao[ag(t)] and its
-// B fragment from bo[t] (strip columns; 80, 96: the Gamma block).
-// phys/ge (GT = 0): every slot over the 12 k-rows, ag = (0, 0, 0, 1, 2)
-//   wave 0: (0,0) (0,1) (0,2) (0,3) (0,4)
-//   wave 1: (1,1) (1,2) (1,3) (1,4) (4,4)
-//   wave 2: (2,2) (2,3) (2,4) (3,3) (3,4)
-//   wave 3: (G0,G0) (G0,G1) -  (G1,G1) -
-// phys/getime (GT = 1): 4 material slots over the 24 k-rows and one Gamma slot
-// over the 12 Gamma rows, ag = (0, 1, 1, 1, 2): a wave issues 4 x 6 + 3 = 27
-// MFMAs per point instead of 30
-//   wave 0: (0,0) (0,1) (0,2) (0,3) | (G0,G0)
-//   wave 1: (0,4) (1,1) (1,2) (1,3) | (G0,G1)
-//   wave 2: (1,4) (2,2) (2,3) (2,4) | (G1,G1)
-//   wave 3: (4,4) (3,3) (3,4)   -   |    -
-// (for ge the second assignment measured 1.3 % slower.)  Empty slots read
-// column 0 and are discarded.  (Fragments repeated within a wave are read
-// again: skipping the repeats with wave-uniform branches split the k-step
-// loop and cost 15 %.)
-struct SvTiles {
-    int ao[3], bo[5];
-    unsigned empty, gam;  // bit t: slot t is empty / a Gamma tile
+// Tile paths.  Waves 0-2 build Y and Z in the item phase, wave 3 the Gamma
+// operands, and each runs its own copy of the chunk loop and staging over its
+// tile path PL, with accumulators for its own slots only (no empty slots, no
+// per-wave Gamma masks; fewer registers than one loop for every wave).  Slot t
+// multiplies A fragment ai(t) by B fragment bi(t); the wave's fragment
+// columns are ao[] / bo[] of its SvTiles (strip columns; 80, 96: the Gamma
+// block).  The last NG slots are Gamma tiles, over the 12 Gamma k-rows only,
+// as are the last NGA A and NGB B fragments.  A wave's largest MFMA share is
+// as small as the tiles allow (the item and MFMA phases are separated by
+// barriers, so a chunk lasts as long as the slowest wave's items plus the
+// slowest wave's MFMAs):
+//   ge:     wave 0: (0,0) (0,1) (0,2) (0,3) (0,4)
+//           wave 1: (1,1) (1,2) (1,3) (1,4) (4,4)
+//           wave 2: (2,2) (2,3) (2,4) (3,3) (3,4)
+//           wave 3: (G0,G0) (G0,G1) (G1,G1)               15 MFMAs per point (wave 3: 9)
+//   getime: wave 0: (0,0) (0,1) (0,2) (0,3) | (G0,G0)
+//           wave 1: (0,4) (1,1) (1,2) (1,3) | (G0,G1)
+//           wave 2: (1,4) (2,2) (2,3) (2,4) | (G1,G1)
+//           wave 3: (4,4) (3,3) (3,4)                     4 x 6 + 3 = 27 MFMAs per point (wave 3: 18)
+// (Fragments repeated within a wave are read again: skipping the repeats with
+// wave-uniform branches split the k-step loop and cost 15 %.)
+template <bool W3_, int NS_, int NA_, int NB_, int NG_, int NGA_, int NGB_>
+struct SvPathDims {
+    static constexpr bool W3 = W3_;  // wave 3's path
+    static constexpr int NS = NS_, NA = NA_, NB = NB_, NG = NG_, NGA = NGA_, NGB = NGB_;
 };
 template <int GT>
-__device__ __forceinline__ constexpr int sv_ag(int t) { return GT ? (t == 0 ? 0 : (t < 4 ? 1 : 2)) : (t < 3 ? 0 : (t < 4 ? 1 : 2)); }
+struct SvPathW;  // waves 0-2
+template <int GT>
+struct SvPath3;  // wave 3
+template <>
+struct SvPathW<0> : SvPathDims<false, 5, 3, 5, 0, 0, 0> {
+    __device__ static constexpr int ai(int t) { return t < 3 ? 0 : t - 2; }
+    __device__ static constexpr int bi(int t) { return t; }
+};
+template <>
+struct SvPath3<0> : SvPathDims<true, 3, 2, 2, 3, 2, 2> {
+    __device__ static constexpr int ai(int t) { return t < 2 ? 0 : 1; }
+    __device__ static constexpr int bi(int t) { return t == 0 ? 0 : 1; }
+};
+template <>
+struct SvPathW<1> : SvPathDims<false, 5, 3, 5, 1, 1, 1> {
+    __device__ static constexpr int ai(int t) { return t == 0 ? 0 : (t < 4 ? 1 : 2); }
+    __device__ static constexpr int bi(int t) { return t; }
+};
+template <>
+struct SvPath3<1> : SvPathDims<true, 3, 2, 3, 0, 0, 0> {
+    __device__ static constexpr int ai(int t) { return t == 0 ? 0 : 1; }
+    __device__ static constexpr int bi(int t) { return t; }
+};
+struct SvTiles {
+    int ao[3], bo[5];
+};
 template <int GT>
 __device__ __forceinline__ SvTiles sv_tiles(int wave)
 {
+    constexpr int G0 = SV_G, G1 = SV_G + 16;
     if (GT == 0) {
-        if (wave == 0) return SvTiles{{0, 0, 0}, {0, 16, 32, 48, 64}, 0u, 0u};
-        if (wave == 1) return SvTiles{{16, 16, 64}, {16, 32, 48, 64, 64}, 0u, 0u};
-        if (wave == 2) return SvTiles{{32, 48, 48}, {32, 48, 64, 48, 64}, 0u, 0u};
-        return SvTiles{{SV_G, SV_G + 16, SV_G + 16}, {SV_G, SV_G + 16, 0, SV_G + 16, 0}, (1u << 2) | (1u << 4),
-                       (1u << 0) | (1u << 1) | (1u << 3)};
+        if (wave == 0) return SvTiles{{0, 0, 0}, {0, 16, 32, 48, 64}};
+        if (wave == 1) return SvTiles{{16, 16, 64}, {16, 32, 48, 64, 64}};
+        if (wave == 2) return SvTiles{{32, 48, 48}, {32, 48, 64, 48, 64}};
+        return SvTiles{{G0, G1, 0}, {G0, G1, 0, 0, 0}};
     }
-    if (wave == 0) return SvTiles{{0, 0, SV_G}, {0, 16, 32, 48, SV_G}, 0u, 1u << 4};
-    if (wave == 1) return SvTiles{{0, 16, SV_G}, {64, 16, 32, 48, SV_G + 16}, 0u, 1u << 4};
-    if (wave == 2) return SvTiles{{16, 32, SV_G + 16}, {64, 32, 48, 64, SV_G + 16}, 0u, 1u << 4};
-    return SvTiles{{64, 48, 0}, {64, 48, 64, 0, 0}, (1u << 3) | (1u << 4), 0u};
+    if (wave == 0) return SvTiles{{0, 0, G0}, {0, 16, 32, 48, G0}};
+    if (wave == 1) return SvTiles{{0, 16, G0}, {64, 16, 32, 48, G1}};
+    if (wave == 2) return SvTiles{{16, 32, G1}, {64, 32, 48, 64, G1}};
+    return SvTiles{{64, 48, 0}, {64, 48, 64, 0, 0}};
 }
 
 // the chunk's qcnt points, the NY k-rows of each (y: A strip, z: B strip);
-// GT: slot 4 (Gamma) over the first 12 k-rows only
-template <class ACC, int NY, int GT>
+// Gamma fragments and slots only over the first 12 k-rows
+template <class ACC, int NY, class PL>
 __device__ __forceinline__ void sv_mfma_chunk(const ACC *y, const ACC *z, int qcnt, int lane, const SvTiles &T,
-                                              typename V4<ACC>::t acc[5])
+                                              typename V4<ACC>::t *acc)
 {
     const int off = (lane >> 4) * SV_RS + (lane & 15);
     const ACC *ya = y + off, *zb = z + off;
@@ -1386,17 +1414,17 @@ __device__ __forceinline__ void sv_mfma_chunk(const ACC *y, const ACC *z, int qc
 #pragma unroll
         for (int ks = 0; ks < NY / 4; ks++) {
             const ACC *yk = ya + 4 * ks * SV_RS, *zk = zb + 4 * ks * SV_RS;
-            const bool g4 = !GT || ks < 3;
-            ACC a[3], b[5];
-            a[0] = yk[T.ao[0]];
-            a[1] = yk[T.ao[1]];
-            if (g4) a[2] = yk[T.ao[2]];
+            const bool gk = ks < 3;
+            ACC a[PL::NA], b[PL::NB];
 #pragma unroll
-            for (int t = 0; t < 4; t++) b[t] = zk[T.bo[t]];
-            if (g4) b[4] = zk[T.bo[4]];
+            for (int i = 0; i < PL::NA; i++)
+                if (gk || i < PL::NA - PL::NGA) a[i] = yk[T.ao[i]];
 #pragma unroll
-            for (int t = 0; t < 4; t++) acc[t] = mfma16(a[sv_ag<GT>(t)], b[t], acc[t]);
-            if (g4) acc[4] = mfma16(a[2], b[4], acc[4]);
+            for (int i = 0; i < PL::NB; i++)
+                if (gk || i < PL::NB - PL::NGB) b[i] = zk[T.bo[i]];
+#pragma unroll
+            for (int t = 0; t < PL::NS; t++)
+                if (gk || t < PL::NS - PL::NG) acc[t] = mfma16(a[PL::ai(t)], b[PL::bi(t)], acc[t]);
         }
         ya += NY * SV_RS;
         zb += NY * SV_RS;
@@ -1404,15 +1432,16 @@ __device__ __forceinline__ void sv_mfma_chunk(const ACC *y, const ACC *z, int qc
 }
 
 // the wave's material tiles into the staged 81 x 81 element matrix (both
-// triangles); GAMMA: its Gamma tiles added to the three diagonal blocks (a
-// Gamma tile touches entries no other one does)
-template <class ACC, bool GAMMA, int GT>
-__device__ __forceinline__ void sv_stage(ACC *sKe, int lane, const SvTiles &T, const typename V4<ACC>::t acc[5])
+// triangles); GAMMA: its Gamma tiles added to the three diagonal blocks (the
+// three Gamma tiles touch disjoint entries, which no material tile writes
+// after the barrier between the two calls)
+template <class ACC, bool GAMMA, class PL>
+__device__ __forceinline__ void sv_stage(ACC *sKe, int lane, const SvTiles &T, const typename V4<ACC>::t *acc)
 {
 #pragma unroll
-    for (int t = 0; t < 5; t++) {
-        if (((T.empty >> t) & 1) || (((T.gam >> t) & 1) != 0) != GAMMA) continue;
-        const int ar = T.ao[sv_ag<GT>(t)], bc = T.bo[t];
+    for (int t = 0; t < PL::NS; t++) {
+        if ((t >= PL::NS - PL::NG) != GAMMA) continue;
+        const int ar = T.ao[PL::ai(t)], bc = T.bo[PL::bi(t)];
         const int C = bc + (lane & 15);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
@@ -1592,95 +1621,108 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
     }
     __syncthreads();
 
-    V acc[5];
-#pragma unroll
-    for (int t = 0; t < 5; t++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) acc[t][r] = (ACC)0;
-    double accd[5][4];
-#pragma unroll
-    for (int t = 0; t < 5; t++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) accd[t][r] = 0.0;
     const SvTiles T = sv_tiles<GT>(wave);
     constexpr int NSLOT = (27 * QC + 63) / 64;  // items per lane (waves 0-2)
-    double acc80[NSLOT];
-#pragma unroll
-    for (int k = 0; k < NSLOT; k++) acc80[k] = 0.0;
+    const int nchunk = (nqp + QC - 1) / QC;
 
     // FP32 values: the element's CSR values are read now and arrive during the
     // chunk loop (27 registers); FP64 values are read before the staging
     const ScatterCol sc = scatter_col(tid, sMask);
     float oldf[27];
+    double old[27];
     if (vals_f32) scatter_load<float>(sc, sRowBase, sRowSx, sRowSy, sMask, (const float *)vals, oldf);
+    ACC *sKe = reinterpret_cast<ACC *>(sRegion);
+    double *s80 = reinterpret_cast<double *>(sRegion + D::S80OFF);
 
-    const int nchunk = (nqp + QC - 1) / QC;
-    for (int ch = 0; ch < nchunk; ch++) {
-        const int qcnt = min(QC, nqp - ch * QC);
-        // operands: wave j < 3 builds Y_j, Z_j (lane = (point, B)) and the
-        // dof-80 products; wave 3 the Gamma pair
-        if (wave < 3) {
+    // Waves 0-2 and wave 3 run their own copy of the chunk loop and staging
+    // (PL: their tile path); every wave passes the same barriers.
+    auto part = [&](auto pl) {
+        using PL = decltype(pl);
+        constexpr bool G3 = PL::W3;  // wave 3: the Gamma operands
+        V acc[PL::NS];
 #pragma unroll
-            for (int sl = 0; sl < NSLOT; sl++) {
-                const int t = lane + 64 * sl;
-                if (t < 27 * qcnt) {
-                    const int ql = t / 27, B = t - 27 * ql, q = sQ[ch * QC + ql];
+        for (int t = 0; t < PL::NS; t++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) acc[t][r] = (ACC)0;
+        double accd[PL::NS][4];
+#pragma unroll
+        for (int t = 0; t < PL::NS; t++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) accd[t][r] = 0.0;
+        double acc80[NSLOT];
+#pragma unroll
+        for (int k = 0; k < NSLOT; k++) acc80[k] = 0.0;
+
+        // operands of chunk ch into the strips yb, zb: wave j < 3 builds Y_j,
+        // Z_j (lane = (point, B)) and the dof-80 products; wave 3 the Gamma pair
+        auto items = [&](int ch, ACC *yb, ACC *zb) {
+            const int qcnt = min(QC, nqp - ch * QC);
+            if constexpr (!G3) {
+#pragma unroll
+                for (int sl = 0; sl < NSLOT; sl++) {
+                    const int t = lane + 64 * sl;
+                    if (t < 27 * qcnt) {
+                        const int ql = t / 27, B = t - 27 * ql, q = sQ[ch * QC + ql];
+                        ACC n[10];
+                        basis_slots<ACC>(sphiA, q & 15, (q >> 4) & 15, (q >> 8) & 15, B, n);
+                        // the last dof's column is the first Gamma column: its Y and Z go to
+                        // the last (padding) column, whose products are discarded
+                        const int col = wave == 2 && B == 26 ? SV_RS - 1 : 27 * wave + B;
+                        acc80[sl] += (double)sv_item_yz<ACC, GT>(sPt[ch * QC + ql], ce, cl, n, wave, yb + ql * NY * SV_RS + col,
+                                                                 zb + ql * NY * SV_RS + col, sZ80[ch * QC + ql]);
+                    }
+                }
+            } else {
+                for (int t = lane; t < 27 * qcnt; t += 64) {
+                    const int ql = t / 27, B = t - 27 * ql, qp = ch * QC + ql, q = sQ[qp];
                     ACC n[10];
                     basis_slots<ACC>(sphiA, q & 15, (q >> 4) & 15, (q >> 8) & 15, B, n);
-                    // the last dof's column is the first Gamma column: its Y and Z go to
-                    // the last (padding) column, whose products are discarded
-                    const int col = wave == 2 && B == 26 ? SV_RS - 1 : 27 * wave + B;
-                    acc80[sl] += (double)sv_item_yz<ACC, GT>(sPt[ch * QC + ql], ce, cl, n, wave, sY + ql * NY * SV_RS + col,
-                                                             sZ + ql * NY * SV_RS + col, sZ80[ch * QC + ql]);
+                    const Pt &P = sPt[qp];
+                    sv_item_gamma<ACC, GT>(P, P.w * mass, n, yb + ql * NY * SV_RS + SV_G + B, zb + ql * NY * SV_RS + SV_G + B);
                 }
             }
-        } else {
-            for (int t = lane; t < 27 * qcnt; t += 64) {
-                const int ql = t / 27, B = t - 27 * ql, qp = ch * QC + ql, q = sQ[qp];
-                ACC n[10];
-                basis_slots<ACC>(sphiA, q & 15, (q >> 4) & 15, (q >> 8) & 15, B, n);
-                const Pt &P = sPt[qp];
-                sv_item_gamma<ACC, GT>(P, P.w * mass, n, sY + ql * NY * SV_RS + SV_G + B, sZ + ql * NY * SV_RS + SV_G + B);
+        };
+        auto contract = [&](int ch, const ACC *yb, const ACC *zb) {
+            sv_mfma_chunk<ACC, NY, PL>(yb, zb, min(QC, nqp - ch * QC), lane, T, acc);
+            // FP32: the partial sums of every 16 Gauss points go into FP64
+            if (sizeof(ACC) == 4 && (((ch + 1) * QC) % 16 == 0 || ch == nchunk - 1)) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+#pragma unroll
+                    for (int t = 0; t < PL::NS; t++) {
+                        accd[t][r] += (double)acc[t][r];
+                        acc[t][r] = (ACC)0;
+                    }
+                }
             }
+        };
+        for (int ch = 0; ch < nchunk; ch++) {
+            items(ch, sY, sZ);
+            __syncthreads();
+            contract(ch, sY, sZ);
+            __syncthreads();
         }
-        __syncthreads();
-        sv_mfma_chunk<ACC, NY, GT>(sY, sZ, qcnt, lane, T, acc);
-        // FP32: the partial sums of every 16 Gauss points go into FP64
-        if (sizeof(ACC) == 4 && (((ch + 1) * QC) % 16 == 0 || ch == nchunk - 1)) {
+        if (sizeof(ACC) == 4) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
 #pragma unroll
-                for (int t = 0; t < 5; t++) {
-                    accd[t][r] += (double)acc[t][r];
-                    acc[t][r] = (ACC)0;
-                }
+                for (int t = 0; t < PL::NS; t++) acc[t][r] = (ACC)accd[t][r];
             }
         }
-        __syncthreads();
-    }
-    if (sizeof(ACC) == 4) {
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-#pragma unroll
-            for (int t = 0; t < 5; t++) acc[t][r] = (ACC)accd[t][r];
-        }
-    }
+        if (!vals_f32) scatter_load<double>(sc, sRowBase, sRowSx, sRowSy, sMask, (const double *)vals, old);
 
-    // ---- element matrix: material tiles (both triangles) and the dof-80
-    // row and column, then Gamma on the diagonal blocks
-    ACC *sKe = reinterpret_cast<ACC *>(sRegion);
-    double *s80 = reinterpret_cast<double *>(sRegion + D::S80OFF);
-    auto stage = [&]() {
-        if (wave < 3) {
+        // ---- element matrix: material tiles (both triangles) and the dof-80
+        // row and column, then Gamma on the diagonal blocks
+        if constexpr (!G3) {
 #pragma unroll
             for (int sl = 0; sl < NSLOT; sl++) {
                 const int t = lane + 64 * sl;
                 if (t < 27 * QC) s80[wave * 27 * QC + t] = acc80[sl];
             }
         }
-        sv_stage<ACC, false, GT>(sKe, lane, T, acc);
+        sv_stage<ACC, false, PL>(sKe, lane, T, acc);
         __syncthreads();
-        if (tid < 81) {
+        if (!G3 && tid < 81) {
             const int j = tid / 27, B = tid - 27 * j;
             double k80 = 0.0;
             for (int ql = 0; ql < QC; ql++) k80 += s80[j * 27 * QC + ql * 27 + B];
@@ -1688,18 +1730,18 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
             sKe[80 * 81 + 3 * B + j] = (ACC)k80;
         }
         __syncthreads();
-        sv_stage<ACC, true, GT>(sKe, lane, T, acc);
+        if constexpr (PL::NG > 0) sv_stage<ACC, true, PL>(sKe, lane, T, acc);
         __syncthreads();
     };
-    if (vals_f32) {
-        stage();
+    if (wave < 3)
+        part(SvPathW<GT>{});
+    else
+        part(SvPath3<GT>{});
+
+    if (vals_f32)
         scatter_store<float, ACC>(sc, sKe, sRowBase, sRowSx, sRowSy, sMask, (float *)vals, oldf);
-    } else {
-        double old[27];
-        scatter_load<double>(sc, sRowBase, sRowSx, sRowSy, sMask, (const double *)vals, old);
-        stage();
+    else
         scatter_store<double, ACC>(sc, sKe, sRowBase, sRowSx, sRowSy, sMask, (double *)vals, old);
-    }
 }
 
 // two work-groups per CU (one for getime FP64: 146 KB of LDS with 2 points per chunk)
