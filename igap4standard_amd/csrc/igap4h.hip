@@ -1498,6 +1498,12 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
     __shared__ __attribute__((aligned(16))) unsigned char sRegion[D::REGION];
     __shared__ ACC sphiA[3 * 4 * 9];
     __shared__ ACC sZ80[NQM * NQM * NQM][NY];  // Z of dof 80 per Gauss point
+    // FP32: the slot values n(B) of the chunk's (point, B) items, built by
+    // wave 3 for the next chunk after its MFMAs (it has the fewest) and read by
+    // every item lane (FP64 has no room left for them at two work-groups per CU)
+    constexpr bool NPRE = sizeof(ACC) == 4;
+    constexpr int NW = (27 * QC + 63) / 64 * 64;  // item columns, one per lane
+    __shared__ ACC sN[NPRE ? 10 : 1][NPRE ? NW : 1];
     __shared__ int sQ[64];
     __shared__ long long sRowBase[81];
     __shared__ int sRowSx[81], sRowSy[81];
@@ -1610,7 +1616,27 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
     }
     __syncthreads();
     const ACC ce = (ACC)(2.0 * c.e345), cl = (ACC)(2.0 * c.lsq), mass = (ACC)mass_d;
-    // Z(80) of every Gauss point, one lane per point
+    // n(B) of the items of chunk ch into sN (wave 3)
+    auto fill_n = [&](int ch) {
+        for (int t = lane; t < 27 * min(QC, nqp - ch * QC); t += 64) {
+            const int ql = t / 27, B = t - 27 * ql, q = sQ[ch * QC + ql];
+            ACC n[10];
+            basis_slots<ACC>(sphiA, q & 15, (q >> 4) & 15, (q >> 8) & 15, B, n);
+#pragma unroll
+            for (int k = 0; k < 10; k++) sN[k][t] = n[k];
+        }
+    };
+    // n(B) of item t of chunk ch
+    auto slots = [&](int ch, int t, int B, ACC n[10]) {
+        if constexpr (NPRE) {
+#pragma unroll
+            for (int k = 0; k < 10; k++) n[k] = sN[k][t];
+        } else {
+            const int q = sQ[ch * QC + t / 27];
+            basis_slots<ACC>(sphiA, q & 15, (q >> 4) & 15, (q >> 8) & 15, B, n);
+        }
+    };
+    // Z(80) of every Gauss point, one lane per point; the first chunk's n(B)
     if (tid < nqp) {
         const int q = sQ[tid];
         ACC n[10], Y[NY], Z[NY];
@@ -1618,6 +1644,8 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
         sv_yz<ACC>(sPt[tid], ce, cl, n, 2, Y, Z);
 #pragma unroll
         for (int k = 0; k < NY; k++) sZ80[tid][k] = Z[k];
+    } else if (NPRE && wave == 3) {
+        fill_n(0);
     }
     __syncthreads();
 
@@ -1662,9 +1690,9 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
                 for (int sl = 0; sl < NSLOT; sl++) {
                     const int t = lane + 64 * sl;
                     if (t < 27 * qcnt) {
-                        const int ql = t / 27, B = t - 27 * ql, q = sQ[ch * QC + ql];
+                        const int ql = t / 27, B = t - 27 * ql;
                         ACC n[10];
-                        basis_slots<ACC>(sphiA, q & 15, (q >> 4) & 15, (q >> 8) & 15, B, n);
+                        slots(ch, t, B, n);
                         // the last dof's column is the first Gamma column: its Y and Z go to
                         // the last (padding) column, whose products are discarded
                         const int col = wave == 2 && B == 26 ? SV_RS - 1 : 27 * wave + B;
@@ -1674,9 +1702,9 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
                 }
             } else {
                 for (int t = lane; t < 27 * qcnt; t += 64) {
-                    const int ql = t / 27, B = t - 27 * ql, qp = ch * QC + ql, q = sQ[qp];
+                    const int ql = t / 27, B = t - 27 * ql, qp = ch * QC + ql;
                     ACC n[10];
-                    basis_slots<ACC>(sphiA, q & 15, (q >> 4) & 15, (q >> 8) & 15, B, n);
+                    slots(ch, t, B, n);
                     const Pt &P = sPt[qp];
                     sv_item_gamma<ACC, GT>(P, P.w * mass, n, yb + ql * NY * SV_RS + SV_G + B, zb + ql * NY * SV_RS + SV_G + B);
                 }
@@ -1684,6 +1712,8 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
         };
         auto contract = [&](int ch, const ACC *yb, const ACC *zb) {
             sv_mfma_chunk<ACC, NY, PL>(yb, zb, min(QC, nqp - ch * QC), lane, T, acc);
+            // every item lane has read sN before the barrier ahead of this phase
+            if (NPRE && G3 && ch + 1 < nchunk) fill_n(ch + 1);
             // FP32: the partial sums of every 16 Gauss points go into FP64
             if (sizeof(ACC) == 4 && (((ch + 1) * QC) % 16 == 0 || ch == nchunk - 1)) {
 #pragma unroll
