@@ -587,6 +587,7 @@ __device__ __forceinline__ int block_off(int i) { return i == 0 ? 0 : (i == 1 ? 
 // they may be read any time after the previous colour's launch.
 struct ScatterCol {
     int cc, r0, cx, cy, cz;
+    int kb, ks;  // staged value k of the thread's column: sKe[kb + ks k]
     bool act;
 };
 __device__ __forceinline__ ScatterCol scatter_col(int tid, const unsigned char *sMask)
@@ -600,6 +601,8 @@ __device__ __forceinline__ ScatterCol scatter_col(int tid, const unsigned char *
     s.cy = (B / 3) % 3;
     s.cz = 3 * (B % 3) + s.cc % 3;
     s.act = s.act && !sMask[s.cc];
+    s.kb = s.r0 * 81 + s.cc;  // rows 3 A + i, columns 3 B + j
+    s.ks = 243;
     return s;
 }
 template <class VAL>
@@ -622,7 +625,7 @@ __device__ __forceinline__ void scatter_store(const ScatterCol &s, const ACC *sK
     // wait for the ones before it
     VAL nv[27];
 #pragma unroll
-    for (int k = 0; k < 27; k++) nv[k] = old[k] + (VAL)sKe[(s.r0 + 3 * k) * 81 + s.cc];
+    for (int k = 0; k < 27; k++) nv[k] = old[k] + (VAL)sKe[s.kb + s.ks * k];
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the loads are in, none left between the stores
 #pragma unroll
     for (int k = 0; k < 27; k++) {
@@ -1432,9 +1435,10 @@ __device__ __forceinline__ void sv_mfma_chunk(const ACC *y, const ACC *z, int qc
 }
 
 // the wave's material tiles into the staged 81 x 81 element matrix (both
-// triangles); GAMMA: its Gamma tiles added to the three diagonal blocks (the
-// three Gamma tiles touch disjoint entries, which no material tile writes
-// after the barrier between the two calls)
+// triangles), stored in the tiles' own order: row 27 i + A, column 27 j + B
+// (the scatter reads it in that order); GAMMA: its Gamma tiles added to the
+// three diagonal blocks (the three Gamma tiles touch disjoint entries, which
+// no material tile writes after the barrier between the two calls)
 template <class ACC, bool GAMMA, class PL>
 __device__ __forceinline__ void sv_stage(ACC *sKe, int lane, const SvTiles &T, const typename V4<ACC>::t *acc)
 {
@@ -1443,23 +1447,26 @@ __device__ __forceinline__ void sv_stage(ACC *sKe, int lane, const SvTiles &T, c
         if ((t >= PL::NS - PL::NG) != GAMMA) continue;
         const int ar = T.ao[PL::ai(t)], bc = T.bo[PL::bi(t)];
         const int C = bc + (lane & 15);
+        if (GAMMA) {
+            const int B = C - SV_G;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int R = ar + acc_row((ACC)0, lane, r);
-            if (GAMMA) {
-                const int A = R - SV_G, B = C - SV_G;
+            for (int r = 0; r < 4; r++) {
+                const int A = ar + acc_row((ACC)0, lane, r) - SV_G;
                 if (A < 27 && B < 27) {
 #pragma unroll
                     for (int d = 0; d < 3; d++) {
-                        sKe[(3 * A + d) * 81 + 3 * B + d] += acc[t][r];
-                        if (ar != bc) sKe[(3 * B + d) * 81 + 3 * A + d] += acc[t][r];
+                        sKe[(27 * d + A) * 81 + 27 * d + B] += acc[t][r];
+                        if (ar != bc) sKe[(27 * d + B) * 81 + 27 * d + A] += acc[t][r];
                     }
                 }
-            } else {
-                const int i = R / 27, A = R - 27 * i, j = C / 27, B = C - 27 * j;
-                sKe[(3 * A + i) * 81 + 3 * B + j] = acc[t][r];
-                if (ar != bc) sKe[(3 * B + j) * 81 + 3 * A + i] = acc[t][r];
             }
+            continue;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int R = ar + acc_row((ACC)0, lane, r);
+            sKe[R * 81 + C] = acc[t][r];
+            if (ar != bc) sKe[C * 81 + R] = acc[t][r];
         }
     }
 }
@@ -1655,7 +1662,9 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
 
     // FP32 values: the element's CSR values are read now and arrive during the
     // chunk loop (27 registers); FP64 values are read before the staging
-    const ScatterCol sc = scatter_col(tid, sMask);
+    ScatterCol sc = scatter_col(tid, sMask);
+    sc.kb = 27 * 81 * sc.r0 + 27 * (sc.cc % 3) + sc.cc / 3;  // row 27 i + A, column 27 j + B
+    sc.ks = 81;
     float oldf[27];
     double old[27];
     if (vals_f32) scatter_load<float>(sc, sRowBase, sRowSx, sRowSy, sMask, (const float *)vals, oldf);
@@ -1752,12 +1761,12 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
         }
         sv_stage<ACC, false, PL>(sKe, lane, T, acc);
         __syncthreads();
-        if (!G3 && tid < 81) {
+        if (!G3 && tid < 81) {  // dof 27 j + B = tid
             const int j = tid / 27, B = tid - 27 * j;
             double k80 = 0.0;
             for (int ql = 0; ql < QC; ql++) k80 += s80[j * 27 * QC + ql * 27 + B];
-            sKe[(3 * B + j) * 81 + 80] = (ACC)k80;
-            sKe[80 * 81 + 3 * B + j] = (ACC)k80;
+            sKe[tid * 81 + 80] = (ACC)k80;
+            sKe[80 * 81 + tid] = (ACC)k80;
         }
         __syncthreads();
         if constexpr (PL::NG > 0) sv_stage<ACC, true, PL>(sKe, lane, T, acc);
