@@ -12,7 +12,9 @@ the oracle.
   semantics and lifted): every step satisfies the oracle's residual.
 - 48^3 getime, 100 steps of dt = 1e-3 from a Gaussian random velocity field:
   steps 1, 50 and 100 satisfy the oracle's getime residual with the same
-  history; the FP32-tangent trajectory matches the FP64 one.
+  history; the FP32-tangent trajectory matches the FP64 one; the tangent at a
+  state with three distinct history slices on the complete rows of three
+  6^3-element windows within 1e-12 (FP64) / 1e-5 (FP32) of the oracle.
 - 256 batched instances {8..24}^3 (seed 0x5EED0004): every instance bit for
   bit equal to the single-instance entry points; the 8^3 instances' R and K
   within 1e-12 of the oracle.
@@ -96,17 +98,14 @@ def window_dofs(pb, ranges):
     return (3 * node[:, None] + np.arange(3)[None, :]).reshape(-1)
 
 
-@pytest.fixture(scope="module")
-def windows64(cube64, oracle):
-    """Oracle blocks of the three windows with MatZeroRowsColumns(diag 1)."""
-    pb, U, _, _, _ = cube64
-    say("64^3: oracle tangent windows")
+def oracle_windows(pb, U, oracle, windows):
+    """Oracle blocks of the element windows with MatZeroRowsColumns(diag 1)."""
     drows = np.zeros(pb.ndof, bool)
     drows[oracle.dirichlet_rows(pb)] = True
-    with concurrent.futures.ThreadPoolExecutor(len(WINDOWS)) as ex:
-        blocks = list(ex.map(lambda w: oracle.tangent_window(pb, U, w[0], w[1]), WINDOWS))
+    with concurrent.futures.ThreadPoolExecutor(len(windows)) as ex:
+        blocks = list(ex.map(lambda w: oracle.tangent_window(pb, U, w[0], w[1]), windows))
     out = []
-    for (lo, hi), W in zip(WINDOWS, blocks):
+    for (lo, hi), W in zip(windows, blocks):
         rows, cols = oracle.window_nodes(pb, lo, hi)
         ri, ci = window_dofs(pb, rows), window_dofs(pb, cols)
         W[drows[ri], :] = 0.0
@@ -116,6 +115,32 @@ def windows64(cube64, oracle):
                 W[k, np.searchsorted(ci, r)] = 1.0
         out.append((lo, hi, ri, ci, W))
     return out
+
+
+def check_windows(pb, vals, rowptr, col, windows, tol):
+    """The device tangent's complete rows of each window against the oracle block."""
+    lut = np.full(pb.ndof, -1, np.int64)
+    for lo, hi, ri, ci, W in windows:
+        lut[ci] = np.arange(len(ci))
+        start, stop = rowptr[ri], rowptr[ri + 1]
+        cnt = stop - start
+        idx = np.concatenate([np.arange(a, b) for a, b in zip(start, stop)])
+        rloc = np.repeat(np.arange(len(ri)), cnt)
+        cloc = lut[col[idx]]
+        assert np.all(cloc >= 0), "a complete row has a column outside its window"
+        G = np.zeros_like(W)
+        G[rloc, cloc] = vals[idx]
+        e = rel(G, W)
+        say("  window", lo, hi, "rows", len(ri), "|dK|/|K| = %.2e" % e)
+        assert e < tol
+        lut[ci] = -1
+
+
+@pytest.fixture(scope="module")
+def windows64(cube64, oracle):
+    pb, U, _, _, _ = cube64
+    say("64^3: oracle tangent windows")
+    return oracle_windows(pb, U, oracle, WINDOWS)
 
 
 @pytest.mark.parametrize("prec,vt,tol", [(L.IGAP4H_FP64, L.IGAP4H_VALS_F64, 1e-12),
@@ -131,21 +156,7 @@ def test_64cube_tangent_windows(cube64, windows64, prec, vt, tol):
     torch.cuda.synchronize()
     vals = dK.cpu().numpy().astype(np.float64)
     del dK
-    lut = np.full(pb.ndof, -1, np.int64)
-    for lo, hi, ri, ci, W in windows64:
-        lut[ci] = np.arange(len(ci))
-        start, stop = rowptr[ri], rowptr[ri + 1]
-        cnt = stop - start
-        idx = np.concatenate([np.arange(a, b) for a, b in zip(start, stop)])
-        rloc = np.repeat(np.arange(len(ri)), cnt)
-        cloc = lut[col[idx]]
-        assert np.all(cloc >= 0), "a complete row has a column outside its window"
-        G = np.zeros_like(W)
-        G[rloc, cloc] = vals[idx]
-        e = rel(G, W)
-        say("  window", lo, hi, "rows", len(ri), "|dK|/|K| = %.2e" % e)
-        assert e < tol
-        lut[ci] = -1
+    check_windows(pb, vals, rowptr, col, windows64, tol)
 
 
 _SOLUTIONS64 = {}
@@ -263,6 +274,45 @@ def test_48cube_getime_100_steps(oracle):
     e = rel(U32, U64)
     say("  |U32 - U64| / |U64| = %.2e, |U| = %.3e" % (e, np.linalg.norm(U64)))
     assert np.linalg.norm(U64) > 0 and e < 1e-8
+
+
+@pytest.fixture(scope="module")
+def getime48_windows(oracle):
+    """48^3 getime at a state with three distinct history slices (the jets
+    along u(s) non-trivial): the context, its pattern and the oracle blocks of
+    three 6^3-element windows (interior, a clamped corner, a clamped face)."""
+    pb = P.transient_problem(48, dt=1e-3)
+    rng = np.random.default_rng(0x5EED0013)
+    n = pb.ndof
+    U0 = P.smooth_state(pb, amp=1e-2, noise=1e-4, nhist=1)
+    U1 = U0 - 1e-3 * P.initial_velocity(pb) + rng.uniform(-1, 1, n) * 1e-4
+    U = U1 + rng.uniform(-1, 1, n) * 1e-4
+    Uh = np.concatenate([U, U1, U0])
+    say("48^3 getime: oracle tangent windows")
+    windows = oracle_windows(pb, Uh, oracle, [((21, 21, 21), (27, 27, 27)), ((0, 0, 0), (6, 6, 6)),
+                                               ((42, 21, 21), (48, 27, 27))])
+    app = App(pb)
+    rowptr, col = app.pattern()
+    yield pb, Uh, app, rowptr, col, windows
+    app.close()
+
+
+@pytest.mark.parametrize("prec,vt,tol", [(L.IGAP4H_FP64, L.IGAP4H_VALS_F64, 1e-12),
+                                         (L.IGAP4H_FP32, L.IGAP4H_VALS_F32, 1e-5),
+                                         (L.IGAP4H_FP32, L.IGAP4H_VALS_F64, 1e-5)])
+def test_48cube_getime_tangent_windows(getime48_windows, prec, vt, tol):
+    """The getime tangent at the time-stepping workload's size on the complete
+    rows of the windows, against the oracle's element sums."""
+    import torch
+    pb, Uh, app, rowptr, col, windows = getime48_windows
+    say("48^3 getime: tangent", prec, vt)
+    dU = torch.tensor(Uh, device="cuda:0")
+    dK = torch.empty(app.nnz, device="cuda:0", dtype=torch.float64 if vt == L.IGAP4H_VALS_F64 else torch.float32)
+    app.tangent_dev(dU.data_ptr(), dK.data_ptr(), prec, vt)
+    torch.cuda.synchronize()
+    vals = dK.cpu().numpy().astype(np.float64)
+    del dK
+    check_windows(pb, vals, rowptr, col, windows, tol)
 
 
 # ------------------------------------------------------------------ 256 instances
