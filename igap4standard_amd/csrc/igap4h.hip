@@ -1212,10 +1212,15 @@ struct SvPoint<ACC, 0> {
     ACC tau[3][3];  // w tau
     ACC w;
 };
+// getime works in the basis (u, v) = (Y0 + a Y1, Y1), a = 2/3, of the jet pair:
+// the averaged form's 2 x 2 factor [[1/4, 1/6], [1/6, 1/8]] on the shear and
+// trace-free components becomes diag(1/4, 1/72), so Z of those components is one
+// product each.  F, H hold F0 + a F1, H0 + a H1 (u = Y(F, H) by linearity).
+constexpr double SV_UV = 2.0 / 3.0;
 template <class ACC>
 struct SvPoint<ACC, 1> {
     ACC F[3][3], H[3][6], F1[3][3], H1[3][6];
-    ACC P00[6], P01[6], P11[6];  // w M_ab on the normal strains
+    ACC P00[6], P01[6], P11[6];  // w M_ab on the normal strains, in the (u, v) basis
     ACC S[6], tau[3][3];          // w x averaged series
     ACC w;
 };
@@ -1285,9 +1290,8 @@ __device__ __forceinline__ void sv_yz(const SvPoint<ACC, 1> &p, ACC ce, ACC cl, 
 #pragma unroll
     for (int k = 3; k < 12; k++) {
         const ACC cw = p.w * (k < 6 ? ce : cl) * ACC(sv_s2(k));
-        const ACC a00 = cw * ACC(0.25), a01 = cw * ACC(1.0 / 6.0), a11 = cw * ACC(0.125);
-        Z[k] = a00 * Y[k] + a01 * Y[12 + k];
-        Z[12 + k] = a01 * Y[k] + a11 * Y[12 + k];
+        Z[k] = (cw * ACC(0.25)) * Y[k];
+        Z[12 + k] = (cw * ACC(1.0 / 72.0)) * Y[12 + k];
     }
 }
 
@@ -1584,20 +1588,23 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
             SvPoint<ACC, 1> &G = reinterpret_cast<SvPoint<ACC, 1> &>(P);
             for (int a = 0; a < 3; a++) {
                 for (int J = 0; J < 3; J++) {
-                    G.F[a][J] = (ACC)cp.F[a][J].c[0];
+                    G.F[a][J] = (ACC)(cp.F[a][J].c[0] + SV_UV * cp.F[a][J].c[1]);
                     G.F1[a][J] = (ACC)cp.F[a][J].c[1];
                     G.tau[a][J] = (ACC)(w * (cp.tau[a][J].c[0] / 4.0 + cp.tau[a][J].c[1] / 6.0 + cp.tau[a][J].c[2] / 8.0));
                 }
                 for (int h = 0; h < 6; h++) {
-                    G.H[a][h] = (ACC)cp.H[a][h].c[0];
+                    G.H[a][h] = (ACC)(cp.H[a][h].c[0] + SV_UV * cp.H[a][h].c[1]);
                     G.H1[a][h] = (ACC)cp.H[a][h].c[1];
                 }
             }
             for (int k = 0; k < 6; k++) {
                 const double *pc = cp.P[k].c, *sc = cp.S[k].c;
-                G.P00[k] = (ACC)(w * (pc[0] / 4.0 + pc[1] / 6.0 + pc[2] / 8.0));
-                G.P01[k] = (ACC)(w * (pc[0] / 6.0 + pc[1] / 8.0));
-                G.P11[k] = (ACC)(w * (pc[0] / 8.0));
+                // the block form in the basis (u, v) = (Y0 + a Y1, Y1), a = SV_UV:
+                // M00, M01 - a M00, M11 - 2 a M01 + a^2 M00
+                const double m00 = pc[0] / 4.0 + pc[1] / 6.0 + pc[2] / 8.0, m01 = pc[0] / 6.0 + pc[1] / 8.0, m11 = pc[0] / 8.0;
+                G.P00[k] = (ACC)(w * m00);
+                G.P01[k] = (ACC)(w * (m01 - SV_UV * m00));
+                G.P11[k] = (ACC)(w * (m11 - 2.0 * SV_UV * m01 + SV_UV * SV_UV * m00));
                 G.S[k] = (ACC)(w * (sc[0] / 4.0 + sc[1] / 6.0 + sc[2] / 8.0));
             }
             G.w = (ACC)w;
