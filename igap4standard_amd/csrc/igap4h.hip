@@ -1211,6 +1211,7 @@ struct SvPoint<ACC, 0> {
     ACC S[6];       // w S   (S00 S11 S22 S01 S02 S12)
     ACC tau[3][3];  // w tau
     ACC w;
+    ACC zc[3];      // Z = zc Y on the shear, even and odd trace-free components: w 2 e345 s^2, w 2 le^2 s^2
 };
 // getime works in the basis (u, v) = (Y0 + a Y1, Y1), a = 2/3, of the jet pair:
 // the averaged form's 2 x 2 factor [[1/4, 1/6], [1/6, 1/8]] on the shear and
@@ -1221,6 +1222,7 @@ template <class ACC>
 struct SvPoint<ACC, 1> {
     ACC F[3][3], H[3][6], F1[3][3], H1[3][6];
     ACC P00[6], P01[6], P11[6];  // w M_ab on the normal strains, in the (u, v) basis
+    ACC zc[6];                   // Z = zc Y on the shear, even and odd trace-free components, u and v
     ACC S[6], tau[3][3];          // w x averaged series
     ACC w;
 };
@@ -1266,20 +1268,17 @@ __device__ __forceinline__ void sym3_apply_add(const R *P, const R *y, R *z)
     z[2] += P[2] * y[0] + P[4] * y[1] + P[5] * y[2];
 }
 
-// Y_j(B) and Z = w M Y_j(B) of one (point, B, j) in registers; ce = 2 e345, cl = 2 le^2
+// Y_j(B) and Z = w M Y_j(B) of one (point, B, j) in registers
 template <class ACC>
-__device__ __forceinline__ void sv_yz(const SvPoint<ACC, 0> &p, ACC ce, ACC cl, const ACC n[10], int j, ACC Y[12], ACC Z[12])
+__device__ __forceinline__ void sv_yz(const SvPoint<ACC, 0> &p, const ACC n[10], int j, ACC Y[12], ACC Z[12])
 {
     sv_y<ACC>(p.F[j], p.H[j], n, Y);
-    const ACC we = p.w * ce * ACC(SV_SH), wl2 = p.w * cl * ACC(SV_T2), wl6 = p.w * cl * ACC(SV_T6);
     sym3_apply(p.P, Y, Z);
 #pragma unroll
-    for (int k = 3; k < 6; k++) Z[k] = we * Y[k];
-#pragma unroll
-    for (int k = 6; k < 12; k++) Z[k] = ((k & 1) ? wl6 : wl2) * Y[k];
+    for (int k = 3; k < 12; k++) Z[k] = p.zc[k < 6 ? 0 : 1 + (k & 1)] * Y[k];
 }
 template <class ACC>
-__device__ __forceinline__ void sv_yz(const SvPoint<ACC, 1> &p, ACC ce, ACC cl, const ACC n[10], int j, ACC Y[24], ACC Z[24])
+__device__ __forceinline__ void sv_yz(const SvPoint<ACC, 1> &p, const ACC n[10], int j, ACC Y[24], ACC Z[24])
 {
     sv_y<ACC>(p.F[j], p.H[j], n, Y);
     sv_y<ACC>(p.F1[j], p.H1[j], n, Y + 12);
@@ -1289,21 +1288,21 @@ __device__ __forceinline__ void sv_yz(const SvPoint<ACC, 1> &p, ACC ce, ACC cl, 
     sym3_apply_add(p.P11, Y + 12, Z + 12);
 #pragma unroll
     for (int k = 3; k < 12; k++) {
-        const ACC cw = p.w * (k < 6 ? ce : cl) * ACC(sv_s2(k));
-        Z[k] = (cw * ACC(0.25)) * Y[k];
-        Z[12 + k] = (cw * ACC(1.0 / 72.0)) * Y[12 + k];
+        const int c = k < 6 ? 0 : 1 + (k & 1);
+        Z[k] = p.zc[2 * c] * Y[k];
+        Z[12 + k] = p.zc[2 * c + 1] * Y[12 + k];
     }
 }
 
 // Y_j(B), Z_j(B) into k-rows 0..NY-1 of the strips at the column y, z point
 // to; returns Y_j(B) . z80, the dof pair's entry K[(B, j), 80] at this point
 template <class ACC, int GT>
-__device__ __forceinline__ ACC sv_item_yz(const SvPoint<ACC, GT> &p, ACC ce, ACC cl, const ACC n[10], int j, ACC *y, ACC *z,
+__device__ __forceinline__ ACC sv_item_yz(const SvPoint<ACC, GT> &p, const ACC n[10], int j, ACC *y, ACC *z,
                                           const ACC *z80)
 {
     constexpr int NY = GT ? 24 : 12;
     ACC Y[NY], Z[NY];
-    sv_yz<ACC>(p, ce, cl, n, j, Y, Z);
+    sv_yz<ACC>(p, n, j, Y, Z);
     ACC d = ACC(0);
 #pragma unroll
     for (int k = 0; k < NY; k++) {
@@ -1608,6 +1607,13 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
                 G.S[k] = (ACC)(w * (sc[0] / 4.0 + sc[1] / 6.0 + sc[2] / 8.0));
             }
             G.w = (ACC)w;
+            // shear (2 e345, 1/4), even (2 le^2, 1/2) and odd (2 le^2, 1/6) trace-free
+            // components; u: x 1/4, v: x 1/72
+            for (int cc = 0; cc < 3; cc++) {
+                const double cw = w * (cc ? 2.0 * c.lsq : 2.0 * c.e345) * sv_s2(cc == 0 ? 3 : 5 + cc);
+                G.zc[2 * cc] = (ACC)(cw / 4.0);
+                G.zc[2 * cc + 1] = (ACC)(cw / 72.0);
+            }
         } else {
             Compact<double> cp;
             compact_of<double>(ub, c, cp);
@@ -1624,12 +1630,13 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
                 G.S[k] = (ACC)(w * cp.S[k]);
             }
             G.w = (ACC)w;
+            for (int cc = 0; cc < 3; cc++) G.zc[cc] = (ACC)(w * (cc ? 2.0 * c.lsq : 2.0 * c.e345) * sv_s2(cc == 0 ? 3 : 5 + cc));
         }
     } else if (tid >= 128 && tid < 128 + 3 * 4 * 9) {
         sphiA[tid - 128] = (ACC)sphi[tid - 128];
     }
     __syncthreads();
-    const ACC ce = (ACC)(2.0 * c.e345), cl = (ACC)(2.0 * c.lsq), mass = (ACC)mass_d;
+    const ACC mass = (ACC)mass_d;
     // n(B) of the items of chunk ch into sN (wave 3)
     auto fill_n = [&](int ch) {
         for (int t = lane; t < 27 * min(QC, nqp - ch * QC); t += 64) {
@@ -1655,7 +1662,7 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
         const int q = sQ[tid];
         ACC n[10], Y[NY], Z[NY];
         basis_slots<ACC>(sphiA, q & 15, (q >> 4) & 15, (q >> 8) & 15, 26, n);
-        sv_yz<ACC>(sPt[tid], ce, cl, n, 2, Y, Z);
+        sv_yz<ACC>(sPt[tid], n, 2, Y, Z);
 #pragma unroll
         for (int k = 0; k < NY; k++) sZ80[tid][k] = Z[k];
     } else if (NPRE && wave == 3) {
@@ -1712,7 +1719,7 @@ __device__ __forceinline__ void tangent_sv_body(const MeshArgs &m, const double 
                         // the last dof's column is the first Gamma column: its Y and Z go to
                         // the last (padding) column, whose products are discarded
                         const int col = wave == 2 && B == 26 ? SV_RS - 1 : 27 * wave + B;
-                        acc80[sl] += (double)sv_item_yz<ACC, GT>(sPt[ch * QC + ql], ce, cl, n, wave, yb + ql * NY * SV_RS + col,
+                        acc80[sl] += (double)sv_item_yz<ACC, GT>(sPt[ch * QC + ql], n, wave, yb + ql * NY * SV_RS + col,
                                                                  zb + ql * NY * SV_RS + col, sZ80[ch * QC + ql]);
                     }
                 }
